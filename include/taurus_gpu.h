@@ -1,0 +1,169 @@
+/*
+ * taurus_gpu.h -- C ABI of the MI355X (gfx950) PNVAP projected-gradient library.
+ *
+ * Drop-in boundary for the hot path of TAURUS_vap: the gauge-angle loop of
+ * module_projection, the two-body contraction of module_fields and the gradient
+ * step of module_gradient.  Every entry point is a plain extern "C" function on
+ * host pointers and sizes, so the reference's Fortran host binds it with
+ * iso_c_binding (bind(C) interfaces, `value` for scalars).
+ *
+ * Conventions (those of the reference's module arrays):
+ *   - matrices are column-major (Fortran order), leading dimension ndim unless
+ *     an lda argument is given;
+ *   - complex(r64) arrays are interleaved (re, im) doubles, i.e. `double _Complex`
+ *     / complex(c_double_complex);
+ *   - single-particle indices in abcd/sp_tr are 1-based, as stored by the
+ *     reference;
+ *   - return value: 0 on success, > 0 a LAPACK-style info, < 0 an error
+ *     (TG_E_*); tg_last_error() gives a message.
+ * The library owns its device buffers; no host pointer is retained after a
+ * call returns.  A context is not re-entrant: call it from one host thread.
+ */
+#ifndef TAURUS_GPU_H
+#define TAURUS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TG_OK                0
+#define TG_E_ARG            -1   /* invalid argument */
+#define TG_E_HIP            -2   /* HIP runtime error */
+#define TG_E_NOMEM          -3   /* device allocation failed */
+#define TG_E_NODEVICE       -4   /* no HIP device */
+#define TG_E_IMAG_GRADIENT  -5   /* calculate_projected_H20: Im(gradient) > 1e-8 */
+#define TG_E_SINGULAR       -6   /* singular overlap matrix (zgetrf info /= 0) */
+
+/* Context flags */
+#define TG_FLAG_PLAIN        1   /* one gauge angle per launch sequence, no batching */
+#define TG_FLAG_TBME_FP32    2   /* keep TBMEs in FP32 on the device (FP64 accumulation) */
+
+typedef struct tg_ctx tg_ctx;
+
+/* Message of the last failing call on this thread. */
+const char* tg_last_error(void);
+
+/*
+ * tg_init -- basis + Hamiltonian upload, once per run.
+ * Replaces the storage set up by set_hamiltonian / read_hamiltonian_reduced
+ * (src/module_hamiltonian.f90:79-106, 533-600: hamil_H0, hamil_H1, hamil_H2dim,
+ * hamil_abcd, hamil_H2, hamil_trperm, :53-58) and the basis arrays
+ * HOsp_2mj, HOsp_2mt, HOsp_l, HOsp_tr (src/module_basis.f90:33-41, 209-238).
+ *   H1      complex ndim x ndim
+ *   abcd    int16[4*ntb] (1-based a,b,c,d per stored element)
+ *   H2      ntb values, float (h2_bytes = 4, rH2 = real32) or double (8)
+ *   trperm  int8[ntb] time-reversal permutation codes
+ * flags: TG_FLAG_* .
+ */
+int tg_init(tg_ctx** ctx, int ndim, const int32_t* sp_2mj, const int32_t* sp_2mt,
+            const int32_t* sp_l, const int32_t* sp_tr, double H0_re, double H0_im,
+            const double* H1, int64_t ntb, const int16_t* abcd, const void* H2,
+            int h2_bytes, const int8_t* trperm, int flags);
+
+/* Frees the context and its device memory. */
+int tg_finalize(tg_ctx* ctx);
+
+/* Switches between batched (flags = 0) and plain per-angle (TG_FLAG_PLAIN) execution. */
+int tg_set_mode(tg_ctx* ctx, int plain);
+
+/*
+ * tg_calculate_fields -- calculate_fields(rhoLR,kappaLR,kappaRL,gammaLR,hspLR,
+ * deltaLR,deltaRL,ndim), src/module_fields.f90:99-189.  All complex ndim x ndim.
+ */
+int tg_calculate_fields(tg_ctx* ctx, const double* rhoLR, const double* kappaLR,
+                        const double* kappaRL, double* gammaLR, double* hspLR,
+                        double* deltaLR, double* deltaRL);
+
+/*
+ * tg_calculate_fields_diag -- calculate_fields_diag(rhoLR,kappaLR,gammaLR,hspLR,
+ * deltaLR,deltaRL,ndim), src/module_fields.f90:204-317 (deltaRL may be NULL,
+ * the Fortran optional argument).
+ */
+int tg_calculate_fields_diag(tg_ctx* ctx, const double* rhoLR, const double* kappaLR,
+                             double* gammaLR, double* hspLR, double* deltaLR,
+                             double* deltaRL);
+
+/*
+ * tg_zskpfa -- ZSKPFA('U','P',n,A,lda,pfaff,...) of src/subroutines_pfaffian.f:12-256
+ * (Parlett-Reid, upper triangle read, INFO > 0 from a zero pivot column gives
+ * pfaff = 0 and info = 0).  `count` matrices laid out at A + k*lda*n (complex).
+ * pfaff: complex[count]; info: int[count].  A is not modified.
+ */
+int tg_zskpfa(int n, int count, const double* A, int lda, double* pfaff, int* info);
+
+/*
+ * tg_calculate_overlap -- calculate_overlap(nocc0,nemp0,ovac0,voveru0,D0,nocc1,
+ * nemp1,ovac1,voveru1,D1,ROT,overlap,ndim), src/module_projection.f90:782-937.
+ * D0, D1, ROT complex ndim x ndim; voveru0 real[ndim-nocc0], voveru1 real[ndim-nocc1].
+ * Returns TG_E_SINGULAR where the reference stops on zgetrf/zgetri info /= 0.
+ */
+int tg_calculate_overlap(int ndim, int nocc0, int nemp0, double ovac0, const double* voveru0,
+                         const double* D0, int nocc1, int nemp1, double ovac1,
+                         const double* voveru1, const double* D1, const double* ROT,
+                         double* overlap);
+
+/*
+ * tg_project_wavefunction -- project_wavefunction(0,iprint,Mphip,Mphin,ndim)
+ * followed by calculate_projected_H20(ndim), src/module_projection.f90:201-373,
+ * 729-759, for the real state (U0, V0) (bogo_U0/bogo_V0).
+ * The canonical basis of the state is an input, as built once per iteration by
+ * construct_canonical_basis (src/module_wavefunctions.f90:777-945, called at
+ * module_projection.f90:219-234): D0 real ndim x ndim (bogo_zD0), nocc0, nemp0,
+ * ovac0 and voveru0 real[ndim-nocc0].
+ * facpi = proj_facpi; valence_Z/N; factor_delta (module_fields.f90:39).
+ * Outputs:
+ *   pnp_scalars  complex[6]: pnp_over, pnp_ener, pnp_prot, pnp_neut, pnp_prot2, pnp_neut2
+ *   pnp_A, pnp_H20, pnp_AH20 complex ndim x ndim (may be NULL)
+ *   field_H20v   real[ndim*ndim]
+ *   imag_grad    (may be NULL) the largest Im part checked by :751-755
+ * Returns TG_E_IMAG_GRADIENT where the reference stops.
+ */
+int tg_project_wavefunction(tg_ctx* ctx, const double* U0, const double* V0,
+                            const double* D0, int nocc0, int nemp0, double ovac0,
+                            const double* voveru0, int Mphip, int Mphin, double facpi,
+                            double valence_Z, double valence_N, double factor_delta,
+                            double* pnp_scalars, double* pnp_A, double* pnp_H20,
+                            double* pnp_AH20, double* field_H20v, double* imag_grad);
+
+/*
+ * tg_project_resident -- same computation on a state already uploaded with
+ * tg_project_wavefunction (or tg_set_state); only the device work runs, results
+ * stay on the device until tg_get_results.  Used for timing the hot path with
+ * inputs resident in HBM.
+ */
+int tg_set_state(tg_ctx* ctx, const double* U0, const double* V0, const double* D0,
+                 int nocc0, int nemp0, double ovac0, const double* voveru0, int Mphip,
+                 int Mphin, double facpi, double valence_Z, double valence_N,
+                 double factor_delta);
+int tg_project_resident(tg_ctx* ctx);
+int tg_sync(tg_ctx* ctx);
+int tg_get_results(tg_ctx* ctx, double* pnp_scalars, double* pnp_A, double* pnp_H20,
+                   double* pnp_AH20, double* field_H20v, double* imag_grad);
+
+/*
+ * tg_calculate_gradient -- calculate_gradient(ndim), src/module_gradient.f90:79-143.
+ * opt_grad selection: gradient_type with the info_H11/info_hsp fall-back (:86-91);
+ * eigen_H11/eigen_hsp real[ndim] (used by gradient_type 1/2, may be NULL for 0);
+ * constraint_20 real[constraint_dim*ndim*ndim], lagrange_lambda1 real[constraint_dim];
+ * gradient_Zim1 in, gradient_Zi out (real[ndim*ndim]); gradient_norm out.
+ */
+int tg_calculate_gradient(int ndim, int gradient_type, double gradient_eta, double gradient_mu,
+                          int info_H11, int info_hsp, const double* eigen_H11,
+                          const double* eigen_hsp, const double* field_H20v,
+                          int constraint_dim, const double* constraint_20,
+                          const double* lagrange_lambda1, const double* gradient_Zim1,
+                          double* gradient_Zi, double* gradient_norm);
+
+/* Device-side timing of the last tg_project_resident (ms), and per-stage kernel
+ * time of the dominant kernels (filled when timing is enabled). */
+int tg_enable_timing(tg_ctx* ctx, int on);
+int tg_get_timing(tg_ctx* ctx, double* ms_total, double* ms_contraction, double* ms_gemm,
+                  double* ms_pfaffian, double* ms_inverse);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TAURUS_GPU_H */

@@ -1,0 +1,255 @@
+// Tiled FP64 GEMM kernels (real / complex operands) for gfx950.
+//
+// One 256-thread workgroup (4 wave64) computes a 64x64 tile of C; each thread
+// owns a 4x4 sub-tile with rows tx+16*i and columns ty+16*j so that the column-
+// major stores of a wave are 16-element contiguous runs.  K advances in chunks
+// of 16 through LDS.  Operands may be real (double), complex (double2) or FP32
+// (float, converted to double when staged): the contraction blocks keep their
+// matrix elements in FP32 while every product and sum is FP64.
+#include "tg_internal.h"
+
+namespace {
+
+constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
+
+template <typename T> struct Stage { typedef T type; };
+template <> struct Stage<float> { typedef double type; };
+
+template <typename T> __device__ inline typename Stage<T>::type stage(T v) { return v; }
+template <> __device__ inline double stage<float>(float v) { return (double)v; }
+
+template <typename T> __device__ inline T zero_of();
+template <> __device__ inline double zero_of<double>() { return 0.0; }
+template <> __device__ inline float zero_of<float>() { return 0.0f; }
+template <> __device__ inline cplx zero_of<cplx>() { return cmk(0.0, 0.0); }
+
+template <typename T> __device__ inline T conj_if(T v, bool c) { return v; }
+template <> __device__ inline cplx conj_if<cplx>(cplx v, bool c) { return c ? cconj(v) : v; }
+
+// acc += a * b for the four type combinations
+__device__ inline void mac(cplx& acc, cplx a, cplx b) { cfma(acc, a, b); }
+__device__ inline void mac(cplx& acc, double a, cplx b) { acc.x = fma(a, b.x, acc.x); acc.y = fma(a, b.y, acc.y); }
+__device__ inline void mac(cplx& acc, cplx a, double b) { acc.x = fma(a.x, b, acc.x); acc.y = fma(a.y, b, acc.y); }
+__device__ inline void mac(double& acc, double a, double b) { acc = fma(a, b, acc); }
+
+template <typename TC> struct Scal;
+template <> struct Scal<cplx> { typedef cplx type; };
+template <> struct Scal<double> { typedef double type; };
+
+__device__ inline cplx epi(cplx alpha, cplx acc, cplx beta, const cplx* cptr, bool use_beta) {
+  cplx r = cmul(alpha, acc);
+  if (use_beta) r = cadd(r, cmul(beta, *cptr));
+  return r;
+}
+__device__ inline double epi(double alpha, double acc, double beta, const double* cptr, bool use_beta) {
+  double r = alpha * acc;
+  if (use_beta) r = fma(beta, *cptr, r);
+  return r;
+}
+
+// One 64x64 tile: C(m0:m0+64, n0:n0+64) of a column-major M x N result.
+template <typename TA, typename TB, typename TC, int OPA, int OPB>
+__device__ inline void gemm_tile(int M, int N, int K, typename Scal<TC>::type alpha,
+                                 const TA* __restrict__ A, int lda,
+                                 const TB* __restrict__ B, int ldb,
+                                 typename Scal<TC>::type beta, bool use_beta,
+                                 TC* __restrict__ C, int ldc, int m0, int n0,
+                                 typename Stage<TA>::type (*As)[BM + 1],
+                                 typename Stage<TB>::type (*Bs)[BN + 1]) {
+  typedef typename Stage<TA>::type SA;
+  typedef typename Stage<TB>::type SB;
+  const int t = threadIdx.x;
+  const int tx = t & 15, ty = t >> 4;
+  TC acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = zero_of<TC>();
+
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    // ---- stage A(m0:+64, k0:+16) into As[k][i]
+    if (OPA == OP_N) {
+      const int i = t & 63;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = (t >> 6) + 4 * r;
+        const int gi = m0 + i, gk = k0 + k;
+        SA v = stage(zero_of<TA>());
+        if (gi < M && gk < K) v = stage(A[gi + (long long)gk * lda]);
+        As[k][i] = v;
+      }
+    } else {
+      const int k = t & 15;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = (t >> 4) + 16 * r;
+        const int gi = m0 + i, gk = k0 + k;
+        SA v = stage(zero_of<TA>());
+        if (gi < M && gk < K) v = conj_if(stage(A[gk + (long long)gi * lda]), OPA == OP_C);
+        As[k][i] = v;
+      }
+    }
+    // ---- stage B(k0:+16, n0:+64) into Bs[k][j]
+    if (OPB == OP_N) {
+      const int k = t & 15;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = (t >> 4) + 16 * r;
+        const int gk = k0 + k, gj = n0 + j;
+        SB v = stage(zero_of<TB>());
+        if (gk < K && gj < N) v = stage(B[gk + (long long)gj * ldb]);
+        Bs[k][j] = v;
+      }
+    } else {
+      const int j = t & 63;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = (t >> 6) + 4 * r;
+        const int gk = k0 + k, gj = n0 + j;
+        SB v = stage(zero_of<TB>());
+        if (gk < K && gj < N) v = conj_if(stage(B[gj + (long long)gk * ldb]), OPB == OP_C);
+        Bs[k][j] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < BK; ++k) {
+      SA a[4];
+      SB b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[k][tx + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[k][ty + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) mac(acc[i][j], a[i], b[j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int gj = n0 + ty + 16 * j;
+    if (gj >= N) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int gi = m0 + tx + 16 * i;
+      if (gi >= M) continue;
+      TC* cp = C + gi + (long long)gj * ldc;
+      *cp = epi(alpha, acc[i][j], beta, cp, use_beta);
+    }
+  }
+}
+
+template <typename TA, typename TB, typename TC, int OPA, int OPB>
+__global__ void __launch_bounds__(NT)
+gemm_batched_kernel(int M, int N, int K, typename Scal<TC>::type alpha,
+                    const TA* __restrict__ A, int lda, long long sA,
+                    const TB* __restrict__ B, int ldb, long long sB,
+                    typename Scal<TC>::type beta, int use_beta,
+                    TC* __restrict__ C, int ldc, long long sC) {
+  __shared__ typename Stage<TA>::type As[BK][BM + 1];
+  __shared__ typename Stage<TB>::type Bs[BK][BN + 1];
+  const long long b = blockIdx.z;
+  gemm_tile<TA, TB, TC, OPA, OPB>(M, N, K, alpha, A + b * sA, lda, B + b * sB, ldb, beta,
+                                  use_beta != 0, C + b * sC, ldc, blockIdx.x * BM,
+                                  blockIdx.y * BN, As, Bs);
+}
+
+template <typename TW>
+__global__ void __launch_bounds__(NT)
+gemm_grouped_kernel(int M, const cplx* __restrict__ X, cplx* __restrict__ C,
+                    const TW* __restrict__ W, const GroupDesc* __restrict__ groups,
+                    const int* __restrict__ tile_group, const int* __restrict__ tile_n0) {
+  __shared__ cplx As[BK][BM + 1];
+  __shared__ double Bs[BK][BN + 1];
+  const int tile = blockIdx.x;
+  const GroupDesc g = groups[tile_group[tile]];
+  const int n0 = tile_n0[tile];
+  const cplx* Xg = X + (long long)g.off * M;
+  cplx* Cg = C + (long long)g.off * M;
+  gemm_tile<cplx, TW, cplx, OP_N, OP_N>(M, g.size, g.size, cmk(1.0, 0.0), Xg, M, W + g.woff,
+                                        g.size, cmk(0.0, 0.0), false, Cg, M, blockIdx.y * BM, n0,
+                                        As, Bs);
+}
+
+template <typename TA, typename TB, typename TC, int OPA, int OPB>
+int launch(hipStream_t s, int M, int N, int K, typename Scal<TC>::type alpha, const TA* A,
+           int lda, long long sA, const TB* B, int ldb, long long sB,
+           typename Scal<TC>::type beta, int use_beta, TC* C, int ldc, long long sC, int batch) {
+  if (M <= 0 || N <= 0 || batch <= 0) return TG_OK;
+  dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
+  hipLaunchKernelGGL((gemm_batched_kernel<TA, TB, TC, OPA, OPB>), grid, dim3(NT), 0, s, M, N,
+                     K, alpha, A, lda, sA, B, ldb, sB, beta, use_beta, C, ldc, sC);
+  TG_HIP(hipGetLastError());
+  return TG_OK;
+}
+
+template <typename TA, typename TB, typename TC>
+int dispatch(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K,
+             typename Scal<TC>::type alpha, const TA* A, int lda, long long sA, const TB* B,
+             int ldb, long long sB, typename Scal<TC>::type beta, int use_beta, TC* C, int ldc,
+             long long sC, int batch) {
+#define TG_CASE(OA, OB)                                                                   \
+  if (opa == OA && opb == OB)                                                             \
+    return launch<TA, TB, TC, OA, OB>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,    \
+                                      use_beta, C, ldc, sC, batch);
+  TG_CASE(OP_N, OP_N) TG_CASE(OP_N, OP_T) TG_CASE(OP_N, OP_C)
+  TG_CASE(OP_T, OP_N) TG_CASE(OP_T, OP_T) TG_CASE(OP_T, OP_C)
+  TG_CASE(OP_C, OP_N) TG_CASE(OP_C, OP_T) TG_CASE(OP_C, OP_C)
+#undef TG_CASE
+  tg_set_error("gemm: bad op");
+  return TG_E_ARG;
+}
+
+inline bool is_zero(cplx z) { return z.x == 0.0 && z.y == 0.0; }
+
+}  // namespace
+
+int gemm_zz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
+            const cplx* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
+            cplx beta, cplx* C, int ldc, long long sC, int batch) {
+  return dispatch<cplx, cplx, cplx>(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
+                                    !is_zero(beta), C, ldc, sC, batch);
+}
+
+int gemm_dz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
+            const double* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
+            cplx beta, cplx* C, int ldc, long long sC, int batch) {
+  if (opa == OP_C) opa = OP_T;
+  return dispatch<double, cplx, cplx>(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
+                                      !is_zero(beta), C, ldc, sC, batch);
+}
+
+int gemm_zd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
+            const cplx* A, int lda, long long sA, const double* B, int ldb, long long sB,
+            cplx beta, cplx* C, int ldc, long long sC, int batch) {
+  if (opb == OP_C) opb = OP_T;
+  return dispatch<cplx, double, cplx>(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
+                                      !is_zero(beta), C, ldc, sC, batch);
+}
+
+int gemm_dd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, double alpha,
+            const double* A, int lda, long long sA, const double* B, int ldb, long long sB,
+            double beta, double* C, int ldc, long long sC, int batch) {
+  if (opa == OP_C) opa = OP_T;
+  if (opb == OP_C) opb = OP_T;
+  return dispatch<double, double, double>(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB,
+                                          beta, beta != 0.0, C, ldc, sC, batch);
+}
+
+int gemm_grouped(hipStream_t s, int M, const cplx* X, cplx* C, const void* W, int w_is_fp32,
+                 const GroupDesc* groups_dev, const int* tile_group_dev, const int* tile_n0_dev,
+                 int ntiles_n, int ntiles_total) {
+  (void)ntiles_n;
+  if (M <= 0 || ntiles_total <= 0) return TG_OK;
+  dim3 grid(ntiles_total, (M + BM - 1) / BM, 1);
+  if (w_is_fp32)
+    hipLaunchKernelGGL(gemm_grouped_kernel<float>, grid, dim3(NT), 0, s, M, X, C,
+                       (const float*)W, groups_dev, tile_group_dev, tile_n0_dev);
+  else
+    hipLaunchKernelGGL(gemm_grouped_kernel<double>, grid, dim3(NT), 0, s, M, X, C,
+                       (const double*)W, groups_dev, tile_group_dev, tile_n0_dev);
+  TG_HIP(hipGetLastError());
+  return TG_OK;
+}

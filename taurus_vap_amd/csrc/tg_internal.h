@@ -1,0 +1,178 @@
+// Internal declarations of the gfx950 PNVAP library (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/taurus_gpu.h"
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+void tg_set_error(const std::string& msg);
+
+#define TG_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      tg_set_error(std::string(#call) + ": " + hipGetErrorString(e_));            \
+      return TG_E_HIP;                                                            \
+    }                                                                             \
+  } while (0)
+
+#define TG_TRY(call)                                                              \
+  do {                                                                            \
+    int r_ = (call);                                                              \
+    if (r_ != TG_OK) return r_;                                                   \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Complex arithmetic on double2 (x = re, y = im)
+// ---------------------------------------------------------------------------
+typedef double2 cplx;
+
+__host__ __device__ inline cplx cmk(double r, double i) { cplx z; z.x = r; z.y = i; return z; }
+__host__ __device__ inline cplx cadd(cplx a, cplx b) { return cmk(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline cplx csub(cplx a, cplx b) { return cmk(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline cplx cmul(cplx a, cplx b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ inline cplx cconj(cplx a) { return cmk(a.x, -a.y); }
+__host__ __device__ inline cplx cscale(cplx a, double s) { return cmk(a.x * s, a.y * s); }
+__host__ __device__ inline double cabs1(cplx a) { return fabs(a.x) + fabs(a.y); }   // IZAMAX measure
+__host__ __device__ inline cplx cinv(cplx a) {
+  // 1/a with scaling (Smith's algorithm)
+  if (fabs(a.x) >= fabs(a.y)) {
+    double r = a.y / a.x, d = a.x + a.y * r;
+    return cmk(1.0 / d, -r / d);
+  } else {
+    double r = a.x / a.y, d = a.y + a.x * r;
+    return cmk(r / d, -1.0 / d);
+  }
+}
+// a += b*c
+__host__ __device__ inline void cfma(cplx& a, cplx b, cplx c) {
+  a.x = fma(b.x, c.x, fma(-b.y, c.y, a.x));
+  a.y = fma(b.x, c.y, fma(b.y, c.x, a.y));
+}
+
+// ---------------------------------------------------------------------------
+// Device buffer helper
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t nbytes);
+  void release();
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// ---------------------------------------------------------------------------
+// GEMM (tg_gemm.hip):  C[b] = alpha * op(A[b]) op(B[b]) + beta * C[b]
+// column-major; op: 'N', 'T', 'C'; element types selected by the entry point.
+// ---------------------------------------------------------------------------
+enum TgOp { OP_N = 0, OP_T = 1, OP_C = 2 };
+
+// complex = complex x complex
+int gemm_zz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
+            const cplx* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
+            cplx beta, cplx* C, int ldc, long long sC, int batch);
+// complex = real x complex
+int gemm_dz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
+            const double* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
+            cplx beta, cplx* C, int ldc, long long sC, int batch);
+// complex = complex x real
+int gemm_zd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
+            const cplx* A, int lda, long long sA, const double* B, int ldb, long long sB,
+            cplx beta, cplx* C, int ldc, long long sC, int batch);
+// real = real x real
+int gemm_dd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, double alpha,
+            const double* A, int lda, long long sA, const double* B, int ldb, long long sB,
+            double beta, double* C, int ldc, long long sC, int batch);
+
+// Grouped GEMM for the contraction blocks:
+//   C_g (M x s_g, ld M) = X_g (M x s_g, ld M) * W_g (s_g x s_g, ld s_g)
+// X_g / C_g start at column offset off_g of X / C; W_g at woff_g (float or double).
+struct GroupDesc {
+  int off;        // first column of the group in X/C
+  int size;       // s_g
+  long long woff; // offset of W_g
+};
+int gemm_grouped(hipStream_t s, int M, const cplx* X, cplx* C, const void* W, int w_is_fp32,
+                 const GroupDesc* groups_dev, const int* tile_group_dev, const int* tile_n0_dev,
+                 int ntiles_n, int ntiles_total);
+
+// ---------------------------------------------------------------------------
+// Linear algebra (tg_linalg.hip)
+// ---------------------------------------------------------------------------
+// In-place Gauss-Jordan inverse with partial pivoting (pivot = largest |re|+|im|,
+// first index on ties, as IZAMAX in ZGETRF).  det[b] = determinant (complex),
+// info[b] = 0 or the first zero-pivot column (1-based).  perm: int workspace n*batch.
+int batched_inverse(hipStream_t s, int n, cplx* A, int lda, long long sA, int batch,
+                    cplx* det, int* info, int* perm);
+// Pfaffian of skew matrices (upper triangle read) by the Parlett-Reid LTL^T
+// reduction with the pivoting of ZSKTF2 (UPLO='U', MODE='P').  The matrix is
+// copied into work (n*n per matrix).  Output: logabs[b] = log|pf|, phase[b] =
+// pf/|pf| (complex), zero[b] = 1 when a pivot column vanished (pf = 0).
+int batched_pfaffian(hipStream_t s, int n, const cplx* A, int lda, long long sA, int batch,
+                     cplx* work, double* logabs, cplx* phase, int* zero);
+
+// ---------------------------------------------------------------------------
+// Contraction plan (tg_fields.hip)
+// ---------------------------------------------------------------------------
+struct Channel {
+  int npos = 0;                 // number of positions (n*n for ph, n(n-1)/2 for pp)
+  int ngroups = 0;
+  std::vector<GroupDesc> groups;
+  std::vector<int> perm;        // position -> linear index (x + y*n) of the element
+  long long wsize = 0;          // sum of s_g^2
+  DevBuf d_perm, d_groups, d_W, d_tile_group, d_tile_n0;
+  int ntiles_n = 0, ntiles_total = 0;
+};
+
+struct tg_ctx {
+  int n = 0;
+  int flags = 0;
+  int plain = 0;
+  hipStream_t stream = nullptr;
+  // basis
+  std::vector<int> sp_2mj, sp_2mt, sp_l, sp_tr;   // sp_tr 0-based here
+  cplx H0;
+  DevBuf d_H1;                                    // complex n*n
+  // contraction channels
+  Channel ph, pp;
+  int w_fp32 = 0;
+  // resident state
+  int A = 0;                                      // number of angles of the state
+  int Mphip = 0, Mphin = 0;
+  double facpi = 0, valZ = 0, valN = 0, factor_delta = 1.0;
+  int nocc = 0, nemp = 0;
+  double ovac = 0;
+  std::vector<double> voveru;
+  std::vector<double> phip, phin;
+  std::vector<cplx> weight;                       // weip * wein
+  // device work areas
+  DevBuf d_U, d_V, d_D, d_ang;                    // real n*n, angles
+  DevBuf d_real6;                                 // Kp Kn Lp Ln Mp Mn (6 n*n real)
+  DevBuf d_work;                                  // per-angle complex matrices
+  DevBuf d_blk;                                   // contraction layouts
+  DevBuf d_scal;                                  // per-angle scalars
+  DevBuf d_out;                                   // pnp accumulators
+  DevBuf d_ovl;                                   // overlap work
+  DevBuf d_int;                                   // int work
+  DevBuf d_pfw;                                   // pfaffian work
+  int have_state = 0;
+  // timing
+  int timing = 0;
+  double t_total = 0, t_contr = 0, t_gemm = 0, t_pf = 0, t_inv = 0;
+};
+
+int contraction_build(tg_ctx* c, int64_t ntb, const int16_t* abcd, const void* H2,
+                      int h2_bytes, const int8_t* trperm);
+// Gamma, DeltaLR, DeltaRL for `na` angles whose densities are stored per angle
+// (complex n*n, stride n*n).  Writes gamma (without H1), deltaLR, deltaRL.
+// hermitian != 0 reproduces calculate_fields_diag (Gamma made hermitian from its
+// upper triangle, DeltaRL = conj(DeltaLR)).
+int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const cplx* kRL,
+                    cplx* gamma, cplx* dLR, cplx* dRL, int hermitian);

@@ -1,0 +1,1180 @@
+// The gauge-angle loop of project_wavefunction(iopt=0) (src/module_projection.f90:
+// 201-373) on gfx950, batched over angles, and the C ABI entry points.
+//
+// MI355X-first restatement of the loop body (the numbers are the reference's):
+//  * The gauge rotation R = diag(e^{i phi_p} on protons, e^{i phi_n} on
+//    neutrons) (generate_rotation_gauge :569-588) has two distinct phases, so
+//      U_phi = U^T R U + V^T R* V      (calculate_thouless :694-696)
+//      V_phi = V^T R U + U^T R* V      (:699-700)
+//      R_n   = D0^T R D0               (calculate_overlap :854-855)
+//    are linear combinations of angle-independent real matrices
+//    (U^T Pi_p U, ... with Pi_p/Pi_n the proton/neutron projectors) built once
+//    per iteration by six real GEMMs; per angle they cost O(n^2).
+//  * A_phi = conj(V_phi U_phi^-1), U~ = U + V A_phi, V~ = V + U A_phi,
+//    rho = V~ V^T, kappaLR = V~ U^T, kappaRL = V U~^T (:702-716,
+//    calculate_densities module_wavefunctions.f90:1073-1087) are batched GEMMs
+//    over all angles (angle = batch index).
+//  * The TBME contraction is the grouped GEMM of tg_fields.hip.
+//  * The overlap uses the reference's Pfaffian formula.  With no fully occupied
+//    canonical state (nocc = 0) the Pfaffian of the 2*nbloc matrix M_reg is
+//    reduced exactly to one of size nbloc (Schur complement of the v/u block):
+//      detR * Pf(M_reg) = Pf(A1) Pf(A1^-1 - R2^T A1 R2)
+//                       = prod sgn(x) (-1)^(nbloc/2) Pf(J + Q^T J Q),
+//    Q = T R2 T, T = diag(|v/u|^1/2), J = blockdiag(sgn(x_j) [[0,1],[-1,0]]):
+//    no LU, no inverse, a Pfaffian 8x cheaper.  With nocc > 0 the reference's
+//    own sequence (LU/inverse of R2, M_reg, Pfaffian of size nreg) runs.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <mutex>
+#include <string>
+
+#include "tg_internal.h"
+
+// ---------------------------------------------------------------------------
+// errors / buffers
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+void tg_set_error(const std::string& msg) { g_err = msg; }
+extern "C" const char* tg_last_error(void) { return g_err.c_str(); }
+
+int DevBuf::ensure(size_t nbytes) {
+  if (nbytes <= bytes && p) return TG_OK;
+  release();
+  if (nbytes == 0) return TG_OK;
+  hipError_t e = hipMalloc(&p, nbytes);
+  if (e != hipSuccess) {
+    p = nullptr;
+    bytes = 0;
+    tg_set_error(std::string("hipMalloc(") + std::to_string(nbytes) + "): " + hipGetErrorString(e));
+    return TG_E_NOMEM;
+  }
+  bytes = nbytes;
+  return TG_OK;
+}
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+namespace {
+
+constexpr int NT = 256;
+inline unsigned nblk(long long tot) { return (unsigned)((tot + NT - 1) / NT); }
+
+// ---------------------------------------------------------------------------
+// elementwise kernels
+// ---------------------------------------------------------------------------
+// U_phi, V_phi of every angle from the six real matrices
+// (R6 = [Kp Kn Lp Ln Mp Mn], Kx = U^T Pi_x U, Lx = V^T Pi_x V, Mx = V^T Pi_x U)
+__global__ void build_uv_phi_kernel(int n, int na, const double* __restrict__ R6,
+                                    const cplx* __restrict__ ph, cplx* __restrict__ Uphi,
+                                    cplx* __restrict__ Vphi, long long stride) {
+  const long long nn = (long long)n * n;
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= nn * na) return;
+  const long long a = idx / nn;
+  const int r = (int)(idx % nn);
+  const int i = r % n, j = r / n;
+  const long long rt = j + (long long)i * n;
+  const cplx ep = ph[2 * a], en = ph[2 * a + 1];
+  const double Kp = R6[r], Kn = R6[nn + r], Lp = R6[2 * nn + r], Ln = R6[3 * nn + r];
+  const double Mp = R6[4 * nn + r], Mn = R6[5 * nn + r], Mpt = R6[4 * nn + rt], Mnt = R6[5 * nn + rt];
+  cplx u, v;
+  // e_p Kp + e_n Kn + conj(e_p) Lp + conj(e_n) Ln
+  u.x = ep.x * (Kp + Lp) + en.x * (Kn + Ln);
+  u.y = ep.y * (Kp - Lp) + en.y * (Kn - Ln);
+  // e_p Mp + e_n Mn + conj(e_p) Mp^T + conj(e_n) Mn^T
+  v.x = ep.x * (Mp + Mpt) + en.x * (Mn + Mnt);
+  v.y = ep.y * (Mp - Mpt) + en.y * (Mn - Mnt);
+  Uphi[a * stride + r] = u;
+  Vphi[a * stride + r] = v;
+}
+
+__global__ void conj_kernel(long long tot, cplx* __restrict__ X) {
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx < tot) X[idx].y = -X[idx].y;
+}
+
+// dst_a = complex(src) for every angle
+__global__ void bcast_real_kernel(long long nn, int na, const double* __restrict__ src,
+                                  cplx* __restrict__ dst) {
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= nn * na) return;
+  dst[idx] = cmk(src[idx % nn], 0.0);
+}
+
+// h_a = Gamma_a + H1
+__global__ void add_h1_kernel(long long nn, int na, const cplx* __restrict__ H1,
+                              const cplx* __restrict__ G, cplx* __restrict__ h) {
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= nn * na) return;
+  h[idx] = cadd(G[idx], H1[idx % nn]);
+}
+
+// H20 = X - X^T + fd * (Y - Z)   (calculate_H20, module_fields.f90:366-391)
+__global__ void h20_combine_kernel(int n, int na, double fd, const cplx* __restrict__ X,
+                                   const cplx* __restrict__ Y, const cplx* __restrict__ Z,
+                                   cplx* __restrict__ H20) {
+  const long long nn = (long long)n * n;
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= nn * na) return;
+  const long long a = idx / nn;
+  const int r = (int)(idx % nn);
+  const int i = r % n, j = r / n;
+  const cplx x = X[idx], xt = X[a * nn + j + (long long)i * n];
+  const cplx y = Y[idx], z = Z[idx];
+  H20[idx] = cmk(x.x - xt.x + fd * (y.x - z.x), x.y - xt.y + fd * (y.y - z.y));
+}
+
+// Per-angle traces (calculate_particle_number :78-143, calculate_expectval_energy
+// module_hamiltonian.f90:1098-1126).  out[a*9 + k]:
+// 0 prot, 1 neut, 2 Tr(rho_p^2), 3 Tr(kRL_p^T kLR_p), 4 Tr(rho_n^2),
+// 5 Tr(kRL_n^T kLR_n), 6 Tr(H1 rho), 7 Tr(Gamma rho), 8 Tr(DeltaLR kRL^T)
+__global__ void __launch_bounds__(NT)
+traces_kernel(int n, const cplx* __restrict__ H1, const cplx* __restrict__ rho,
+              const cplx* __restrict__ kLR, const cplx* __restrict__ kRL,
+              const cplx* __restrict__ G, const cplx* __restrict__ dLR, long long stride,
+              cplx* __restrict__ out) {
+  const long long a = blockIdx.x;
+  const int h = n / 2;
+  const cplx* r_ = rho + a * stride;
+  const cplx* kl = kLR + a * stride;
+  const cplx* kr = kRL + a * stride;
+  const cplx* g_ = G + a * stride;
+  const cplx* dl = dLR + a * stride;
+  cplx acc[9];
+  for (int k = 0; k < 9; ++k) acc[k] = cmk(0.0, 0.0);
+  const long long nn = (long long)n * n;
+  for (long long idx = threadIdx.x; idx < nn; idx += NT) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    const long long t = j + (long long)i * n;   // (j,i)
+    const cplx rji = r_[t];
+    const bool pi = i < h, pj = j < h;
+    if (i == j) acc[pi ? 0 : 1] = cadd(acc[pi ? 0 : 1], r_[idx]);
+    if (pi == pj) {
+      cfma(acc[pi ? 2 : 4], r_[idx], rji);
+      cfma(acc[pi ? 3 : 5], kr[idx], kl[idx]);
+    }
+    cfma(acc[6], H1[idx], rji);
+    cfma(acc[7], g_[idx], rji);
+    cfma(acc[8], dl[idx], kr[idx]);
+  }
+  __shared__ cplx red[NT];
+  for (int k = 0; k < 9; ++k) {
+    red[threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[a * 9 + k] = red[0];
+    __syncthreads();
+  }
+}
+
+// Q = T (e_p Pp + e_n Pn)[nemp:, nemp:] T and JQ = J Q (reduced overlap).
+// JQ row 2j = s_j Q row 2j+1, row 2j+1 = -s_j Q row 2j.
+__global__ void build_q_kernel(int n, int nemp, int nb, int na, const double* __restrict__ Pp,
+                               const double* __restrict__ Pn, const double* __restrict__ Tsc,
+                               const double* __restrict__ sgn, const cplx* __restrict__ ph,
+                               cplx* __restrict__ Q, cplx* __restrict__ JQ) {
+  const long long mm = (long long)nb * nb;
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= mm * na) return;
+  const long long a = idx / mm;
+  const int r = (int)(idx % mm);
+  const int i = r % nb, j = r / nb;
+  const cplx ep = ph[2 * a], en = ph[2 * a + 1];
+  auto q = [&](int ii) {
+    const long long o = (ii + nemp) + (long long)(j + nemp) * n;
+    const double pp = Pp[o], pn = Pn[o];
+    const double s = Tsc[ii] * Tsc[j];
+    return cmk(s * (ep.x * pp + en.x * pn), s * (ep.y * pp + en.y * pn));
+  };
+  Q[idx] = q(i);
+  const int pr = i >> 1;
+  const double sj = sgn[pr];
+  if ((i & 1) == 0) {
+    const cplx v = q(i + 1);
+    JQ[idx] = cmk(sj * v.x, sj * v.y);
+  } else {
+    const cplx v = q(i - 1);
+    JQ[idx] = cmk(-sj * v.x, -sj * v.y);
+  }
+}
+
+// K += J (the sgn-weighted symplectic unit)
+__global__ void add_j_kernel(int nb, int na, const double* __restrict__ sgn, cplx* __restrict__ K) {
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  const long long pairs = (long long)(nb / 2) * na;
+  if (idx >= pairs) return;
+  const long long a = idx / (nb / 2);
+  const int p = (int)(idx % (nb / 2));
+  const int i = 2 * p;
+  const long long mm = (long long)nb * nb;
+  cplx* k = K + a * mm;
+  k[i + (long long)(i + 1) * nb].x += sgn[p];
+  k[(i + 1) + (long long)i * nb].x -= sgn[p];
+}
+
+// Rn2 = (e_p Pp + e_n Pn)[nemp:, nemp:] for the direct overlap
+__global__ void build_rn2_kernel(int n, int nemp, int n2, int na, const double* __restrict__ Pp,
+                                 const double* __restrict__ Pn, const cplx* __restrict__ ph,
+                                 cplx* __restrict__ R2) {
+  const long long mm = (long long)n2 * n2;
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= mm * na) return;
+  const long long a = idx / mm;
+  const int r = (int)(idx % mm);
+  const int i = r % n2, j = r / n2;
+  const cplx ep = ph[2 * a], en = ph[2 * a + 1];
+  const long long o = (i + nemp) + (long long)(j + nemp) * n;
+  R2[idx] = cmk(ep.x * Pp[o] + en.x * Pn[o], ep.y * Pp[o] + en.y * Pn[o]);
+}
+
+// M_reg of calculate_overlap (:891-918) from v/u and Rinv (n2 x n2)
+__global__ void build_mreg_kernel(int n2, int nb0, int nb1, int nreg, int na,
+                                  const double* __restrict__ vou0, const double* __restrict__ vou1,
+                                  const cplx* __restrict__ Rinv, cplx* __restrict__ M) {
+  const long long mm = (long long)nreg * nreg;
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= mm * na) return;
+  const long long a = idx / mm;
+  const int r = (int)(idx % mm);
+  const int i = r % nreg, j = r / nreg;   // 0-based row i, column j
+  const cplx* Ri = Rinv + a * (long long)n2 * n2;
+  cplx v = cmk(0.0, 0.0);
+  if (i < nb1 && j < nb1) {
+    // upper-left: M(i,i+1) = vou1(i) for even i, M(i,i-1) = vou1(i) for odd i
+    if ((i & 1) == 0 && j == i + 1) v.x = vou1[i];
+    if ((i & 1) == 1 && j == i - 1) v.x = vou1[i];
+  } else if (i >= nb1 && j >= nb1) {
+    const int jj = i - nb1;
+    if ((i & 1) == 0 && j == i + 1) v.x = -vou0[jj];
+    if ((i & 1) == 1 && j == i - 1) v.x = -vou0[jj];
+  } else if (i < nb1 && j >= nb1) {
+    const cplx x = Ri[i + (long long)(j - nb1) * n2];
+    v = cmk(-x.x, -x.y);
+  } else {
+    v = Ri[j + (long long)(i - nb1) * n2];
+  }
+  M[idx] = v;
+}
+
+// sum_gauge scalars (:425-458): f1 = w*o, pnp += f1*{1, E, Z, N, Z2, N2}
+// per-angle in: over[a], scal[a*9+k]; writes f1/f2 per angle and the 6 sums
+__global__ void __launch_bounds__(NT)
+gauge_scalars_kernel(int na, int n, cplx H0, const cplx* __restrict__ w, const cplx* __restrict__ over,
+                     const cplx* __restrict__ sc, cplx* __restrict__ f12, cplx* __restrict__ ener,
+                     cplx* __restrict__ pnp) {
+  __shared__ cplx red[6][NT];
+  cplx acc[6];
+  for (int k = 0; k < 6; ++k) acc[k] = cmk(0.0, 0.0);
+  for (int a = threadIdx.x; a < na; a += NT) {
+    const cplx* s = sc + (long long)a * 9;
+    const cplx prot = s[0], neut = s[1];
+    const cplx prot2 = csub(cadd(prot, cmul(prot, prot)), csub(s[2], s[3]));
+    const cplx neut2 = csub(cadd(neut, cmul(neut, neut)), csub(s[4], s[5]));
+    cplx e = cadd(H0, s[6]);
+    e = cadd(e, cscale(cadd(s[7], s[8]), 0.5));
+    const cplx f1 = cmul(w[a], over[a]);
+    f12[2 * a] = f1;
+    f12[2 * a + 1] = cmul(f1, e);
+    ener[a] = e;
+    acc[0] = cadd(acc[0], f1);
+    cfma(acc[1], e, f1);
+    cfma(acc[2], prot, f1);
+    cfma(acc[3], neut, f1);
+    cfma(acc[4], prot2, f1);
+    cfma(acc[5], neut2, f1);
+  }
+  for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = cadd(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) pnp[threadIdx.x] = red[threadIdx.x][0];
+}
+
+// pnp_A = sum f1 A, pnp_H20 = sum f1 H20, pnp_AH20 = sum f2 A   (angles in order)
+__global__ void gauge_matrices_kernel(long long nn, int na, const cplx* __restrict__ f12,
+                                      const cplx* __restrict__ Aphi, const cplx* __restrict__ H20,
+                                      long long stride, cplx* __restrict__ pA,
+                                      cplx* __restrict__ pH, cplx* __restrict__ pAH) {
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= nn) return;
+  cplx sa = cmk(0, 0), sh = cmk(0, 0), sah = cmk(0, 0);
+  for (int a = 0; a < na; ++a) {
+    const cplx f1 = f12[2 * a], f2 = f12[2 * a + 1];
+    const cplx av = Aphi[a * stride + idx], hv = H20[a * stride + idx];
+    cfma(sa, av, f1);
+    cfma(sh, hv, f1);
+    cfma(sah, av, f2);
+  }
+  pA[idx] = sa;
+  pH[idx] = sh;
+  pAH[idx] = sah;
+}
+
+// calculate_projected_H20 (:729-759); imag[block] = max Im over the block
+__global__ void __launch_bounds__(NT)
+projected_h20_kernel(int n, const cplx* __restrict__ pnp, const cplx* __restrict__ pA,
+                     const cplx* __restrict__ pH, const cplx* __restrict__ pAH,
+                     double* __restrict__ H20v, double* __restrict__ imag) {
+  const long long nn = (long long)n * n;
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  const cplx over = pnp[0];
+  const cplx oinv = cinv(over);
+  const cplx ener = cmul(pnp[1], oinv);
+  double im = -INFINITY;
+  if (idx < nn) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    const long long t = j + (long long)i * n;
+    const cplx A = cscale(cmul(csub(pA[idx], pA[t]), oinv), 0.5);
+    const cplx H = cscale(cmul(csub(pH[idx], pH[t]), oinv), 0.5);
+    const cplx AH = cscale(cmul(csub(pAH[idx], pAH[t]), oinv), 0.5);
+    const cplx x = cadd(csub(AH, cmul(ener, A)), H);
+    H20v[idx] = x.x;
+    const cplx y = cmul(cadd(csub(pAH[idx], cmul(ener, pA[idx])), pH[idx]), oinv);
+    im = y.y;
+  }
+  __shared__ double red[NT];
+  red[threadIdx.x] = im;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) imag[blockIdx.x] = red[0];
+}
+
+// overlap assembly: o = pre * phase * exp(logabs + lnorm)
+__global__ void overlap_finish_kernel(int na, const double* __restrict__ logabs,
+                                      const cplx* __restrict__ phase, const int* __restrict__ zero,
+                                      const cplx* __restrict__ pre, double lnorm,
+                                      cplx* __restrict__ over) {
+  int a = blockIdx.x * NT + threadIdx.x;
+  if (a >= na) return;
+  if (zero[a]) { over[a] = cmk(0.0, 0.0); return; }
+  const double m = exp(logabs[a] + lnorm);
+  const cplx z = cmul(pre[a], phase[a]);
+  over[a] = cmk(m * z.x, m * z.y);
+}
+
+__global__ void fill_cplx_kernel(int na, cplx v, cplx* __restrict__ x) {
+  int a = blockIdx.x * NT + threadIdx.x;
+  if (a < na) x[a] = v;
+}
+
+// pre[a] = signfac * det[a] for the direct overlap
+__global__ void scale_det_kernel(int na, double s, const cplx* __restrict__ det, cplx* __restrict__ pre) {
+  int a = blockIdx.x * NT + threadIdx.x;
+  if (a < na) pre[a] = cscale(det[a], s);
+}
+
+}  // namespace
+
+// ===========================================================================
+// host orchestration
+// ===========================================================================
+namespace {
+
+struct Timer {
+  tg_ctx* c;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double* slot;
+  Timer(tg_ctx* c_, double* s) : c(c_), slot(s) {
+    if (c->timing) {
+      (void)hipEventCreate(&e0);
+      (void)hipEventCreate(&e1);
+      (void)hipEventRecord(e0, c->stream);
+    }
+  }
+  ~Timer() {
+    if (c->timing) {
+      (void)hipEventRecord(e1, c->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      *slot += ms;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    }
+  }
+};
+
+int sign_pow(long long e) { return (e % 2 == 0) ? 1 : -1; }
+
+// per-angle complex buffers of the loop body
+enum { W_UPHI, W_VPHI, W_UT, W_VT, W_RHO, W_KLR, W_KRL, W_G, W_H, W_DLR, W_DRL, W_T, W_X, W_Y,
+       W_Z, W_COUNT };
+
+int project_device(tg_ctx* c) {
+  const int n = c->n, h = n / 2, A = c->A;
+  const long long nn = (long long)n * n;
+  hipStream_t s = c->stream;
+  const int chunk = c->plain ? 1 : A;
+  const bool proj = std::max(c->Mphip, c->Mphin) > 1;
+  const double* U = c->d_U.as<double>();
+  const double* V = c->d_V.as<double>();
+  const double* D = c->d_D.as<double>();
+  double* R6 = c->d_real6.as<double>();      // Kp Kn Lp Ln Mp Mn Pp Pn
+  const cplx one = cmk(1, 0), zero = cmk(0, 0);
+
+  // ---- angle-independent real products (once per iteration)
+  {
+    Timer tm(c, &c->t_gemm);
+    for (int sp = 0; sp < 2; ++sp) {
+      const int r0 = sp * h;
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, U + r0, n, 0, U + r0, n, 0, 0.0, R6 + (0 + sp) * nn, n, 0, 1));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, V + r0, n, 0, V + r0, n, 0, 0.0, R6 + (2 + sp) * nn, n, 0, 1));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, V + r0, n, 0, U + r0, n, 0, 0.0, R6 + (4 + sp) * nn, n, 0, 1));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, D + r0, n, 0, D + r0, n, 0, 0.0, R6 + (6 + sp) * nn, n, 0, 1));
+    }
+  }
+  const double* Pp = R6 + 6 * nn;
+  const double* Pn = R6 + 7 * nn;
+
+  // ---- work areas
+  const size_t wsz = (size_t)W_COUNT * chunk * nn;
+  TG_TRY(c->d_work.ensure(sizeof(cplx) * (wsz + 2 * (size_t)A * nn)));
+  cplx* work = c->d_work.as<cplx>();
+  auto Wb = [&](int k) { return work + (size_t)k * chunk * nn; };
+  cplx* Akeep = work + wsz;                  // A_phi of every angle
+  cplx* Hkeep = Akeep + (size_t)A * nn;      // H20_phi of every angle
+  // scalars: ph (2A) | traces (9A) | over (A) | ener (A) | f12 (2A) | pnp (8) | pre (A) | det (A)
+  //          | phase (A) | w (A) | logabs (A doubles) | imag (doubles)
+  TG_TRY(c->d_scal.ensure(sizeof(cplx) * (19 * (size_t)A + 8) + sizeof(double) * A +
+                          sizeof(double) * (nn / NT + 2)));
+  cplx* d_ph = c->d_scal.as<cplx>();
+  cplx* d_tr = d_ph + 2 * A;
+  cplx* d_over = d_tr + 9 * A;
+  cplx* d_ener = d_over + A;
+  cplx* d_f12 = d_ener + A;
+  cplx* d_pnp = d_f12 + 2 * A;
+  cplx* d_pre = d_pnp + 8;
+  cplx* d_det = d_pre + A;
+  cplx* d_phase = d_det + A;
+  cplx* d_w = d_phase + A;
+  double* d_logabs = reinterpret_cast<double*>(d_w + A);
+  double* d_imag = d_logabs + A;
+  TG_TRY(c->d_int.ensure(sizeof(int) * ((size_t)A * (2 * n + 4) + 8)));
+  int* d_info = c->d_int.as<int>();
+  int* d_zero = d_info + A;
+  int* d_perm = d_zero + A;
+
+  // angles, phases and weights
+  {
+    std::vector<cplx> hph(2 * A), hw(A);
+    for (int a = 0; a < A; ++a) {
+      hph[2 * a] = cmk(cos(c->phip[a]), sin(c->phip[a]));
+      hph[2 * a + 1] = cmk(cos(c->phin[a]), sin(c->phin[a]));
+      hw[a] = c->weight[a];
+    }
+    TG_HIP(hipMemcpyAsync(d_ph, hph.data(), sizeof(cplx) * 2 * A, hipMemcpyHostToDevice, s));
+    TG_HIP(hipMemcpyAsync(d_w, hw.data(), sizeof(cplx) * A, hipMemcpyHostToDevice, s));
+    TG_HIP(hipStreamSynchronize(s));   // host vectors go out of scope
+  }
+
+  // ---- overlap set-up (per iteration, host scalars)
+  const int nocc = c->nocc, nemp = c->nemp;
+  const int nbloc = n - nocc - nemp;
+  const bool reduced = proj && nocc == 0 && nbloc > 0 && (nbloc % 2 == 0);
+  const double ovac = (nbloc == 0) ? 0.0 : c->ovac;
+  // signfac of :811-814 with state 0 = state 1
+  const int n0 = n - nocc, n2 = n - nemp;
+  const long long e1 = (long long)n2 * (n2 + 1) / 2;
+  const long long e2 = (long long)nocc * (nocc - 1) / 2;
+  const long long e3 = (long long)nocc * (nocc + 1) + (long long)nocc * (n0 + n2) + (long long)nocc * n0;
+  const int signfac = sign_pow(e1) * sign_pow(e2) * sign_pow(e3);
+  std::vector<double> Tsc, sg;
+  double prod_sgn = 1.0;
+  if (reduced) {
+    Tsc.resize(nbloc);
+    sg.resize(nbloc / 2);
+    for (int p = 0; p < nbloc / 2; ++p) {
+      const double x = c->voveru[nemp + 2 * p];
+      sg[p] = (x < 0) ? -1.0 : 1.0;
+      prod_sgn *= sg[p];
+      Tsc[2 * p] = Tsc[2 * p + 1] = sqrt(fabs(x));
+    }
+  }
+  TG_TRY(c->d_ovl.ensure(sizeof(double) * (2 * (size_t)n + 8)));
+  double* d_T = c->d_ovl.as<double>();
+  double* d_sg = d_T + n;
+  if (reduced) {
+    TG_HIP(hipMemcpyAsync(d_T, Tsc.data(), sizeof(double) * nbloc, hipMemcpyHostToDevice, s));
+    TG_HIP(hipMemcpyAsync(d_sg, sg.data(), sizeof(double) * (nbloc / 2), hipMemcpyHostToDevice, s));
+    TG_HIP(hipStreamSynchronize(s));
+  }
+
+  // ---- loop over angle chunks
+  for (int a0 = 0; a0 < A; a0 += chunk) {
+    const int na = std::min(chunk, A - a0);
+    const long long tot = nn * na;
+    cplx* Uphi = Wb(W_UPHI);
+    cplx* Vphi = Wb(W_VPHI);
+    cplx* Aphi = Akeep + (size_t)a0 * nn;
+    cplx* Ut = Wb(W_UT);
+    cplx* Vt = Wb(W_VT);
+    cplx* rho = Wb(W_RHO);
+    cplx* kLR = Wb(W_KLR);
+    cplx* kRL = Wb(W_KRL);
+    cplx* G = Wb(W_G);
+    cplx* hsp = Wb(W_H);
+    cplx* dLR = Wb(W_DLR);
+    cplx* dRL = Wb(W_DRL);
+    cplx* T = Wb(W_T);
+    cplx* X = Wb(W_X);
+    cplx* Y = Wb(W_Y);
+    cplx* Z = Wb(W_Z);
+    cplx* H20 = Hkeep + (size_t)a0 * nn;
+
+    // Thouless matrix A_phi (calculate_thouless :684-718)
+    {
+      Timer tm(c, &c->t_inv);
+      hipLaunchKernelGGL(build_uv_phi_kernel, dim3(nblk(tot)), dim3(NT), 0, s, n, na, R6,
+                         d_ph + 2 * a0, Uphi, Vphi, nn);
+      TG_HIP(hipGetLastError());
+      TG_TRY(batched_inverse(s, n, Uphi, n, nn, na, d_det, d_info, d_perm));
+    }
+    {
+      Timer tm(c, &c->t_gemm);
+      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, Vphi, n, nn, Uphi, n, nn, zero, Aphi, n, nn, na));
+      hipLaunchKernelGGL(conj_kernel, dim3(nblk(tot)), dim3(NT), 0, s, tot, Aphi);
+      hipLaunchKernelGGL(bcast_real_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, U, Ut);
+      hipLaunchKernelGGL(bcast_real_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, V, Vt);
+      TG_HIP(hipGetLastError());
+      TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, V, n, 0, Aphi, n, nn, one, Ut, n, nn, na));
+      TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, U, n, 0, Aphi, n, nn, one, Vt, n, nn, na));
+      // densities (calculate_densities with conj(U~), conj(V~))
+      TG_TRY(gemm_zd(s, OP_N, OP_T, n, n, n, one, Vt, n, nn, V, n, 0, zero, rho, n, nn, na));
+      TG_TRY(gemm_zd(s, OP_N, OP_T, n, n, n, one, Vt, n, nn, U, n, 0, zero, kLR, n, nn, na));
+      TG_TRY(gemm_dz(s, OP_N, OP_T, n, n, n, one, V, n, 0, Ut, n, nn, zero, kRL, n, nn, na));
+    }
+    // fields
+    {
+      Timer tm(c, &c->t_contr);
+      TG_TRY(contraction_run(c, na, rho, kLR, proj ? kRL : nullptr, G, dLR, dRL, proj ? 0 : 1));
+      hipLaunchKernelGGL(add_h1_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, c->d_H1.as<cplx>(),
+                         G, hsp);
+      TG_HIP(hipGetLastError());
+    }
+    // particle numbers and energy
+    hipLaunchKernelGGL(traces_kernel, dim3(na), dim3(NT), 0, s, n, c->d_H1.as<cplx>(), rho, kLR,
+                       kRL, G, dLR, nn, d_tr + 9 * (size_t)a0);
+    TG_HIP(hipGetLastError());
+    // H20 (calculate_H20 with U~, V~)
+    {
+      Timer tm(c, &c->t_gemm);
+      TG_TRY(gemm_zz(s, OP_T, OP_N, n, n, n, one, Ut, n, nn, hsp, n, nn, zero, T, n, nn, na));
+      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, T, n, nn, Vt, n, nn, zero, X, n, nn, na));
+      TG_TRY(gemm_zz(s, OP_T, OP_N, n, n, n, one, Ut, n, nn, dLR, n, nn, zero, T, n, nn, na));
+      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, T, n, nn, Ut, n, nn, zero, Y, n, nn, na));
+      TG_TRY(gemm_zz(s, OP_T, OP_N, n, n, n, one, Vt, n, nn, dRL, n, nn, zero, T, n, nn, na));
+      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, T, n, nn, Vt, n, nn, zero, Z, n, nn, na));
+      hipLaunchKernelGGL(h20_combine_kernel, dim3(nblk(tot)), dim3(NT), 0, s, n, na,
+                         c->factor_delta, X, Y, Z, H20);
+      TG_HIP(hipGetLastError());
+    }
+    // overlap
+    if (!proj) {
+      hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, cmk(1.0, 0.0),
+                         d_over + a0);
+      TG_HIP(hipGetLastError());
+    } else if (reduced) {
+      Timer tm(c, &c->t_pf);
+      const int nb = nbloc;
+      const long long mm = (long long)nb * nb;
+      // reuse X/Y/Z/T as Q/JQ/K/pfaffian work (nb <= n)
+      cplx* Q = X;
+      cplx* JQ = Y;
+      cplx* K = Z;
+      hipLaunchKernelGGL(build_q_kernel, dim3(nblk(mm * na)), dim3(NT), 0, s, n, nemp, nb, na, Pp,
+                         Pn, d_T, d_sg, d_ph + 2 * a0, Q, JQ);
+      TG_HIP(hipGetLastError());
+      TG_TRY(gemm_zz(s, OP_T, OP_N, nb, nb, nb, one, Q, nb, mm, JQ, nb, mm, zero, K, nb, mm, na));
+      hipLaunchKernelGGL(add_j_kernel, dim3(nblk((long long)(nb / 2) * na)), dim3(NT), 0, s, nb, na,
+                         d_sg, K);
+      TG_HIP(hipGetLastError());
+      TG_TRY(batched_pfaffian(s, nb, K, nb, mm, na, T, d_logabs, d_phase, d_zero));
+      const double pre = signfac * prod_sgn * sign_pow(nb / 2);
+      hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, cmk(pre, 0.0), d_pre);
+      hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, d_logabs,
+                         d_phase, d_zero, d_pre, ovac, d_over + a0);
+      TG_HIP(hipGetLastError());
+    } else {
+      // reference sequence: R2 -> inverse + det -> M_reg -> Pfaffian(nreg)
+      Timer tm(c, &c->t_pf);
+      const int nreg = 2 * nbloc;
+      const long long m2 = (long long)n2 * n2, mr = (long long)nreg * nreg;
+      TG_TRY(c->d_pfw.ensure(sizeof(cplx) * (size_t)na * (m2 + 2 * mr) + sizeof(double) * 2 * n));
+      cplx* R2 = c->d_pfw.as<cplx>();
+      cplx* M = R2 + (size_t)na * m2;
+      cplx* PW = M + (size_t)na * mr;
+      double* d_vou = reinterpret_cast<double*>(PW + (size_t)na * mr);
+      TG_HIP(hipMemcpyAsync(d_vou, c->voveru.data() + nemp, sizeof(double) * nbloc,
+                            hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(build_rn2_kernel, dim3(nblk(m2 * na)), dim3(NT), 0, s, n, nemp, n2, na, Pp,
+                         Pn, d_ph + 2 * a0, R2);
+      TG_HIP(hipGetLastError());
+      TG_TRY(batched_inverse(s, n2, R2, n2, m2, na, d_det, d_info, d_perm));
+      if (nbloc > 0) {
+        hipLaunchKernelGGL(build_mreg_kernel, dim3(nblk(mr * na)), dim3(NT), 0, s, n2, nbloc, nbloc,
+                           nreg, na, d_vou, d_vou, R2, M);
+        TG_HIP(hipGetLastError());
+        TG_TRY(batched_pfaffian(s, nreg, M, nreg, mr, na, PW, d_logabs, d_phase, d_zero));
+      } else {
+        std::vector<double> zl(na, 0.0);
+        std::vector<cplx> on(na, cmk(1, 0));
+        std::vector<int> zz(na, 0);
+        TG_HIP(hipMemcpyAsync(d_logabs, zl.data(), sizeof(double) * na, hipMemcpyHostToDevice, s));
+        TG_HIP(hipMemcpyAsync(d_phase, on.data(), sizeof(cplx) * na, hipMemcpyHostToDevice, s));
+        TG_HIP(hipMemcpyAsync(d_zero, zz.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
+        TG_HIP(hipStreamSynchronize(s));
+      }
+      hipLaunchKernelGGL(scale_det_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, (double)signfac,
+                         d_det, d_pre);
+      hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, d_logabs,
+                         d_phase, d_zero, d_pre, ovac, d_over + a0);
+      TG_HIP(hipGetLastError());
+    }
+  }
+
+  // ---- sum_gauge and calculate_projected_H20
+  TG_TRY(c->d_out.ensure(sizeof(cplx) * 3 * nn + sizeof(double) * nn));
+  cplx* pA = c->d_out.as<cplx>();
+  cplx* pH = pA + nn;
+  cplx* pAH = pH + nn;
+  double* H20v = reinterpret_cast<double*>(pAH + nn);
+  hipLaunchKernelGGL(gauge_scalars_kernel, dim3(1), dim3(NT), 0, s, A, n, c->H0, d_w, d_over, d_tr,
+                     d_f12, d_ener, d_pnp);
+  hipLaunchKernelGGL(gauge_matrices_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, A, d_f12, Akeep,
+                     Hkeep, nn, pA, pH, pAH);
+  hipLaunchKernelGGL(projected_h20_kernel, dim3(nblk(nn)), dim3(NT), 0, s, n, d_pnp, pA, pH, pAH,
+                     H20v, d_imag);
+  TG_HIP(hipGetLastError());
+  return TG_OK;
+}
+
+int upload_state(tg_ctx* c, const double* U0, const double* V0, const double* D0, int nocc0,
+                 int nemp0, double ovac0, const double* voveru0, int Mphip, int Mphin,
+                 double facpi, double valZ, double valN, double factor_delta) {
+  const int n = c->n;
+  if (!U0 || !V0 || !D0 || Mphip < 1 || Mphin < 1 || nocc0 < 0 || nemp0 < 0 ||
+      nocc0 + nemp0 > n || (n - nocc0 > 0 && !voveru0)) {
+    tg_set_error("tg_set_state: invalid argument");
+    return TG_E_ARG;
+  }
+  const size_t nn = (size_t)n * n;
+  TG_TRY(c->d_U.ensure(sizeof(double) * nn));
+  TG_TRY(c->d_V.ensure(sizeof(double) * nn));
+  TG_TRY(c->d_D.ensure(sizeof(double) * nn));
+  TG_TRY(c->d_real6.ensure(sizeof(double) * 8 * nn));
+  TG_HIP(hipMemcpyAsync(c->d_U.p, U0, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+  TG_HIP(hipMemcpyAsync(c->d_V.p, V0, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+  TG_HIP(hipMemcpyAsync(c->d_D.p, D0, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+  TG_HIP(hipStreamSynchronize(c->stream));
+  c->nocc = nocc0;
+  c->nemp = nemp0;
+  c->ovac = ovac0;
+  c->voveru.assign(voveru0, voveru0 + (n - nocc0));
+  c->Mphip = Mphip;
+  c->Mphin = Mphin;
+  c->facpi = facpi;
+  c->valZ = valZ;
+  c->valN = valN;
+  c->factor_delta = factor_delta;
+  // assign_angles (:112-185)
+  c->A = Mphip * Mphin;
+  c->phip.resize(c->A);
+  c->phin.resize(c->A);
+  c->weight.resize(c->A);
+  int i = 0;
+  for (int jp = 1; jp <= Mphip; ++jp)
+    for (int jn = 1; jn <= Mphin; ++jn) {
+      const double pp = (Mphip % 2 == 1) ? facpi * (jp - 1.0) / Mphip : facpi * (jp - 0.5) / Mphip;
+      const double pn = (Mphin % 2 == 1) ? facpi * (jn - 1.0) / Mphin : facpi * (jn - 0.5) / Mphin;
+      c->phip[i] = pp;
+      c->phin[i] = pn;
+      const cplx wp = cmk(cos(-pp * valZ) / Mphip, sin(-pp * valZ) / Mphip);
+      const cplx wn = cmk(cos(-pn * valN) / Mphin, sin(-pn * valN) / Mphin);
+      c->weight[i] = cmul(wp, wn);
+      ++i;
+    }
+  c->have_state = 1;
+  return TG_OK;
+}
+
+int fetch_results(tg_ctx* c, double* scal, double* pA, double* pH, double* pAH, double* H20v,
+                  double* imag) {
+  const int n = c->n;
+  const size_t nn = (size_t)n * n;
+  TG_HIP(hipStreamSynchronize(c->stream));
+  const cplx* d_out = c->d_out.as<cplx>();
+  cplx* d_ph = c->d_scal.as<cplx>();
+  const int A = c->A;
+  cplx* d_pnp = d_ph + 2 * A + 9 * A + A + A + 2 * A;
+  cplx* d_w = d_pnp + 8 + 3 * A;
+  double* d_imag = reinterpret_cast<double*>(d_w + A) + A;
+  if (scal) TG_HIP(hipMemcpy(scal, d_pnp, sizeof(cplx) * 6, hipMemcpyDeviceToHost));
+  if (pA) TG_HIP(hipMemcpy(pA, d_out, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
+  if (pH) TG_HIP(hipMemcpy(pH, d_out + nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
+  if (pAH) TG_HIP(hipMemcpy(pAH, d_out + 2 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
+  if (H20v)
+    TG_HIP(hipMemcpy(H20v, reinterpret_cast<const double*>(d_out + 3 * nn), sizeof(double) * nn,
+                     hipMemcpyDeviceToHost));
+  const int nb = (int)((nn + NT - 1) / NT);
+  std::vector<double> im(nb);
+  TG_HIP(hipMemcpy(im.data(), d_imag, sizeof(double) * nb, hipMemcpyDeviceToHost));
+  double mx = -INFINITY;
+  for (double v : im) mx = std::max(mx, v);
+  if (imag) *imag = mx;
+  if (mx > 1.0e-8) {
+    tg_set_error("The gradient has non-zero imaginary part: " + std::to_string(mx));
+    return TG_E_IMAG_GRADIENT;
+  }
+  return TG_OK;
+}
+
+int get_device_ready() {
+  int cnt = 0;
+  hipError_t e = hipGetDeviceCount(&cnt);
+  if (e != hipSuccess || cnt < 1) {
+    tg_set_error("no HIP device available");
+    return TG_E_NODEVICE;
+  }
+  return TG_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int tg_init(tg_ctx** out, int ndim, const int32_t* sp_2mj, const int32_t* sp_2mt,
+            const int32_t* sp_l, const int32_t* sp_tr, double H0_re, double H0_im,
+            const double* H1, int64_t ntb, const int16_t* abcd, const void* H2, int h2_bytes,
+            const int8_t* trperm, int flags) {
+  if (!out || ndim <= 0 || ndim % 2 != 0 || !sp_2mj || !sp_2mt || !sp_l || !sp_tr || !H1 ||
+      ntb < 0 || (ntb > 0 && (!abcd || !H2 || !trperm)) || (h2_bytes != 4 && h2_bytes != 8)) {
+    tg_set_error("tg_init: invalid argument");
+    return TG_E_ARG;
+  }
+  TG_TRY(get_device_ready());
+  tg_ctx* c = new tg_ctx();
+  c->n = ndim;
+  c->flags = flags;
+  c->plain = (flags & TG_FLAG_PLAIN) ? 1 : 0;
+  c->w_fp32 = (flags & TG_FLAG_TBME_FP32) ? 1 : 0;
+  c->sp_2mj.assign(sp_2mj, sp_2mj + ndim);
+  c->sp_2mt.assign(sp_2mt, sp_2mt + ndim);
+  c->sp_l.assign(sp_l, sp_l + ndim);
+  c->sp_tr.resize(ndim);
+  for (int i = 0; i < ndim; ++i) {
+    c->sp_tr[i] = sp_tr[i] - 1;
+    if (c->sp_tr[i] < 0 || c->sp_tr[i] >= ndim) {
+      delete c;
+      tg_set_error("tg_init: sp_tr out of range");
+      return TG_E_ARG;
+    }
+  }
+  c->H0 = cmk(H0_re, H0_im);
+  int r = TG_OK;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    tg_set_error("tg_init: hipStreamCreate failed");
+    return TG_E_HIP;
+  }
+  r = c->d_H1.ensure(sizeof(cplx) * (size_t)ndim * ndim);
+  if (r == TG_OK && hipMemcpy(c->d_H1.p, H1, sizeof(cplx) * (size_t)ndim * ndim,
+                              hipMemcpyHostToDevice) != hipSuccess) r = TG_E_HIP;
+  if (r == TG_OK) r = contraction_build(c, ntb, abcd, H2, h2_bytes, trperm);
+  if (r != TG_OK) {
+    tg_finalize(c);
+    return r;
+  }
+  *out = c;
+  return TG_OK;
+}
+
+int tg_finalize(tg_ctx* c) {
+  if (!c) return TG_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (DevBuf* b : {&c->d_H1, &c->d_U, &c->d_V, &c->d_D, &c->d_ang, &c->d_real6, &c->d_work,
+                    &c->d_blk, &c->d_scal, &c->d_out, &c->d_ovl, &c->d_int, &c->d_pfw})
+    b->release();
+  for (Channel* ch : {&c->ph, &c->pp})
+    for (DevBuf* b : {&ch->d_perm, &ch->d_groups, &ch->d_W, &ch->d_tile_group, &ch->d_tile_n0})
+      b->release();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return TG_OK;
+}
+
+int tg_set_mode(tg_ctx* c, int plain) {
+  if (!c) return TG_E_ARG;
+  c->plain = plain ? 1 : 0;
+  return TG_OK;
+}
+
+int tg_calculate_fields(tg_ctx* c, const double* rhoLR, const double* kappaLR,
+                        const double* kappaRL, double* gammaLR, double* hspLR, double* deltaLR,
+                        double* deltaRL) {
+  if (!c || !rhoLR || !kappaLR || !kappaRL || !gammaLR || !hspLR || !deltaLR || !deltaRL) {
+    tg_set_error("tg_calculate_fields: null argument");
+    return TG_E_ARG;
+  }
+  const int n = c->n;
+  const size_t nn = (size_t)n * n;
+  TG_TRY(c->d_work.ensure(sizeof(cplx) * 8 * nn));
+  cplx* w = c->d_work.as<cplx>();
+  TG_HIP(hipMemcpyAsync(w, rhoLR, sizeof(cplx) * nn, hipMemcpyHostToDevice, c->stream));
+  TG_HIP(hipMemcpyAsync(w + nn, kappaLR, sizeof(cplx) * nn, hipMemcpyHostToDevice, c->stream));
+  TG_HIP(hipMemcpyAsync(w + 2 * nn, kappaRL, sizeof(cplx) * nn, hipMemcpyHostToDevice, c->stream));
+  TG_TRY(contraction_run(c, 1, w, w + nn, w + 2 * nn, w + 3 * nn, w + 5 * nn, w + 6 * nn, 0));
+  hipLaunchKernelGGL(add_h1_kernel, dim3(nblk(nn)), dim3(NT), 0, c->stream, (long long)nn, 1,
+                     c->d_H1.as<cplx>(), w + 3 * nn, w + 4 * nn);
+  TG_HIP(hipGetLastError());
+  TG_HIP(hipStreamSynchronize(c->stream));
+  TG_HIP(hipMemcpy(gammaLR, w + 3 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(hspLR, w + 4 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(deltaLR, w + 5 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(deltaRL, w + 6 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
+  return TG_OK;
+}
+
+int tg_calculate_fields_diag(tg_ctx* c, const double* rhoLR, const double* kappaLR,
+                             double* gammaLR, double* hspLR, double* deltaLR, double* deltaRL) {
+  if (!c || !rhoLR || !kappaLR || !gammaLR || !hspLR || !deltaLR) {
+    tg_set_error("tg_calculate_fields_diag: null argument");
+    return TG_E_ARG;
+  }
+  const int n = c->n;
+  const size_t nn = (size_t)n * n;
+  TG_TRY(c->d_work.ensure(sizeof(cplx) * 8 * nn));
+  cplx* w = c->d_work.as<cplx>();
+  TG_HIP(hipMemcpyAsync(w, rhoLR, sizeof(cplx) * nn, hipMemcpyHostToDevice, c->stream));
+  TG_HIP(hipMemcpyAsync(w + nn, kappaLR, sizeof(cplx) * nn, hipMemcpyHostToDevice, c->stream));
+  TG_TRY(contraction_run(c, 1, w, w + nn, nullptr, w + 3 * nn, w + 5 * nn, w + 6 * nn, 1));
+  hipLaunchKernelGGL(add_h1_kernel, dim3(nblk(nn)), dim3(NT), 0, c->stream, (long long)nn, 1,
+                     c->d_H1.as<cplx>(), w + 3 * nn, w + 4 * nn);
+  TG_HIP(hipGetLastError());
+  TG_HIP(hipStreamSynchronize(c->stream));
+  TG_HIP(hipMemcpy(gammaLR, w + 3 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(hspLR, w + 4 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(deltaLR, w + 5 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
+  if (deltaRL) TG_HIP(hipMemcpy(deltaRL, w + 6 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
+  return TG_OK;
+}
+
+int tg_zskpfa(int n, int count, const double* A, int lda, double* pfaff, int* info) {
+  if (n < 0 || count < 0 || lda < std::max(1, n) || (count > 0 && (!A || !pfaff || !info))) {
+    tg_set_error("tg_zskpfa: invalid argument");
+    return TG_E_ARG;
+  }
+  if (count == 0) return TG_OK;
+  if (n == 0 || n % 2 == 1) {
+    for (int k = 0; k < count; ++k) {
+      pfaff[2 * k] = (n == 0) ? 1.0 : 0.0;
+      pfaff[2 * k + 1] = 0.0;
+      info[k] = 0;
+    }
+    return TG_OK;
+  }
+  TG_TRY(get_device_ready());
+  const size_t m = (size_t)lda * n;
+  cplx *dA = nullptr, *dW = nullptr, *dPh = nullptr;
+  double* dL = nullptr;
+  int* dZ = nullptr;
+  int r = TG_OK;
+  if (hipMalloc(&dA, sizeof(cplx) * m * count) != hipSuccess ||
+      hipMalloc(&dW, sizeof(cplx) * (size_t)n * n * count) != hipSuccess ||
+      hipMalloc(&dPh, sizeof(cplx) * count) != hipSuccess ||
+      hipMalloc(&dL, sizeof(double) * count) != hipSuccess ||
+      hipMalloc(&dZ, sizeof(int) * count) != hipSuccess) {
+    tg_set_error("tg_zskpfa: hipMalloc failed");
+    r = TG_E_NOMEM;
+  }
+  if (r == TG_OK && hipMemcpy(dA, A, sizeof(cplx) * m * count, hipMemcpyHostToDevice) != hipSuccess)
+    r = TG_E_HIP;
+  if (r == TG_OK) r = batched_pfaffian(0, n, dA, lda, (long long)m, count, dW, dL, dPh, dZ);
+  std::vector<double> L(count);
+  std::vector<cplx> Ph(count);
+  std::vector<int> Zr(count);
+  if (r == TG_OK && (hipDeviceSynchronize() != hipSuccess ||
+                     hipMemcpy(L.data(), dL, sizeof(double) * count, hipMemcpyDeviceToHost) != hipSuccess ||
+                     hipMemcpy(Ph.data(), dPh, sizeof(cplx) * count, hipMemcpyDeviceToHost) != hipSuccess ||
+                     hipMemcpy(Zr.data(), dZ, sizeof(int) * count, hipMemcpyDeviceToHost) != hipSuccess))
+    r = TG_E_HIP;
+  if (r == TG_OK)
+    for (int k = 0; k < count; ++k) {
+      const double mag = Zr[k] ? 0.0 : exp(L[k]);
+      pfaff[2 * k] = mag * Ph[k].x;
+      pfaff[2 * k + 1] = mag * Ph[k].y;
+      info[k] = 0;
+    }
+  if (dA) (void)hipFree(dA);
+  if (dW) (void)hipFree(dW);
+  if (dPh) (void)hipFree(dPh);
+  if (dL) (void)hipFree(dL);
+  if (dZ) (void)hipFree(dZ);
+  if (r == TG_E_HIP) tg_set_error("tg_zskpfa: HIP error");
+  return r;
+}
+
+int tg_calculate_overlap(int ndim, int nocc0, int nemp0, double ovac0, const double* voveru0,
+                         const double* D0, int nocc1, int nemp1, double ovac1,
+                         const double* voveru1, const double* D1, const double* ROT,
+                         double* overlap) {
+  const int n = ndim;
+  if (n <= 0 || !D0 || !D1 || !ROT || !overlap || nocc0 < 0 || nocc1 < 0 || nemp0 < 0 ||
+      nemp1 < 0 || nocc0 + nemp0 > n || nocc1 + nemp1 > n) {
+    tg_set_error("tg_calculate_overlap: invalid argument");
+    return TG_E_ARG;
+  }
+  // dimensions and phase factor (:803-818)
+  const int nempm = std::min(nemp0, nemp1);
+  const int nbloc0 = n - nocc0 - nempm, nbloc1 = n - nocc1 - nempm;
+  const int nreg = nbloc0 + nbloc1;
+  const int n0 = n - nocc0, n1 = n - nocc1, n2 = n - nempm;
+  const long long e1 = (long long)n2 * (n2 + 1) / 2;
+  const long long e2 = (long long)nocc0 * (nocc0 - 1) / 2;
+  const long long e3 = (long long)nocc0 * (nocc0 + 1) / 2 + (long long)nocc1 * (nocc1 + 1) / 2 +
+                       (long long)nocc0 * (n0 + n2) + (long long)nocc1 * n1;
+  const double signfac = sign_pow(e1) * sign_pow(e2) * sign_pow(e3);
+  const double normvac = 0.5 * (ovac0 + ovac1);
+  const int isla0 = (n0 - nemp0 == 0) ? 1 : 0, isla1 = (n1 - nemp1 == 0) ? 2 : 0;
+  cplx ov = cmk(0, 0);
+  switch (isla0 + isla1) {
+    case 1: if (nocc0 > n - nemp1) { overlap[0] = overlap[1] = 0; return TG_OK; } break;
+    case 2: if (nocc1 > n - nemp0) { overlap[0] = overlap[1] = 0; return TG_OK; } break;
+    case 3:
+      if (nocc0 == nocc1) ov = cmk(1, 0);
+      else { overlap[0] = overlap[1] = 0; return TG_OK; }
+      break;
+    default: break;
+  }
+  if ((nbloc0 > 0 && !voveru0) || (nbloc1 > 0 && !voveru1)) {
+    tg_set_error("tg_calculate_overlap: voveru missing");
+    return TG_E_ARG;
+  }
+  TG_TRY(get_device_ready());
+  const size_t nn = (size_t)n * n, m2 = (size_t)n2 * n2, mr = (size_t)nreg * nreg;
+  cplx *dD0 = nullptr, *dD1 = nullptr, *dR = nullptr, *dT = nullptr, *dRn = nullptr, *dM = nullptr,
+       *dPW = nullptr, *dDet = nullptr, *dPh = nullptr;
+  double *dL = nullptr, *dv0 = nullptr, *dv1 = nullptr;
+  int *dInfo = nullptr, *dPerm = nullptr, *dZ = nullptr;
+  int r = TG_OK;
+  auto al = [&](void** p, size_t b) {
+    if (r == TG_OK && hipMalloc(p, std::max<size_t>(b, 16)) != hipSuccess) r = TG_E_NOMEM;
+  };
+  al((void**)&dD0, sizeof(cplx) * nn); al((void**)&dD1, sizeof(cplx) * nn);
+  al((void**)&dR, sizeof(cplx) * nn); al((void**)&dT, sizeof(cplx) * nn);
+  al((void**)&dRn, sizeof(cplx) * nn); al((void**)&dM, sizeof(cplx) * mr);
+  al((void**)&dPW, sizeof(cplx) * mr); al((void**)&dDet, sizeof(cplx));
+  al((void**)&dPh, sizeof(cplx)); al((void**)&dL, sizeof(double));
+  al((void**)&dv0, sizeof(double) * (size_t)std::max(nbloc0, 1));
+  al((void**)&dv1, sizeof(double) * (size_t)std::max(nbloc1, 1));
+  al((void**)&dInfo, sizeof(int)); al((void**)&dPerm, sizeof(int) * (size_t)n2 + 16);
+  al((void**)&dZ, sizeof(int));
+  hipStream_t s = 0;
+  const cplx one = cmk(1, 0), zero = cmk(0, 0);
+  if (r == TG_OK) {
+    if (hipMemcpy(dD0, D0, sizeof(cplx) * nn, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dD1, D1, sizeof(cplx) * nn, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dR, ROT, sizeof(cplx) * nn, hipMemcpyHostToDevice) != hipSuccess ||
+        (nbloc0 > 0 && hipMemcpy(dv0, voveru0 + nempm, sizeof(double) * nbloc0, hipMemcpyHostToDevice) != hipSuccess) ||
+        (nbloc1 > 0 && hipMemcpy(dv1, voveru1 + nempm, sizeof(double) * nbloc1, hipMemcpyHostToDevice) != hipSuccess))
+      r = TG_E_HIP;
+  }
+  // Rn = D0^H (ROT D1); R2 = Rn(nempm:, nempm:)
+  if (r == TG_OK) r = gemm_zz(s, OP_N, OP_N, n, n, n, one, dR, n, 0, dD1, n, 0, zero, dT, n, 0, 1);
+  if (r == TG_OK) r = gemm_zz(s, OP_C, OP_N, n2, n2, n, one, dD0 + (size_t)nempm * n, n, 0,
+                              dT + (size_t)nempm * n, n, 0, zero, dRn, n2, 0, 1);
+  if (r == TG_OK) r = batched_inverse(s, n2, dRn, n2, 0, 1, dDet, dInfo, dPerm);
+  int info = 0;
+  cplx det = cmk(0, 0);
+  if (r == TG_OK && (hipDeviceSynchronize() != hipSuccess ||
+                     hipMemcpy(&info, dInfo, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+                     hipMemcpy(&det, dDet, sizeof(cplx), hipMemcpyDeviceToHost) != hipSuccess))
+    r = TG_E_HIP;
+  if (r == TG_OK && info != 0) {
+    tg_set_error("In calculate_overlap got info = " + std::to_string(info) + " from zgetrf");
+    r = TG_E_SINGULAR;
+  }
+  if (r == TG_OK && (isla0 + isla1) != 3) {
+    if (nreg > 0 && nreg % 2 == 0) {
+      hipLaunchKernelGGL(build_mreg_kernel, dim3(nblk((long long)mr)), dim3(NT), 0, s, n2, nbloc0,
+                         nbloc1, nreg, 1, dv0, dv1, dRn, dM);
+      if (hipGetLastError() != hipSuccess) r = TG_E_HIP;
+      if (r == TG_OK) r = batched_pfaffian(s, nreg, dM, nreg, 0, 1, dPW, dL, dPh, dZ);
+      double la = 0;
+      cplx ph = cmk(0, 0);
+      int z = 0;
+      if (r == TG_OK && (hipDeviceSynchronize() != hipSuccess ||
+                         hipMemcpy(&la, dL, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+                         hipMemcpy(&ph, dPh, sizeof(cplx), hipMemcpyDeviceToHost) != hipSuccess ||
+                         hipMemcpy(&z, dZ, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess))
+        r = TG_E_HIP;
+      if (r == TG_OK) {
+        if (z) {
+          ov = cmk(0, 0);
+        } else {
+          const double m = exp(la + normvac);
+          const cplx d = cscale(det, signfac);
+          ov = cscale(cmul(d, ph), m);
+        }
+      }
+    } else {
+      ov = cmk((nreg == 0) ? 1.0 : 0.0, 0.0);
+      if (nreg == 0) ov = cscale(cmul(cscale(det, signfac), ov), exp(normvac));
+    }
+  } else if (r == TG_OK) {
+    // both Slater determinants: overlap = detR * signfac * exp(normvac)
+    ov = cscale(cscale(det, signfac), exp(normvac));
+  }
+  for (void* p : {(void*)dD0, (void*)dD1, (void*)dR, (void*)dT, (void*)dRn, (void*)dM, (void*)dPW,
+                  (void*)dDet, (void*)dPh, (void*)dL, (void*)dv0, (void*)dv1, (void*)dInfo,
+                  (void*)dPerm, (void*)dZ})
+    if (p) (void)hipFree(p);
+  if (r == TG_E_HIP) tg_set_error("tg_calculate_overlap: HIP error");
+  if (r == TG_E_NOMEM) tg_set_error("tg_calculate_overlap: hipMalloc failed");
+  if (r == TG_OK) {
+    overlap[0] = ov.x;
+    overlap[1] = ov.y;
+  }
+  return r;
+}
+
+int tg_set_state(tg_ctx* c, const double* U0, const double* V0, const double* D0, int nocc0,
+                 int nemp0, double ovac0, const double* voveru0, int Mphip, int Mphin,
+                 double facpi, double valence_Z, double valence_N, double factor_delta) {
+  if (!c) return TG_E_ARG;
+  return upload_state(c, U0, V0, D0, nocc0, nemp0, ovac0, voveru0, Mphip, Mphin, facpi, valence_Z,
+                      valence_N, factor_delta);
+}
+
+int tg_project_resident(tg_ctx* c) {
+  if (!c || !c->have_state) {
+    tg_set_error("tg_project_resident: no state");
+    return TG_E_ARG;
+  }
+  c->t_total = c->t_contr = c->t_gemm = c->t_pf = c->t_inv = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing) {
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, c->stream);
+  }
+  int r = project_device(c);
+  if (c->timing) {
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    c->t_total = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  return r;
+}
+
+int tg_sync(tg_ctx* c) {
+  if (!c) return TG_E_ARG;
+  TG_HIP(hipStreamSynchronize(c->stream));
+  return TG_OK;
+}
+
+int tg_get_results(tg_ctx* c, double* pnp_scalars, double* pnp_A, double* pnp_H20,
+                   double* pnp_AH20, double* field_H20v, double* imag_grad) {
+  if (!c || !c->have_state) return TG_E_ARG;
+  return fetch_results(c, pnp_scalars, pnp_A, pnp_H20, pnp_AH20, field_H20v, imag_grad);
+}
+
+int tg_project_wavefunction(tg_ctx* c, const double* U0, const double* V0, const double* D0,
+                            int nocc0, int nemp0, double ovac0, const double* voveru0, int Mphip,
+                            int Mphin, double facpi, double valence_Z, double valence_N,
+                            double factor_delta, double* pnp_scalars, double* pnp_A,
+                            double* pnp_H20, double* pnp_AH20, double* field_H20v,
+                            double* imag_grad) {
+  if (!c) return TG_E_ARG;
+  TG_TRY(upload_state(c, U0, V0, D0, nocc0, nemp0, ovac0, voveru0, Mphip, Mphin, facpi, valence_Z,
+                      valence_N, factor_delta));
+  TG_TRY(tg_project_resident(c));
+  return fetch_results(c, pnp_scalars, pnp_A, pnp_H20, pnp_AH20, field_H20v, imag_grad);
+}
+
+int tg_calculate_gradient(int ndim, int gradient_type, double gradient_eta, double gradient_mu,
+                          int info_H11, int info_hsp, const double* eigen_H11,
+                          const double* eigen_hsp, const double* field_H20v, int constraint_dim,
+                          const double* constraint_20, const double* lagrange_lambda1,
+                          const double* gradient_Zim1, double* gradient_Zi,
+                          double* gradient_norm) {
+  const int n = ndim;
+  if (n <= 0 || !field_H20v || !gradient_Zim1 || !gradient_Zi || !gradient_norm ||
+      constraint_dim < 0 || (constraint_dim > 0 && (!constraint_20 || !lagrange_lambda1))) {
+    tg_set_error("tg_calculate_gradient: invalid argument");
+    return TG_E_ARG;
+  }
+  int opt = gradient_type;
+  if (info_H11 != 0 && gradient_type == 1) opt = 0;
+  if (info_hsp != 0 && gradient_type == 2) opt = 0;
+  double eta = gradient_eta, mu = gradient_mu;
+  if (opt == 1 || opt == 2) {
+    const double* ev = (opt == 1) ? eigen_H11 : eigen_hsp;
+    if (!ev) {
+      tg_set_error("tg_calculate_gradient: eigenvalues missing");
+      return TG_E_ARG;
+    }
+    double amax = 0, amin = INFINITY;
+    for (int i = 0; i < n; ++i) {
+      amax = std::max(amax, fabs(ev[i]));
+      amin = std::min(amin, fabs(ev[i]));
+    }
+    const double emax = (opt == 1 ? 4.0 : 2.0) * amax, emin = 2.0 * amin;
+    const double cond = emax / emin;
+    eta = pow(2.0 / (sqrt(emax) + sqrt(emin)), 2);
+    mu = pow((sqrt(cond) - 1) / (sqrt(cond) + 1), 2);
+  }
+  // Z_i = -eta (H20 - sum lambda C20) + mu Z_{i-1}; ||Z||_F / eta
+  const size_t n2 = (size_t)n * n;
+  double sum = 0;
+  for (size_t i = 0; i < n2; ++i) {
+    double sg = 0;
+    for (int j = 0; j < constraint_dim; ++j) sg += lagrange_lambda1[j] * constraint_20[j * n2 + i];
+    const double z = -eta * (field_H20v[i] - sg) + mu * gradient_Zim1[i];
+    gradient_Zi[i] = z;
+    sum += z * z;
+  }
+  *gradient_norm = sqrt(sum) / eta;
+  return TG_OK;
+}
+
+int tg_enable_timing(tg_ctx* c, int on) {
+  if (!c) return TG_E_ARG;
+  c->timing = on ? 1 : 0;
+  return TG_OK;
+}
+
+int tg_get_timing(tg_ctx* c, double* ms_total, double* ms_contraction, double* ms_gemm,
+                  double* ms_pfaffian, double* ms_inverse) {
+  if (!c) return TG_E_ARG;
+  if (ms_total) *ms_total = c->t_total;
+  if (ms_contraction) *ms_contraction = c->t_contr;
+  if (ms_gemm) *ms_gemm = c->t_gemm;
+  if (ms_pfaffian) *ms_pfaffian = c->t_pf;
+  if (ms_inverse) *ms_inverse = c->t_inv;
+  return TG_OK;
+}
+
+}  // extern "C"

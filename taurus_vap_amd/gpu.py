@@ -1,0 +1,260 @@
+"""Host-side binding of the gfx950 library (libtaurus_gpu.so) through its C ABI.
+
+The functions mirror the reference routines they replace -- same names,
+argument meaning and error behaviour (a nonzero status raises, where the
+reference stops) -- so tests and drivers read like the reference's own code:
+
+  set_hamiltonian storage        -> Context(...)          (tg_init)
+  calculate_fields               -> Context.calculate_fields
+  calculate_fields_diag          -> Context.calculate_fields_diag
+  ZSKPFA('U','P')                -> zskpfa
+  calculate_overlap              -> calculate_overlap
+  project_wavefunction(iopt=0)   -> Context.project_wavefunction
+  calculate_gradient             -> calculate_gradient
+
+There is no CPU fallback: importing works anywhere, but every call needs the
+built library and a HIP device and raises otherwise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.environ.get("TAURUS_GPU_LIB", os.path.join(HERE, "libtaurus_gpu.so"))
+
+TG_FLAG_PLAIN = 1
+TG_FLAG_TBME_FP32 = 2
+
+EXPORTS = [
+    "tg_last_error", "tg_init", "tg_finalize", "tg_set_mode", "tg_calculate_fields",
+    "tg_calculate_fields_diag", "tg_zskpfa", "tg_calculate_overlap",
+    "tg_project_wavefunction", "tg_set_state", "tg_project_resident", "tg_sync",
+    "tg_get_results", "tg_calculate_gradient", "tg_enable_timing", "tg_get_timing",
+]
+
+_lib = None
+
+
+class TaurusGPUError(RuntimeError):
+    pass
+
+
+def load_library():
+    """Loads libtaurus_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise TaurusGPUError("libtaurus_gpu.so not built (run taurus_vap_amd.build)")
+    lib = ctypes.CDLL(LIBPATH)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    D = ctypes.c_double
+    lib.tg_last_error.restype = ctypes.c_char_p
+    lib.tg_init.argtypes = [ctypes.POINTER(P), I, P, P, P, P, D, D, P, ctypes.c_int64, P, P, I, P, I]
+    lib.tg_finalize.argtypes = [P]
+    lib.tg_set_mode.argtypes = [P, I]
+    lib.tg_calculate_fields.argtypes = [P] * 8
+    lib.tg_calculate_fields_diag.argtypes = [P] * 7
+    lib.tg_zskpfa.argtypes = [I, I, P, I, P, P]
+    lib.tg_calculate_overlap.argtypes = [I, I, I, D, P, P, I, I, D, P, P, P, P]
+    lib.tg_project_wavefunction.argtypes = [P, P, P, P, I, I, D, P, I, I, D, D, D, D, P, P, P, P, P, P]
+    lib.tg_set_state.argtypes = [P, P, P, P, I, I, D, P, I, I, D, D, D, D]
+    lib.tg_project_resident.argtypes = [P]
+    lib.tg_sync.argtypes = [P]
+    lib.tg_get_results.argtypes = [P, P, P, P, P, P, P]
+    lib.tg_calculate_gradient.argtypes = [I, I, D, D, I, I, P, P, P, I, P, P, P, P, P]
+    lib.tg_enable_timing.argtypes = [P, I]
+    lib.tg_get_timing.argtypes = [P, P, P, P, P, P]
+    _lib = lib
+    return lib
+
+
+def _check(status: int, what: str):
+    if status != 0:
+        msg = load_library().tg_last_error().decode(errors="replace")
+        raise TaurusGPUError("%s failed (%d): %s" % (what, status, msg))
+
+
+def _c(a, dtype):
+    """Contiguous column-major copy with the given dtype."""
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """Device context: basis + Hamiltonian uploaded once (tg_init)."""
+
+    def __init__(self, ndim, sp_2mj, sp_2mt, sp_l, sp_tr, H0, H1, abcd, H2, trperm,
+                 plain=False, tbme_fp32=None):
+        lib = load_library()
+        self.n = int(ndim)
+        H2 = np.ascontiguousarray(H2)
+        if H2.dtype not in (np.float32, np.float64):
+            raise ValueError("H2 must be float32 or float64")
+        if tbme_fp32 is None:
+            tbme_fp32 = H2.dtype == np.float32
+        flags = (TG_FLAG_PLAIN if plain else 0) | (TG_FLAG_TBME_FP32 if tbme_fp32 else 0)
+        self._keep = [np.ascontiguousarray(x, dtype=np.int32) for x in (sp_2mj, sp_2mt, sp_l, sp_tr)]
+        H1 = _c(H1, np.complex128)
+        abcd = np.ascontiguousarray(np.asarray(abcd, dtype=np.int16).reshape(-1))
+        trperm = np.ascontiguousarray(trperm, dtype=np.int8)
+        ntb = trperm.size
+        if abcd.size != 4 * ntb or H2.size != ntb:
+            raise ValueError("abcd / H2 / trperm sizes disagree")
+        h = ctypes.c_void_p()
+        H0 = complex(H0)
+        _check(lib.tg_init(ctypes.byref(h), self.n, *[_p(x) for x in self._keep], H0.real, H0.imag,
+                           _p(H1), ntb, _p(abcd), _p(H2), H2.itemsize, _p(trperm), flags), "tg_init")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().tg_finalize(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_mode(self, plain: bool):
+        _check(load_library().tg_set_mode(self._h, 1 if plain else 0), "tg_set_mode")
+
+    # calculate_fields (src/module_fields.f90:99-189)
+    def calculate_fields(self, rhoLR, kappaLR, kappaRL):
+        n = self.n
+        ins = [_c(x, np.complex128) for x in (rhoLR, kappaLR, kappaRL)]
+        outs = [np.zeros((n, n), np.complex128, order="F") for _ in range(4)]
+        _check(load_library().tg_calculate_fields(self._h, *[_p(x) for x in ins],
+                                                  *[_p(x) for x in outs]), "tg_calculate_fields")
+        return tuple(outs)   # gammaLR, hspLR, deltaLR, deltaRL
+
+    # calculate_fields_diag (src/module_fields.f90:204-317)
+    def calculate_fields_diag(self, rhoLR, kappaLR):
+        n = self.n
+        ins = [_c(x, np.complex128) for x in (rhoLR, kappaLR)]
+        outs = [np.zeros((n, n), np.complex128, order="F") for _ in range(4)]
+        _check(load_library().tg_calculate_fields_diag(self._h, *[_p(x) for x in ins],
+                                                       *[_p(x) for x in outs]),
+               "tg_calculate_fields_diag")
+        return tuple(outs)
+
+    # project_wavefunction(iopt=0) + calculate_projected_H20 (module_projection.f90:201, 729)
+    def project_wavefunction(self, U0, V0, D0, nocc0, nemp0, ovac0, voveru0, Mphip, Mphin,
+                             facpi, valence_Z, valence_N, factor_delta=1.0):
+        n = self.n
+        U0, V0, D0 = (_c(x, np.float64) for x in (U0, V0, D0))
+        vou = np.ascontiguousarray(np.asarray(voveru0, dtype=np.float64).reshape(-1))
+        if vou.size == 0:
+            vou = np.zeros(1)
+        scal = np.zeros(6, np.complex128)
+        pA, pH, pAH = (np.zeros((n, n), np.complex128, order="F") for _ in range(3))
+        H20v = np.zeros(n * n)
+        imag = np.zeros(1)
+        _check(load_library().tg_project_wavefunction(
+            self._h, _p(U0), _p(V0), _p(D0), int(nocc0), int(nemp0), float(ovac0), _p(vou),
+            int(Mphip), int(Mphin), float(facpi), float(valence_Z), float(valence_N),
+            float(factor_delta), _p(scal), _p(pA), _p(pH), _p(pAH), _p(H20v), _p(imag)),
+            "tg_project_wavefunction")
+        return dict(over=scal[0], ener=scal[1], prot=scal[2], neut=scal[3], prot2=scal[4],
+                    neut2=scal[5], A=pA, H20=pH, AH20=pAH, H20v=H20v, imag=float(imag[0]))
+
+    # resident-state API used for timing
+    def set_state(self, U0, V0, D0, nocc0, nemp0, ovac0, voveru0, Mphip, Mphin, facpi,
+                  valence_Z, valence_N, factor_delta=1.0):
+        U0, V0, D0 = (_c(x, np.float64) for x in (U0, V0, D0))
+        vou = np.ascontiguousarray(np.asarray(voveru0, dtype=np.float64).reshape(-1))
+        if vou.size == 0:
+            vou = np.zeros(1)
+        self._state_keep = (U0, V0, D0, vou)
+        _check(load_library().tg_set_state(self._h, _p(U0), _p(V0), _p(D0), int(nocc0), int(nemp0),
+                                           float(ovac0), _p(vou), int(Mphip), int(Mphin),
+                                           float(facpi), float(valence_Z), float(valence_N),
+                                           float(factor_delta)), "tg_set_state")
+
+    def project_resident(self):
+        _check(load_library().tg_project_resident(self._h), "tg_project_resident")
+
+    def sync(self):
+        _check(load_library().tg_sync(self._h), "tg_sync")
+
+    def get_results(self):
+        n = self.n
+        scal = np.zeros(6, np.complex128)
+        H20v = np.zeros(n * n)
+        imag = np.zeros(1)
+        _check(load_library().tg_get_results(self._h, _p(scal), None, None, None, _p(H20v), _p(imag)),
+               "tg_get_results")
+        return dict(over=scal[0], ener=scal[1], prot=scal[2], neut=scal[3], H20v=H20v,
+                    imag=float(imag[0]))
+
+    def enable_timing(self, on=True):
+        _check(load_library().tg_enable_timing(self._h, 1 if on else 0), "tg_enable_timing")
+
+    def timing(self):
+        v = [ctypes.c_double() for _ in range(5)]
+        _check(load_library().tg_get_timing(self._h, *[ctypes.byref(x) for x in v]), "tg_get_timing")
+        return dict(zip(("total", "contraction", "gemm", "pfaffian", "inverse"), [x.value for x in v]))
+
+
+def zskpfa(A):
+    """ZSKPFA('U','P') on one matrix or a stack (count, n, n); returns pf (and info)."""
+    A = np.asarray(A, dtype=np.complex128)
+    single = A.ndim == 2
+    if single:
+        A = A[None]
+    cnt, n, _ = A.shape
+    buf = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))   # column-major per matrix
+    pf = np.zeros(cnt, np.complex128)
+    info = np.zeros(cnt, np.int32)
+    _check(load_library().tg_zskpfa(n, cnt, _p(buf), max(1, n), _p(pf), _p(info)), "tg_zskpfa")
+    return (pf[0], int(info[0])) if single else (pf, info)
+
+
+def calculate_overlap(nocc0, nemp0, ovac0, voveru0, D0, nocc1, nemp1, ovac1, voveru1, D1, ROT):
+    """calculate_overlap (src/module_projection.f90:782-937)."""
+    n = np.asarray(D0).shape[0]
+    D0, D1, ROT = (_c(x, np.complex128) for x in (D0, D1, ROT))
+    v0 = np.ascontiguousarray(np.asarray(voveru0, dtype=np.float64).reshape(-1))
+    v1 = np.ascontiguousarray(np.asarray(voveru1, dtype=np.float64).reshape(-1))
+    if v0.size == 0:
+        v0 = np.zeros(1)
+    if v1.size == 0:
+        v1 = np.zeros(1)
+    out = np.zeros(1, np.complex128)
+    _check(load_library().tg_calculate_overlap(n, int(nocc0), int(nemp0), float(ovac0), _p(v0), _p(D0),
+                                               int(nocc1), int(nemp1), float(ovac1), _p(v1), _p(D1),
+                                               _p(ROT), _p(out)), "tg_calculate_overlap")
+    return complex(out[0])
+
+
+def calculate_gradient(field_H20v, constraint_20, lagrange_lambda1, gradient_Zim1,
+                       gradient_type, gradient_eta, gradient_mu, eigen_H11=None, eigen_hsp=None,
+                       info_H11=0, info_hsp=0):
+    """calculate_gradient (src/module_gradient.f90:79-143); returns (Zi, norm)."""
+    H = np.ascontiguousarray(field_H20v, dtype=np.float64)
+    n = int(round(np.sqrt(H.size)))
+    lam = np.ascontiguousarray(np.asarray(lagrange_lambda1 if lagrange_lambda1 is not None else [],
+                                          dtype=np.float64).reshape(-1))
+    ncons = lam.size
+    c20 = (np.ascontiguousarray(constraint_20, dtype=np.float64).reshape(-1) if ncons else np.zeros(1))
+    if ncons == 0:
+        lam = np.zeros(1)
+    Zm = np.ascontiguousarray(gradient_Zim1, dtype=np.float64)
+    Zi = np.zeros(n * n)
+    norm = np.zeros(1)
+    e11 = None if eigen_H11 is None else np.ascontiguousarray(eigen_H11, dtype=np.float64)
+    ehs = None if eigen_hsp is None else np.ascontiguousarray(eigen_hsp, dtype=np.float64)
+    _check(load_library().tg_calculate_gradient(n, int(gradient_type), float(gradient_eta),
+                                                float(gradient_mu), int(info_H11), int(info_hsp),
+                                                _p(e11), _p(ehs), _p(H), ncons, _p(c20), _p(lam),
+                                                _p(Zm), _p(Zi), _p(norm)), "tg_calculate_gradient")
+    return Zi, float(norm[0])

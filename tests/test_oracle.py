@@ -74,19 +74,33 @@ def test_own_canonical_basis(case):
     assert abs(r["over"] - g["pnp_over"]) < 1e-9 * abs(g["pnp_over"])
 
 
+def eigen_for_gradient(g):
+    """diagonalize_hsp_and_H11(0) restated (module_gradient.f90:158-453) on the
+    fixture's state; used when gradient_type > 0."""
+    ncons = int(g["constraint_dim"])
+    return P.diagonalize_hsp_and_H11(
+        g["U0"], g["V0"], golden_ham(g), _basis(g), g["constraint_HO"] if ncons else None,
+        g["lagrange_lambda1"] if ncons else [], ncons, int(g["constraint_pair"]),
+        float(g["factor_delta"]), int(g["Mphip"]))
+
+
 def test_gradient_step_matches_reference():
+    """calculate_gradient, with the heavy-ball recipe of gradient_type 1 fed by
+    the restated hsp/H11 diagonalisation for the Mg24 example."""
     for case in ("sd64", "pf64", "mg24_32"):
         g = load_golden(case)
         ncons = int(g["constraint_dim"])
         lam = g["lagrange_lambda1"] if ncons else []
         c20 = g["constraint_20"] if ncons else None
-        if int(g["gradient_type"]) != 0:
-            continue
         n = int(g["ndim"])
-        Zi, norm, eta, mu = P.calculate_gradient(g["field_H20v"], c20, lam, np.zeros(n * n), 0,
-                                                 float(g["gradient_eta"]), float(g["gradient_mu"]))
+        ehsp, eH11 = eigen_for_gradient(g)
+        Zi, norm, eta, mu = P.calculate_gradient(g["field_H20v"], c20, lam, np.zeros(n * n),
+                                                 int(g["gradient_type"]), float(g["gradient_eta"]),
+                                                 float(g["gradient_mu"]), eigen_H11=eH11,
+                                                 eigen_hsp=ehsp)
         assert np.max(np.abs(Zi - g["gradient_Zi"])) < 1e-13
         assert abs(norm - float(g["gradient_norm"])) < 1e-10 * float(g["gradient_norm"])
+    assert int(load_golden("mg24_32")["gradient_type"]) == 1
 
 
 # ---------------------------------------------------------------------------
