@@ -1,0 +1,266 @@
+#!/usr/bin/env python3
+"""PNVAP gradient iterations/s on one MI355X.
+
+Workload (default): e_max = 6 harmonic-oscillator basis (336 single-particle
+states), 15 x 15 Fomenko gauge angles, 10 protons / 12 neutrons, seeded
+synthetic JT-coupled interaction V ~ N(0, 2 MeV) exp(-(Na+Nb+Nc+Nd)/4)
+(12.9M stored m-scheme elements), SPEs hbar*omega (N + 3/2), seed state a real
+Thouless rotation (Z ~ N(0, 0.1)) of the lowest-filled Slater determinant.
+TBMEs stored in FP32, all arithmetic FP64.
+
+One step = one projected-gradient iteration of the hot path with the state
+resident in HBM: the gauge-angle loop of project_wavefunction over all 225
+angles (Thouless matrices, transition densities, TBME contraction, overlaps,
+energies, H20), sum_gauge, calculate_projected_H20 and the gradient step.
+
+Prints one JSON line.  --gpus N > 1 is not supported (one GPU per run).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PNVAP gradient iters/s (e_max=6, 15x15 angles) vs plain FP64 GPU, |dE|<1 keV"
+PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_flops(ctx_info, n, A, nb):
+    """Algorithmic FP64 flops per launch of the timed kernels (real flops;
+    complex multiply-add = 8, real x complex = 4)."""
+    return {
+        "kernel:grouped_ph": 4.0 * ctx_info["wph"] * A,
+        "kernel:grouped_pp": 4.0 * ctx_info["wpp"] * 2 * A,
+        "kernel:gj_inverse": 8.0 * n ** 3 * A,
+        "kernel:pfaffian": (4.0 / 3.0) * nb ** 3 * A,
+    }
+
+
+def contraction_sizes(h):
+    """sum of s_g^2 of the ph and pp block channels (as built by tg_init)."""
+    from collections import Counter
+    b = h["basis"]
+    m, t, l = b.sp_2mj.astype(int), b.sp_2mt.astype(int), b.sp_l.astype(int)
+    n = b.dim
+    x, y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    ph = Counter(zip((m[x] - m[y]).ravel(), (t[x] - t[y]).ravel(), ((l[x] + l[y]) & 1).ravel()))
+    sel = x > y
+    pp = Counter(zip((m[x] + m[y])[sel], (t[x] + t[y])[sel], ((l[x] + l[y]) & 1)[sel]))
+    return dict(wph=float(sum(v * v for v in ph.values())), wpp=float(sum(v * v for v in pp.values())))
+
+
+def run_gpu(args, ham, plain, fp32, steps, warmup, timing=True):
+    from taurus_vap_amd import host
+    ctx, h, w, U, V = host.workload_context(args.workload, seed=args.seed, tbme_fp32=fp32,
+                                            plain=plain, ham=ham)
+    cb = host.construct_canonical_basis(U, V)
+    n = h["basis"].dim
+    M = args.angles if args.angles else w.M
+    ctx.set_state(U, V, cb["D"], cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"], M, M,
+                  2.0 * math.pi, float(w.Z), float(w.N))
+    # gradient step with two constraint arrays (the shape of Q20 + Jx)
+    rng = np.random.default_rng(args.seed + 7)
+    c20 = rng.normal(size=(2, n * n)) * 0.01
+    ctx.set_gradient(args.eta, args.mu, c20, [0.05, -0.02])
+    for _ in range(warmup):
+        ctx.project_resident()
+    ctx.sync()
+    if timing:
+        ctx.enable_timing(True)
+        ctx.reset_timing()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.project_resident()
+    ctx.sync()
+    t1 = time.perf_counter()
+    res = ctx.get_results()
+    Zi, gnorm = ctx.get_gradient()
+    kt = {}
+    if timing:
+        for key in ("total", "kernel:grouped_ph", "kernel:grouped_pp", "kernel:gj_inverse",
+                    "kernel:pfaffian", "contraction", "gemm", "gemm_h20", "overlap",
+                    "thouless_inverse"):
+            kt[key] = ctx.timing(key)
+        ctx.enable_timing(False)
+    ctx.close()
+    E = (res["ener"] / res["over"]).real
+    nbloc = n - cb["nocc"] - cb["nemp"]
+    return dict(seconds=t1 - t0, steps=steps, E=E, Z=(res["prot"] / res["over"]).real,
+                N=(res["neut"] / res["over"]).real, gnorm=gnorm, timing=kt, n=n, A=M * M,
+                nb=nbloc, nocc=cb["nocc"], nemp=cb["nemp"], UV=(U, V))
+
+
+def cpu_baseline(args, ham, threads, UV):
+    """Reference TAURUS_vap built from its own sources (oracle/_ref, FP64 TBMEs,
+    OpenMP + threaded OpenBLAS) on the same deck, .red file and state: inside one
+    run of oracle/ref_harness.f90 ('time' mode) the reference's
+    project_wavefunction(0,...) is timed at 1x2 and 2x2 gauge angles, plus its
+    gradient step; the iteration time at 15x15 angles is the linear fit in the
+    angle count.  The reference's own set-up (reading the 220 MB .red file,
+    ~1 min) is not timed."""
+    import resource
+    exe = os.path.join(ROOT, "oracle", "_ref", "harness64.exe")
+    if not os.path.exists(exe):
+        log("reference binary not built: cpu_baseline = null")
+        return None
+    from taurus_vap_amd import inputs as I, hamiltonian as H
+    w = I.WORKLOADS[args.workload]
+    A = (args.angles or w.M) ** 2
+
+    def big_stack():
+        soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+    with tempfile.TemporaryDirectory() as td:
+        I.write_workload(w, td, seed=args.seed, hamil_read=1, iters=1, eta=args.eta, mu=args.mu,
+                         grad_eps=1e-14, UV=UV)
+        H.write_red(os.path.join(td, w.name + ".red"), "synthetic %s seed %d" % (w.name, args.seed),
+                    1, ham["H0"], ham["H1"], ham["abcd"], ham["H2"], ham["trperm"], rh2_bytes=8)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OPENBLAS_NUM_THREADS=str(threads),
+                   OMP_STACKSIZE="256M")
+        t0 = time.perf_counter()
+        with open(os.path.join(td, "input.txt")) as fin, open(os.path.join(td, "out.txt"), "w") as fout:
+            r = subprocess.run([exe, "time", "times.txt", "2"], stdin=fin, stdout=fout,
+                               stderr=subprocess.STDOUT, cwd=td, env=env, timeout=1500,
+                               preexec_fn=big_stack)
+        log("reference harness run: %.1f s (rc %d)" % (time.perf_counter() - t0, r.returncode))
+        if r.returncode != 0:
+            return None
+        rows = np.loadtxt(os.path.join(td, "times.txt"), ndmin=2)
+    best = {}
+    for k, a, tp, tg, e in rows:
+        a = int(a)
+        best[a] = min(best.get(a, (1e30, 1e30))[0], tp), min(best.get(a, (1e30, 1e30))[1], tg)
+    (t2, g2), (t4, g4) = best[2], best[4]
+    per_angle = max((t4 - t2) / 2.0, 1e-9)
+    fixed = max(t2 - 2.0 * per_angle, 0.0)
+    t_iter = fixed + A * per_angle + min(g2, g4)
+    return dict(value=1.0 / t_iter, unit="iterations/s", cores=threads, kind="reference",
+                sample="reference project_wavefunction timed at 1x2 (%.2f s) and 2x2 (%.2f s) gauge "
+                       "angles + gradient step (%.3f s), linear fit %.3f s + %.3f s/angle "
+                       "extrapolated to %d angles" % (t2, t4, min(g2, g4), fixed, per_angle, A))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="emax6")
+    ap.add_argument("--angles", type=int, default=0, help="gauge angles per species (default: workload)")
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--eta", type=float, default=0.005)
+    ap.add_argument("--mu", type=float, default=0.3)
+    ap.add_argument("--mode", choices=["batched", "plain"], default="batched")
+    ap.add_argument("--tbme", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--plain-steps", type=int, default=1,
+                    help="timed steps of the plain FP64 per-angle path (0: skip)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the batched gauge-angle path is measured on a "
+              "single MI355X (--gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from taurus_vap_amd import hamiltonian as H, inputs as I
+    w = I.WORKLOADS[args.workload]
+    t = time.perf_counter()
+    ham = H.workload_hamiltonian(w, args.seed)
+    log("hamiltonian %s: n=%d, %d stored TBMEs (%.1f s)" % (args.workload, ham["basis"].dim,
+                                                           len(ham["trperm"]), time.perf_counter() - t))
+    sizes = contraction_sizes(ham)
+
+    main_run = run_gpu(args, ham, plain=(args.mode == "plain"), fp32=(args.tbme == "fp32"),
+                       steps=args.steps, warmup=args.warmup)
+    ips = main_run["steps"] / main_run["seconds"]
+    log("optimized: %.3f it/s, E = %.9f MeV, <Z> = %.6f, <N> = %.6f, |grad| = %.4e"
+        % (ips, main_run["E"], main_run["Z"], main_run["N"], main_run["gnorm"]))
+    for k, (ms, c) in main_run["timing"].items():
+        if c:
+            log("  %-22s %9.3f ms/step  (%d intervals)" % (k, ms / main_run["steps"], c))
+
+    # dominant timed kernel and its roofline
+    n, A, nb = main_run["n"], main_run["A"], main_run["nb"]
+    fl = algorithmic_flops(sizes, n, A, nb)
+    kern = {k: v for k, v in main_run["timing"].items() if k.startswith("kernel:") and v[1] > 0}
+    roofline = None
+    if kern:
+        dom = max(kern, key=lambda k: kern[k][0])
+        ms, cnt = kern[dom]
+        avg_s = ms / cnt / 1e3
+        achieved = fl[dom] / avg_s / 1e12
+        roofline = {"bound": "mfma", "kernel": dom.split(":", 1)[1], "achieved": achieved,
+                    "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                    "traffic": None, "flop_per_launch": fl[dom], "avg_launch_ms": avg_s * 1e3}
+
+    # plain FP64 per-angle drop-in (denominator of the metric) and |dE| vs FP64
+    plain = None
+    if args.plain_steps > 0:
+        pr = run_gpu(args, ham, plain=True, fp32=False, steps=args.plain_steps, warmup=0, timing=False)
+        plain = dict(value=pr["steps"] / pr["seconds"], E=pr["E"])
+        log("plain FP64: %.4f it/s, E = %.9f MeV" % (plain["value"], pr["E"]))
+    else:
+        pr = run_gpu(args, ham, plain=False, fp32=False, steps=1, warmup=0, timing=False)
+        plain = dict(value=None, E=pr["E"])
+    dE_keV = abs(main_run["E"] - plain["E"]) * 1e3
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(args, ham, args.cpu_threads, main_run["UV"])
+        except Exception as e:      # a failed baseline must not hide the measurement
+            log("cpu baseline failed: %r" % (e,))
+
+    out = {
+        "metric": METRIC,
+        "value": ips,
+        "unit": "iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * main_run["seconds"] / main_run["steps"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "%s: e_max=6 HO basis (%d sp states), %dx%d gauge angles, %dp/%dn, seeded "
+                        "synthetic JT interaction (%d stored m-scheme TBMEs), TBMEs %s / FP64 "
+                        "arithmetic, state resident in HBM" % (
+                            args.workload, n, int(math.sqrt(A)), int(math.sqrt(A)), w.Z, w.N,
+                            len(ham["trperm"]), args.tbme.upper()),
+            "mode": args.mode,
+            "parallelism": "single GPU, gauge angles batched",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_fp64_gpu": None if plain["value"] is None else {"value": plain["value"],
+                                                               "unit": "iterations/s"},
+        "speedup_vs_plain_fp64_gpu": None if plain["value"] is None else ips / plain["value"],
+        "dE_vs_fp64_keV": dE_keV,
+        "E_proj_MeV": main_run["E"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

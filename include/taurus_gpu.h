@@ -156,11 +156,28 @@ int tg_calculate_gradient(int ndim, int gradient_type, double gradient_eta, doub
                           const double* lagrange_lambda1, const double* gradient_Zim1,
                           double* gradient_Zi, double* gradient_norm);
 
-/* Device-side timing of the last tg_project_resident (ms), and per-stage kernel
- * time of the dominant kernels (filled when timing is enabled). */
+/*
+ * tg_set_gradient / tg_get_gradient -- calculate_gradient with the step
+ * parameters of gradient_type 0 (src/module_gradient.f90:97-141) run on the
+ * device right after each tg_project_resident: Z = -eta (field_H20v -
+ * sum_j lambda_j constraint_20_j) + mu Z_prev, with Z_prev the previous step's
+ * Z (gradient_Zim1 = gradient_Zi, taurus_vap.f90:104); gradient_norm = ||Z||_F/eta.
+ */
+int tg_set_gradient(tg_ctx* ctx, double gradient_eta, double gradient_mu, int constraint_dim,
+                    const double* constraint_20, const double* lagrange_lambda1);
+int tg_get_gradient(tg_ctx* ctx, double* gradient_Zi, double* gradient_norm);
+
+/*
+ * Device timing: with timing enabled, every stage and the dominant kernels of
+ * tg_project_resident record HIP event pairs on the context stream (no
+ * synchronisation is added); tg_get_timing returns the summed milliseconds and
+ * the number of intervals recorded under `key` ("total", "kernel:gj_inverse",
+ * "kernel:pfaffian", "kernel:grouped_ph", "kernel:grouped_pp", "contraction",
+ * "gemm", "gemm_h20", "overlap", "thouless_inverse").
+ */
 int tg_enable_timing(tg_ctx* ctx, int on);
-int tg_get_timing(tg_ctx* ctx, double* ms_total, double* ms_contraction, double* ms_gemm,
-                  double* ms_pfaffian, double* ms_inverse);
+int tg_reset_timing(tg_ctx* ctx);
+int tg_get_timing(tg_ctx* ctx, const char* key, double* ms_sum, int* count);
 
 #ifdef __cplusplus
 }

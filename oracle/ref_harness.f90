@@ -16,6 +16,11 @@
 !                           exactly as calculate_overlap calls it              !
 !                           (module_projection.f90:921).                       !
 !   over  <infile> <outfile> calculate_overlap on given canonical data         !
+!   time  <outfile> <K>     reads the deck like iter (no initial constraint    !
+!                           adjustment), then K times: project_wavefunction    !
+!                           (0,...) at 1x2 and at 2x2 gauge angles and the     !
+!                           gradient step (calculate_lagrange_multipliers +    !
+!                           calculate_gradient); writes the seconds as text.   !
 !                                                                              !
 ! Dump format (stream, little endian): repeated records                        !
 !   char(32) name | int32 kind | int32 rank | int64 dims(rank) | data          !
@@ -116,6 +121,8 @@ select case (trim(mode))
     call run_pfaff(trim(f1), trim(f2))
   case ('over')
     call run_over(trim(f1), trim(f2))
+  case ('time')
+    call run_time(trim(f1), trim(f2))
   case default
     stop 'ref_harness: mode must be iter, pfaff or over'
 end select
@@ -409,5 +416,55 @@ enddo
 close(98)
 close(ud)
 end subroutine run_over
+
+!------------------------------------------------------------------------------!
+! Wall-clock timing of the reference's projected-gradient step                 !
+!------------------------------------------------------------------------------!
+subroutine run_time(outfile, kstr)
+character(*), intent(in) :: outfile, kstr
+integer :: iter_max, iter_write, iter_print, k, kmax, isamp, mp, mn
+integer(i64) :: c0, c1, c2, rate
+real(r64) :: tproj, tgrad
+
+read(kstr,*) kmax
+call read_input(iter_max,iter_write,iter_print)
+call set_nucleus
+call set_basis
+call set_hamiltonian
+call set_wavefunctions
+call set_fields
+call set_particle_number
+call set_pairs
+call set_angular_momentum
+call set_multipoles
+call set_radius
+call set_projection
+call generate_wavefunction(HOsp_dim)
+call check_symmetries(HOsp_dim)
+call set_constraints
+call set_gradient
+
+open(ud, file=outfile, status='replace', form='formatted')
+do k = 1, kmax
+  do isamp = 1, 2
+    mp = isamp
+    mn = 2
+    call system_clock(c0, rate)
+    field_H20v = zero
+    call calculate_densities_real(bogo_U0,bogo_V0,dens_rhoRR,dens_kappaRR,HOsp_dim)
+    call update_constraints_qpbasis
+    call project_wavefunction(0,0,mp,mn,HOsp_dim)
+    call system_clock(c1)
+    call calculate_lagrange_multipliers(field_H20v,HOsp_dim)
+    if ( gradient_type > 0 ) call diagonalize_hsp_and_H11(0,HOsp_dim)
+    call calculate_gradient(HOsp_dim)
+    call system_clock(c2)
+    tproj = real(c1-c0,r64) / real(rate,r64)
+    tgrad = real(c2-c1,r64) / real(rate,r64)
+    write(ud,'(i4,i6,2es16.8,es20.10)') k, mp*mn, tproj, tgrad, real(pnp_ener/pnp_over)
+  enddo
+enddo
+close(ud)
+end subroutine run_time
 
 end program ref_harness

@@ -1,0 +1,180 @@
+// m-scheme two-body matrix elements in the reference's storage convention.
+//
+// Restates, for JT-scheme (ANTOINE-type) interactions, the producer side of
+// the TBME list consumed by the contraction:
+//   * the m-scheme loop of set_hamiltonian_2body (src/module_hamiltonian.f90:
+//     330-378): one representative per symmetry class, a<b, c>=a, d>c (d<=b when
+//     c==a), M = m_a+m_b >= 0, parity / T_z / M conservation, cut |V| > 1e-16,
+//     stored in loop order;
+//   * uncouple_JT (:828-885): sum over J and T of N_ab^-1 N_cd^-1 CG CG CG CG V;
+//   * reconstruct_2body_timerev (:970-1018): the int8 permutation/phase code of
+//     the time-reversed element.
+// Clebsch-Gordan coefficients come from the Racah single-sum formula evaluated
+// in long double with exact factorials, tabulated once for all (j1, j2, J, m1, m2).
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/taurus_hamil.h"
+
+namespace {
+
+long double fact(int k) {
+  long double f = 1.0L;
+  for (int i = 2; i <= k; ++i) f *= (long double)i;
+  return f;
+}
+
+// <j1 m1 j2 m2 | J M>, all arguments twice their values
+double cg_racah(int j1, int j2, int J, int m1, int m2) {
+  const int M = m1 + m2;
+  if (abs(m1) > j1 || abs(m2) > j2 || abs(M) > J) return 0.0;
+  if (((j1 + m1) & 1) || ((j2 + m2) & 1) || ((J + M) & 1)) return 0.0;
+  if (J < abs(j1 - j2) || J > j1 + j2 || ((j1 + j2 + J) & 1)) return 0.0;
+  const int a = (j1 + j2 - J) / 2, b = (j1 - j2 + J) / 2, c = (-j1 + j2 + J) / 2;
+  const int d = (j1 + j2 + J) / 2 + 1;
+  long double pre = (long double)(J + 1) * fact(a) * fact(b) * fact(c) / fact(d);
+  pre *= fact((j1 + m1) / 2) * fact((j1 - m1) / 2) * fact((j2 + m2) / 2) * fact((j2 - m2) / 2) *
+         fact((J + M) / 2) * fact((J - M) / 2);
+  long double sum = 0.0L;
+  for (int k = 0; k <= 200; ++k) {
+    const int e1 = a - k, e2 = (j1 - m1) / 2 - k, e3 = (j2 + m2) / 2 - k;
+    const int e4 = (J - j2 + m1) / 2 + k, e5 = (J - j1 - m2) / 2 + k;
+    if (e1 < 0 || e2 < 0 || e3 < 0) break;
+    if (e4 < 0 || e5 < 0) continue;
+    long double den = fact(k) * fact(e1) * fact(e2) * fact(e3) * fact(e4) * fact(e5);
+    sum += ((k & 1) ? -1.0L : 1.0L) / den;
+  }
+  return (double)(sqrtl(pre) * sum);
+}
+
+struct CGTable {
+  int jmax2;   // largest 2j
+  int Jmax;    // largest J
+  std::vector<double> v;
+  int dj, dJ, dm;
+  void build(int jm2) {
+    jmax2 = jm2;
+    Jmax = jm2;
+    dj = jm2 + 1;
+    dJ = Jmax + 1;
+    dm = 2 * jm2 + 1;
+    v.assign((size_t)dj * dj * dJ * dm * dm, 0.0);
+    for (int j1 = 1; j1 <= jm2; j1 += 2)
+      for (int j2 = 1; j2 <= jm2; j2 += 2)
+        for (int J = 0; J <= Jmax; ++J)
+          for (int m1 = -j1; m1 <= j1; m1 += 2)
+            for (int m2 = -j2; m2 <= j2; m2 += 2) at(j1, j2, J, m1, m2) = cg_racah(j1, j2, 2 * J, m1, m2);
+  }
+  double& at(int j1, int j2, int J, int m1, int m2) {
+    return v[(((size_t)j1 * dj + j2) * dJ + J) * dm * dm + (size_t)(m1 + jmax2) * dm + (m2 + jmax2)];
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+double th_clebsch_gordan(int j1, int j2, int j3, int m1, int m2, int m3) {
+  if (m1 + m2 != m3) return 0.0;
+  return cg_racah(j1, j2, j3, m1, m2);
+}
+
+int64_t th_mscheme(int n, const int32_t* sp_2j, const int32_t* sp_2mj, const int32_t* sp_2mt,
+                   const int32_t* sp_l, const int32_t* sp_sh, int nsh, int jmax,
+                   const double* Vcpd, double scaling_2b, int64_t capacity, int16_t* abcd,
+                   double* H2) {
+  if (n <= 0 || nsh <= 0 || jmax < 0 || (abcd && (!Vcpd || !H2))) return -1;
+  int jm2 = 0;
+  for (int i = 0; i < n; ++i) jm2 = sp_2j[i] > jm2 ? sp_2j[i] : jm2;
+  CGTable cg;
+  cg.build(jm2);
+  // isospin CG <1/2 t1 1/2 t2 | T MT>
+  auto cgt = [](int T, int t1, int t2) { return cg_racah(1, 1, 2 * T, t1, t2); };
+  const double Vcut = 1.0e-16;
+  auto V = [&](int mmt, int J, int a, int b, int c, int d) {
+    // hamil_H2cpd(0:3, 0:jmax, nsh, nsh, nsh, nsh), shells 1-based
+    return Vcpd[(size_t)mmt + 4 * ((size_t)J + (size_t)(jmax + 1) *
+                                   ((size_t)(a - 1) + nsh * ((size_t)(b - 1) + nsh *
+                                   ((size_t)(c - 1) + nsh * (size_t)(d - 1)))))];
+  };
+  int64_t cnt = 0;
+  for (int a = 0; a < n; ++a) {
+    const int la = sp_l[a], ma = sp_2mj[a], ta = sp_2mt[a];
+    for (int b = a + 1; b < n; ++b) {
+      const int lb = sp_l[b], mb = sp_2mj[b], tb = sp_2mt[b];
+      if (ma + mb < 0) continue;
+      for (int c = a; c < n; ++c) {
+        const int lc = sp_l[c], mc = sp_2mj[c], tc = sp_2mt[c];
+        const int dmax = (c == a) ? b : n - 1;
+        for (int d = c + 1; d <= dmax; ++d) {
+          const int ld = sp_l[d], md = sp_2mj[d], td = sp_2mt[d];
+          if (((la + lb) & 1) != ((lc + ld) & 1)) continue;
+          if (ta + tb != tc + td) continue;
+          if (ma + mb != mc + md) continue;
+          if (!abcd) {   // counting pass: selection rules only (an upper bound)
+            ++cnt;
+            continue;
+          }
+          // uncouple_JT
+          const int ja = sp_2j[a], jb = sp_2j[b], jc = sp_2j[c], jd = sp_2j[d];
+          const int sha = sp_sh[a], shb = sp_sh[b], shc = sp_sh[c], shd = sp_sh[d];
+          const int MJ = ma + mb, MT = ta + tb;
+          const double dab = (sha == shb) ? 1.0 : 0.0, dcd = (shc == shd) ? 1.0 : 0.0;
+          double sum = 0.0;
+          const int J0 = abs(MJ) / 2, J1 = ((ja + jb) / 2 < (jc + jd) / 2) ? (ja + jb) / 2 : (jc + jd) / 2;
+          for (int J = J0; J <= J1; ++J) {
+            const double cg1 = cg.at(ja, jb, J, ma, mb);
+            const double cg2 = cg.at(jc, jd, J, mc, md);
+            const int T0 = abs(MT) / 2;
+            for (int T = T0; T <= 1; ++T) {
+              const int mmt = MT / 2 + 2 * T;
+              const double v = V(mmt, J, sha, shb, shc, shd);
+              if (fabs(v) < 1.0e-12) continue;
+              const double ph = ((J + T) & 1) ? -1.0 : 1.0;
+              const double nab = sqrt(1.0 - dab * ph) / (1.0 + dab);
+              const double ncd = sqrt(1.0 - dcd * ph) / (1.0 + dcd);
+              if (nab <= 1.0e-12 || ncd <= 1.0e-12) continue;
+              const double ninv = 1.0 / (nab * ncd);
+              sum += ninv * cg1 * cg2 * cgt(T, ta, tb) * cgt(T, tc, td) * v;
+            }
+          }
+          sum *= scaling_2b;
+          if (fabs(sum) > Vcut) {
+            if (cnt >= capacity) return -2;
+            abcd[4 * cnt] = (int16_t)(a + 1);
+            abcd[4 * cnt + 1] = (int16_t)(b + 1);
+            abcd[4 * cnt + 2] = (int16_t)(c + 1);
+            abcd[4 * cnt + 3] = (int16_t)(d + 1);
+            H2[cnt] = sum;
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+  return cnt;
+}
+
+int th_timerev(int n, const int32_t* sp_2j, const int32_t* sp_tr, int64_t ntb,
+               const int16_t* abcd, int8_t* trperm) {
+  if (n <= 0 || ntb < 0 || (ntb > 0 && (!abcd || !trperm))) return -1;
+  for (int64_t k = 0; k < ntb; ++k) {
+    const int ia = abcd[4 * k] - 1, ib = abcd[4 * k + 1] - 1, ic = abcd[4 * k + 2] - 1,
+              id = abcd[4 * k + 3] - 1;
+    if (ia < 0 || ib < 0 || ic < 0 || id < 0 || ia >= n || ib >= n || ic >= n || id >= n) return -1;
+    int code = 0;
+    double phase = (((sp_2j[ia] + sp_2j[ib] + sp_2j[ic] + sp_2j[id]) / 2) & 1) ? -1.0 : 1.0;
+    int ta = sp_tr[ia], tb = sp_tr[ib], tc = sp_tr[ic], td = sp_tr[id];
+    if (ta > tb) { int t = ta; ta = tb; tb = t; phase = -phase; code += 1; }
+    if (tc > td) { int t = tc; tc = td; td = t; phase = -phase; code += 2; }
+    if (ta > tc || (ta == tc && tb > td)) code += 4;
+    if (phase < 0.0) code -= 8;
+    trperm[k] = (int8_t)code;
+  }
+  return 0;
+}
+
+}  // extern "C"

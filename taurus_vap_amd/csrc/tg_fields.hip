@@ -266,12 +266,18 @@ int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const c
     hipLaunchKernelGGL(gather_kernel, dim3(nblk((long long)na * npp)), dim3(NT), 0, s, na, npp,
                        ppp, kRL, nn, two * na, na, Xpp);
   TG_HIP(hipGetLastError());
-  TG_TRY(gemm_grouped(s, na, Xph, Cph, c->ph.d_W.p, c->w_fp32, c->ph.d_groups.as<GroupDesc>(),
-                      c->ph.d_tile_group.as<int>(), c->ph.d_tile_n0.as<int>(), 0,
-                      c->ph.ntiles_total));
-  TG_TRY(gemm_grouped(s, two * na, Xpp, Cpp, c->pp.d_W.p, c->w_fp32,
-                      c->pp.d_groups.as<GroupDesc>(), c->pp.d_tile_group.as<int>(),
-                      c->pp.d_tile_n0.as<int>(), 0, c->pp.ntiles_total));
+  {
+    ScopedTiming tk(c, "kernel:grouped_ph");
+    TG_TRY(gemm_grouped(s, na, Xph, Cph, c->ph.d_W.p, c->w_fp32, c->ph.d_groups.as<GroupDesc>(),
+                        c->ph.d_tile_group.as<int>(), c->ph.d_tile_n0.as<int>(), 0,
+                        c->ph.ntiles_total));
+  }
+  {
+    ScopedTiming tk(c, "kernel:grouped_pp");
+    TG_TRY(gemm_grouped(s, two * na, Xpp, Cpp, c->pp.d_W.p, c->w_fp32,
+                        c->pp.d_groups.as<GroupDesc>(), c->pp.d_tile_group.as<int>(),
+                        c->pp.d_tile_n0.as<int>(), 0, c->pp.ntiles_total));
+  }
   hipLaunchKernelGGL(scatter_ph_kernel, dim3(nblk((long long)na * nph)), dim3(NT), 0, s, na, nph,
                      pph, Cph, nn, gamma);
   hipLaunchKernelGGL(scatter_pp_kernel, dim3(nblk((long long)na * npp + (long long)na * n)),

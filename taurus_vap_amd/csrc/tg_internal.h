@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -163,10 +164,30 @@ struct tg_ctx {
   DevBuf d_int;                                   // int work
   DevBuf d_pfw;                                   // pfaffian work
   int have_state = 0;
-  // timing
+  // on-device gradient step (calculate_gradient, gradient_type 0)
+  int grad_on = 0;
+  double grad_eta = 0, grad_mu = 0;
+  int grad_ncons = 0;
+  std::vector<double> grad_lambda;
+  DevBuf d_c20, d_Z;                              // constraint_20; Zi, Zim1, norm
+  // timing: event pairs recorded without synchronisation, summed at collection
+  struct Pending { std::string key; hipEvent_t e0, e1; };
   int timing = 0;
-  double t_total = 0, t_contr = 0, t_gemm = 0, t_pf = 0, t_inv = 0;
+  std::vector<Pending> pending;
+  std::map<std::string, double> tsum;
+  std::map<std::string, int> tcount;
 };
+
+// Records a [begin, end) event pair on the context stream under `key` when
+// timing is enabled; the elapsed time is added at the next collection.
+struct ScopedTiming {
+  tg_ctx* c;
+  hipEvent_t e0 = nullptr;
+  const char* key;
+  ScopedTiming(tg_ctx* c_, const char* k);
+  ~ScopedTiming();
+};
+int timing_collect(tg_ctx* c);
 
 int contraction_build(tg_ctx* c, int64_t ntb, const int16_t* abcd, const void* H2,
                       int h2_bytes, const int8_t* trperm);

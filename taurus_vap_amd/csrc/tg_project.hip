@@ -386,29 +386,60 @@ __global__ void scale_det_kernel(int na, double s, const cplx* __restrict__ det,
 // ===========================================================================
 namespace {
 
-struct Timer {
-  tg_ctx* c;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  double* slot;
-  Timer(tg_ctx* c_, double* s) : c(c_), slot(s) {
-    if (c->timing) {
-      (void)hipEventCreate(&e0);
-      (void)hipEventCreate(&e1);
-      (void)hipEventRecord(e0, c->stream);
+}  // namespace
+
+ScopedTiming::ScopedTiming(tg_ctx* c_, const char* k) : c(c_), key(k) {
+  if (c->timing && hipEventCreate(&e0) == hipSuccess) (void)hipEventRecord(e0, c->stream);
+}
+ScopedTiming::~ScopedTiming() {
+  if (!c->timing || !e0) return;
+  hipEvent_t e1 = nullptr;
+  if (hipEventCreate(&e1) != hipSuccess) return;
+  (void)hipEventRecord(e1, c->stream);
+  c->pending.push_back({std::string(key), e0, e1});
+}
+int timing_collect(tg_ctx* c) {
+  if (c->pending.empty()) return TG_OK;
+  TG_HIP(hipStreamSynchronize(c->stream));
+  for (auto& p : c->pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, p.e0, p.e1) == hipSuccess) {
+      c->tsum[p.key] += ms;
+      c->tcount[p.key] += 1;
     }
+    (void)hipEventDestroy(p.e0);
+    (void)hipEventDestroy(p.e1);
   }
-  ~Timer() {
-    if (c->timing) {
-      (void)hipEventRecord(e1, c->stream);
-      (void)hipEventSynchronize(e1);
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      *slot += ms;
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
-    }
+  c->pending.clear();
+  return TG_OK;
+}
+
+namespace {
+
+// calculate_gradient with fixed eta, mu (gradient_type 0, module_gradient.f90:
+// 122-141): Z = -eta (H20v - sum_j lambda_j C20_j) + mu Zm1; part[b] = sum Z^2.
+__global__ void __launch_bounds__(NT)
+gradient_kernel(long long nn, double eta, double mu, int ncons, const double* __restrict__ lam,
+                const double* __restrict__ c20, const double* __restrict__ H20v,
+                const double* __restrict__ Zm1, double* __restrict__ Z, double* __restrict__ part) {
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  double z2 = 0.0;
+  if (idx < nn) {
+    double sg = 0.0;
+    for (int j = 0; j < ncons; ++j) sg += lam[j] * c20[(long long)j * nn + idx];
+    const double z = -eta * (H20v[idx] - sg) + mu * Zm1[idx];
+    Z[idx] = z;
+    z2 = z * z;
   }
-};
+  __shared__ double red[NT];
+  red[threadIdx.x] = z2;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
 
 int sign_pow(long long e) { return (e % 2 == 0) ? 1 : -1; }
 
@@ -430,7 +461,7 @@ int project_device(tg_ctx* c) {
 
   // ---- angle-independent real products (once per iteration)
   {
-    Timer tm(c, &c->t_gemm);
+    ScopedTiming tm(c, "gemm");
     for (int sp = 0; sp < 2; ++sp) {
       const int r0 = sp * h;
       TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, U + r0, n, 0, U + r0, n, 0, 0.0, R6 + (0 + sp) * nn, n, 0, 1));
@@ -539,14 +570,15 @@ int project_device(tg_ctx* c) {
 
     // Thouless matrix A_phi (calculate_thouless :684-718)
     {
-      Timer tm(c, &c->t_inv);
+      ScopedTiming tm(c, "thouless_inverse");
       hipLaunchKernelGGL(build_uv_phi_kernel, dim3(nblk(tot)), dim3(NT), 0, s, n, na, R6,
                          d_ph + 2 * a0, Uphi, Vphi, nn);
       TG_HIP(hipGetLastError());
+      ScopedTiming tk(c, "kernel:gj_inverse");
       TG_TRY(batched_inverse(s, n, Uphi, n, nn, na, d_det, d_info, d_perm));
     }
     {
-      Timer tm(c, &c->t_gemm);
+      ScopedTiming tm(c, "gemm");
       TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, Vphi, n, nn, Uphi, n, nn, zero, Aphi, n, nn, na));
       hipLaunchKernelGGL(conj_kernel, dim3(nblk(tot)), dim3(NT), 0, s, tot, Aphi);
       hipLaunchKernelGGL(bcast_real_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, U, Ut);
@@ -561,7 +593,7 @@ int project_device(tg_ctx* c) {
     }
     // fields
     {
-      Timer tm(c, &c->t_contr);
+      ScopedTiming tm(c, "contraction");
       TG_TRY(contraction_run(c, na, rho, kLR, proj ? kRL : nullptr, G, dLR, dRL, proj ? 0 : 1));
       hipLaunchKernelGGL(add_h1_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, c->d_H1.as<cplx>(),
                          G, hsp);
@@ -573,7 +605,7 @@ int project_device(tg_ctx* c) {
     TG_HIP(hipGetLastError());
     // H20 (calculate_H20 with U~, V~)
     {
-      Timer tm(c, &c->t_gemm);
+      ScopedTiming tm(c, "gemm_h20");
       TG_TRY(gemm_zz(s, OP_T, OP_N, n, n, n, one, Ut, n, nn, hsp, n, nn, zero, T, n, nn, na));
       TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, T, n, nn, Vt, n, nn, zero, X, n, nn, na));
       TG_TRY(gemm_zz(s, OP_T, OP_N, n, n, n, one, Ut, n, nn, dLR, n, nn, zero, T, n, nn, na));
@@ -590,7 +622,7 @@ int project_device(tg_ctx* c) {
                          d_over + a0);
       TG_HIP(hipGetLastError());
     } else if (reduced) {
-      Timer tm(c, &c->t_pf);
+      ScopedTiming tm(c, "overlap");
       const int nb = nbloc;
       const long long mm = (long long)nb * nb;
       // reuse X/Y/Z/T as Q/JQ/K/pfaffian work (nb <= n)
@@ -604,7 +636,10 @@ int project_device(tg_ctx* c) {
       hipLaunchKernelGGL(add_j_kernel, dim3(nblk((long long)(nb / 2) * na)), dim3(NT), 0, s, nb, na,
                          d_sg, K);
       TG_HIP(hipGetLastError());
-      TG_TRY(batched_pfaffian(s, nb, K, nb, mm, na, T, d_logabs, d_phase, d_zero));
+      {
+        ScopedTiming tk(c, "kernel:pfaffian");
+        TG_TRY(batched_pfaffian(s, nb, K, nb, mm, na, T, d_logabs, d_phase, d_zero));
+      }
       const double pre = signfac * prod_sgn * sign_pow(nb / 2);
       hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, cmk(pre, 0.0), d_pre);
       hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, d_logabs,
@@ -612,7 +647,7 @@ int project_device(tg_ctx* c) {
       TG_HIP(hipGetLastError());
     } else {
       // reference sequence: R2 -> inverse + det -> M_reg -> Pfaffian(nreg)
-      Timer tm(c, &c->t_pf);
+      ScopedTiming tm(c, "overlap");
       const int nreg = 2 * nbloc;
       const long long m2 = (long long)n2 * n2, mr = (long long)nreg * nreg;
       TG_TRY(c->d_pfw.ensure(sizeof(cplx) * (size_t)na * (m2 + 2 * mr) + sizeof(double) * 2 * n));
@@ -661,6 +696,19 @@ int project_device(tg_ctx* c) {
   hipLaunchKernelGGL(projected_h20_kernel, dim3(nblk(nn)), dim3(NT), 0, s, n, d_pnp, pA, pH, pAH,
                      H20v, d_imag);
   TG_HIP(hipGetLastError());
+  if (c->grad_on) {
+    // Z buffers: [Zcur | Zprev | lambda | partial sums]; the new Z becomes the
+    // momentum term of the next step (gradient_Zim1 = gradient_Zi)
+    double* Zb = c->d_Z.as<double>();
+    double* Zcur = Zb;
+    double* Zprev = Zb + nn;
+    double* lam = Zb + 2 * nn;
+    double* part = lam + std::max(c->grad_ncons, 1);
+    hipLaunchKernelGGL(gradient_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, c->grad_eta, c->grad_mu,
+                       c->grad_ncons, lam, c->d_c20.as<double>(), H20v, Zprev, Zcur, part);
+    TG_HIP(hipGetLastError());
+    TG_HIP(hipMemcpyAsync(Zprev, Zcur, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
+  }
   return TG_OK;
 }
 
@@ -810,8 +858,10 @@ int tg_init(tg_ctx** out, int ndim, const int32_t* sp_2mj, const int32_t* sp_2mt
 int tg_finalize(tg_ctx* c) {
   if (!c) return TG_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)timing_collect(c);
   for (DevBuf* b : {&c->d_H1, &c->d_U, &c->d_V, &c->d_D, &c->d_ang, &c->d_real6, &c->d_work,
-                    &c->d_blk, &c->d_scal, &c->d_out, &c->d_ovl, &c->d_int, &c->d_pfw})
+                    &c->d_blk, &c->d_scal, &c->d_out, &c->d_ovl, &c->d_int, &c->d_pfw, &c->d_c20,
+                    &c->d_Z})
     b->release();
   for (Channel* ch : {&c->ph, &c->pp})
     for (DevBuf* b : {&ch->d_perm, &ch->d_groups, &ch->d_W, &ch->d_tile_group, &ch->d_tile_n0})
@@ -1069,24 +1119,8 @@ int tg_project_resident(tg_ctx* c) {
     tg_set_error("tg_project_resident: no state");
     return TG_E_ARG;
   }
-  c->t_total = c->t_contr = c->t_gemm = c->t_pf = c->t_inv = 0;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (c->timing) {
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    (void)hipEventRecord(e0, c->stream);
-  }
-  int r = project_device(c);
-  if (c->timing) {
-    (void)hipEventRecord(e1, c->stream);
-    (void)hipEventSynchronize(e1);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    c->t_total = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-  }
-  return r;
+  ScopedTiming tm(c, "total");
+  return project_device(c);
 }
 
 int tg_sync(tg_ctx* c) {
@@ -1162,18 +1196,69 @@ int tg_calculate_gradient(int ndim, int gradient_type, double gradient_eta, doub
 
 int tg_enable_timing(tg_ctx* c, int on) {
   if (!c) return TG_E_ARG;
+  TG_TRY(timing_collect(c));
   c->timing = on ? 1 : 0;
   return TG_OK;
 }
 
-int tg_get_timing(tg_ctx* c, double* ms_total, double* ms_contraction, double* ms_gemm,
-                  double* ms_pfaffian, double* ms_inverse) {
+int tg_reset_timing(tg_ctx* c) {
   if (!c) return TG_E_ARG;
-  if (ms_total) *ms_total = c->t_total;
-  if (ms_contraction) *ms_contraction = c->t_contr;
-  if (ms_gemm) *ms_gemm = c->t_gemm;
-  if (ms_pfaffian) *ms_pfaffian = c->t_pf;
-  if (ms_inverse) *ms_inverse = c->t_inv;
+  TG_TRY(timing_collect(c));
+  c->tsum.clear();
+  c->tcount.clear();
+  return TG_OK;
+}
+
+int tg_get_timing(tg_ctx* c, const char* key, double* ms_sum, int* count) {
+  if (!c || !key) return TG_E_ARG;
+  TG_TRY(timing_collect(c));
+  auto it = c->tsum.find(key);
+  if (ms_sum) *ms_sum = (it == c->tsum.end()) ? 0.0 : it->second;
+  if (count) *count = (it == c->tsum.end()) ? 0 : c->tcount[key];
+  return TG_OK;
+}
+
+int tg_set_gradient(tg_ctx* c, double gradient_eta, double gradient_mu, int constraint_dim,
+                    const double* constraint_20, const double* lagrange_lambda1) {
+  if (!c || constraint_dim < 0 || (constraint_dim > 0 && (!constraint_20 || !lagrange_lambda1))) {
+    tg_set_error("tg_set_gradient: invalid argument");
+    return TG_E_ARG;
+  }
+  const size_t nn = (size_t)c->n * c->n;
+  c->grad_on = 1;
+  c->grad_eta = gradient_eta;
+  c->grad_mu = gradient_mu;
+  c->grad_ncons = constraint_dim;
+  const size_t nl = std::max(constraint_dim, 1);
+  TG_TRY(c->d_Z.ensure(sizeof(double) * (2 * nn + nl + nn / NT + 2)));
+  TG_TRY(c->d_c20.ensure(sizeof(double) * std::max<size_t>(1, (size_t)constraint_dim * nn)));
+  double* Zb = c->d_Z.as<double>();
+  TG_HIP(hipMemset(Zb, 0, sizeof(double) * 2 * nn));
+  if (constraint_dim > 0) {
+    TG_HIP(hipMemcpy(Zb + 2 * nn, lagrange_lambda1, sizeof(double) * constraint_dim,
+                     hipMemcpyHostToDevice));
+    TG_HIP(hipMemcpy(c->d_c20.p, constraint_20, sizeof(double) * constraint_dim * nn,
+                     hipMemcpyHostToDevice));
+  }
+  return TG_OK;
+}
+
+int tg_get_gradient(tg_ctx* c, double* gradient_Zi, double* gradient_norm) {
+  if (!c || !c->grad_on) {
+    tg_set_error("tg_get_gradient: tg_set_gradient was not called");
+    return TG_E_ARG;
+  }
+  const size_t nn = (size_t)c->n * c->n;
+  TG_HIP(hipStreamSynchronize(c->stream));
+  const double* Zb = c->d_Z.as<double>();
+  if (gradient_Zi) TG_HIP(hipMemcpy(gradient_Zi, Zb, sizeof(double) * nn, hipMemcpyDeviceToHost));
+  const size_t nb = (nn + NT - 1) / NT;
+  std::vector<double> part(nb);
+  TG_HIP(hipMemcpy(part.data(), Zb + 2 * nn + std::max(c->grad_ncons, 1), sizeof(double) * nb,
+                   hipMemcpyDeviceToHost));
+  double sum = 0;
+  for (double v : part) sum += v;
+  if (gradient_norm) *gradient_norm = sqrt(sum) / c->grad_eta;
   return TG_OK;
 }
 

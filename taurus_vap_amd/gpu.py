@@ -32,7 +32,8 @@ EXPORTS = [
     "tg_last_error", "tg_init", "tg_finalize", "tg_set_mode", "tg_calculate_fields",
     "tg_calculate_fields_diag", "tg_zskpfa", "tg_calculate_overlap",
     "tg_project_wavefunction", "tg_set_state", "tg_project_resident", "tg_sync",
-    "tg_get_results", "tg_calculate_gradient", "tg_enable_timing", "tg_get_timing",
+    "tg_get_results", "tg_calculate_gradient", "tg_set_gradient", "tg_get_gradient",
+    "tg_enable_timing", "tg_reset_timing", "tg_get_timing",
 ]
 
 _lib = None
@@ -67,8 +68,11 @@ def load_library():
     lib.tg_sync.argtypes = [P]
     lib.tg_get_results.argtypes = [P, P, P, P, P, P, P]
     lib.tg_calculate_gradient.argtypes = [I, I, D, D, I, I, P, P, P, I, P, P, P, P, P]
+    lib.tg_set_gradient.argtypes = [P, D, D, I, P, P]
+    lib.tg_get_gradient.argtypes = [P, P, P]
     lib.tg_enable_timing.argtypes = [P, I]
-    lib.tg_get_timing.argtypes = [P, P, P, P, P, P]
+    lib.tg_reset_timing.argtypes = [P]
+    lib.tg_get_timing.argtypes = [P, ctypes.c_char_p, P, P]
     _lib = lib
     return lib
 
@@ -196,13 +200,37 @@ class Context:
         return dict(over=scal[0], ener=scal[1], prot=scal[2], neut=scal[3], H20v=H20v,
                     imag=float(imag[0]))
 
+    # calculate_gradient (gradient_type 0) on the device after each resident step
+    def set_gradient(self, gradient_eta, gradient_mu, constraint_20=None, lagrange_lambda1=None):
+        lam = np.ascontiguousarray(np.asarray(lagrange_lambda1 if lagrange_lambda1 is not None else [],
+                                              dtype=np.float64).reshape(-1))
+        ncons = lam.size
+        c20 = (np.ascontiguousarray(constraint_20, dtype=np.float64).reshape(-1) if ncons
+               else np.zeros(1))
+        if ncons == 0:
+            lam = np.zeros(1)
+        _check(load_library().tg_set_gradient(self._h, float(gradient_eta), float(gradient_mu), ncons,
+                                              _p(c20), _p(lam)), "tg_set_gradient")
+
+    def get_gradient(self):
+        Zi = np.zeros(self.n * self.n)
+        norm = np.zeros(1)
+        _check(load_library().tg_get_gradient(self._h, _p(Zi), _p(norm)), "tg_get_gradient")
+        return Zi, float(norm[0])
+
     def enable_timing(self, on=True):
         _check(load_library().tg_enable_timing(self._h, 1 if on else 0), "tg_enable_timing")
 
-    def timing(self):
-        v = [ctypes.c_double() for _ in range(5)]
-        _check(load_library().tg_get_timing(self._h, *[ctypes.byref(x) for x in v]), "tg_get_timing")
-        return dict(zip(("total", "contraction", "gemm", "pfaffian", "inverse"), [x.value for x in v]))
+    def reset_timing(self):
+        _check(load_library().tg_reset_timing(self._h), "tg_reset_timing")
+
+    def timing(self, key):
+        """(summed ms, number of intervals) recorded under key."""
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int()
+        _check(load_library().tg_get_timing(self._h, key.encode(), ctypes.byref(ms), ctypes.byref(cnt)),
+               "tg_get_timing")
+        return ms.value, cnt.value
 
 
 def zskpfa(A):

@@ -145,7 +145,8 @@ def synth_jt_tbme(shells, seed: int, sigma: float = 1.0, pairing_shift: float = 
                     v = float(rng.normal(0.0, sigma)) * fac
                     if pairing_shift and T == 1 and J == 0 and a == b and (a, b) == (c, d):
                         v += pairing_shift
-                    out.append((a + 1, b + 1, c + 1, d + 1, T, J, v))
+                    # the value is defined as the 10-decimal number the file holds
+                    out.append((a + 1, b + 1, c + 1, d + 1, T, J, float("%.10f" % v)))
     return out
 
 
@@ -328,6 +329,7 @@ class Workload:
     gradient_type: int = 0
     eta: float = 0.01
     mu: float = 0.0
+    seed_sigma: float = 0.1   # width of the random Thouless matrix of the seed state
 
 
 WORKLOADS = {
@@ -337,26 +339,30 @@ WORKLOADS = {
     "pf": Workload("pf", "pf", 5, 4, 9, 1.0, -1.5, [0.0, 4.0 / 3.0, 8.0 / 3.0, 4.0], iters=300,
                    constraints={"Constraint multipole Q20": 0.2}),
     # no-core e_max=4 (140 states), 8p/8n, 11x11
-    "emax4": Workload("emax4", "emax4", 8, 8, 11, 2.0, 0.0, "hw", decay=4.0, iters=10),
+    "emax4": Workload("emax4", "emax4", 8, 8, 11, 2.0, 0.0, "hw", decay=4.0, iters=10,
+                      seed_sigma=0.02),
     # e_max=6 (336 states), 10p/12n, Q20 + Jx, 15x15 angles
     "emax6": Workload("emax6", "emax6", 10, 12, 15, 2.0, 0.0, "hw", decay=4.0, iters=100,
                       constraints={"Constraint multipole Q20": 0.2,
-                                   "Constraint ang. mom. Jx": 0.0}),
+                                   "Constraint ang. mom. Jx": 0.0}, seed_sigma=0.01),
 }
 
 
 def workload_spe(w: Workload, shells):
+    """Single-particle energies as written in the .sho (6 decimals)."""
     if w.spe == "hw":
-        return [w.hw * (2 * n + l + 1.5) for (n, l, j2) in shells]
-    return list(w.spe)
+        e = [w.hw * (2 * n + l + 1.5) for (n, l, j2) in shells]
+    else:
+        e = list(w.spe)
+    return [float("%.6f" % x) for x in e]
 
 
 def write_workload(w: Workload, outdir: str, seed: int = 1234, M: int | None = None,
                    iters: int | None = None, hamil_read: int = 0, iter_print: int = 0,
                    gradient_type: int | None = None, eta: float | None = None,
-                   mu: float | None = None, grad_eps: float = 1e-12):
-    """Writes <outdir>/{<name>.sho, input.txt, initial_wf.txt} for workload w.
-    Returns (basis, U, V)."""
+                   mu: float | None = None, grad_eps: float = 1e-12, UV=None):
+    """Writes <outdir>/{<name>.sho, input.txt, initial_wf.txt} for workload w
+    (the seed state is UV = (U, V) when given).  Returns (basis, U, V)."""
     os.makedirs(outdir, exist_ok=True)
     shells = space_shells(w.space)
     basis = Basis(shells)
@@ -373,6 +379,9 @@ def write_workload(w: Workload, outdir: str, seed: int = 1234, M: int | None = N
                gradient_type=w.gradient_type if gradient_type is None else gradient_type,
                eta=w.eta if eta is None else eta, mu=w.mu if mu is None else mu,
                grad_eps=grad_eps, constraints=w.constraints)
-    U, V = seed_wavefunction(basis, w.Z, w.N, seed=seed + 1)
+    if UV is None:
+        U, V = seed_wavefunction(basis, w.Z, w.N, seed=seed + 1, sigma=w.seed_sigma)
+    else:
+        U, V = UV
     write_wavefunction_text(os.path.join(outdir, "initial_wf.txt"), shells, U, V)
     return basis, U, V
