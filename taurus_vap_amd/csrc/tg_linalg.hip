@@ -121,6 +121,199 @@ gj_inverse_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
   }
 }
 
+// ---------------------------------------------------------------------------
+// Blocked Gauss-Jordan inverse, one workgroup per matrix, thread r owns row r
+// (n <= GJ_THREADS).  Columns are processed in panels of NB: the panel row of
+// each thread lives in registers and is reduced with partial pivoting (pivot
+// rows exchanged through LDS); the row interchanges and the accumulated
+// elimination of the panel are then applied to every other column as one
+// rank-NB update,  c <- P c + (T - E) (P c)[panel rows],  where T are the
+// panel's transformed columns (the in-place inverse convention) and E the
+// identity columns of the panel.  Traffic per matrix drops from n passes over
+// the matrix to n/NB passes.
+// ---------------------------------------------------------------------------
+constexpr int GJ_THREADS = 512;
+constexpr int GJ_NB = 24;
+constexpr int GJ_CH = 16;
+
+__device__ inline void wave_argmax(double& v, int& i) {
+  for (int off = 32; off > 0; off >>= 1) {
+    double v2 = __shfl_xor(v, off, 64);
+    int i2 = __shfl_xor(i, off, 64);
+    if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; }
+  }
+}
+
+// block-wide argmax (first index on ties) for GJ_THREADS threads
+__device__ inline void gj_argmax(double v, int i, double* wv, int* wi, double& bv, int& bi) {
+  wave_argmax(v, i);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { wv[wid] = v; wi[wid] = i; }
+  __syncthreads();
+  bv = wv[0];
+  bi = wi[0];
+  for (int w = 1; w < GJ_THREADS / 64; ++w)
+    if (wv[w] > bv || (wv[w] == bv && wi[w] < bi)) { bv = wv[w]; bi = wi[w]; }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(GJ_THREADS)
+gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* __restrict__ det,
+                  int* __restrict__ info, int* __restrict__ perm_all) {
+  __shared__ cplx prow[GJ_NB];
+  __shared__ cplx sw[2][GJ_NB];
+  __shared__ cplx cb[GJ_NB][GJ_CH];
+  __shared__ double wv[GJ_THREADS / 64];
+  __shared__ int wi[GJ_THREADS / 64];
+  __shared__ int perm[1024];
+  __shared__ cplx spiv;
+  const int t = threadIdx.x;
+  const bool own = t < n;
+  cplx* a = Aall + (long long)blockIdx.x * sA;
+  cplx d = cmk(1.0, 0.0);
+  int inf = 0;
+  for (int k0 = 0; k0 < n; k0 += GJ_NB) {
+    const int w = min(GJ_NB, n - k0);
+    cplx pr[GJ_NB];
+#pragma unroll
+    for (int m = 0; m < GJ_NB; ++m)
+      pr[m] = (own && m < w) ? a[t + (long long)(k0 + m) * lda] : cmk(0.0, 0.0);
+    // ---- panel reduction in registers
+    for (int mm = 0; mm < w; ++mm) {
+      const int k = k0 + mm;
+      cplx pv = cmk(0.0, 0.0);
+#pragma unroll
+      for (int m = 0; m < GJ_NB; ++m)
+        if (m == mm) pv = pr[m];
+      double v = (own && t >= k) ? cabs1(pv) : -1.0;
+      int vi = (own && t >= k) ? t : n;
+      double mx;
+      int p;
+      gj_argmax(v, vi, wv, wi, mx, p);
+      const bool singular = !(mx > 0.0) || p < k || p >= n;
+      if (singular) {
+        if (inf == 0) inf = k + 1;
+        p = k;
+      }
+      if (t == 0) perm[k] = p;
+      if (p != k) {
+        if (t == p)
+#pragma unroll
+          for (int m = 0; m < GJ_NB; ++m) sw[0][m] = pr[m];
+        if (t == k)
+#pragma unroll
+          for (int m = 0; m < GJ_NB; ++m) sw[1][m] = pr[m];
+        __syncthreads();
+        if (t == p)
+#pragma unroll
+          for (int m = 0; m < GJ_NB; ++m) pr[m] = sw[1][m];
+        if (t == k)
+#pragma unroll
+          for (int m = 0; m < GJ_NB; ++m) pr[m] = sw[0][m];
+        d = cmk(-d.x, -d.y);
+      }
+      if (singular) {
+        d = cmk(0.0, 0.0);
+        __syncthreads();
+        continue;
+      }
+      if (t == k) {
+        cplx piv = cmk(0.0, 0.0);
+#pragma unroll
+        for (int m = 0; m < GJ_NB; ++m)
+          if (m == mm) piv = pr[m];
+        const cplx pinv = cinv(piv);
+#pragma unroll
+        for (int m = 0; m < GJ_NB; ++m) {
+          pr[m] = (m == mm) ? pinv : cmul(pr[m], pinv);
+          prow[m] = pr[m];
+        }
+        spiv = piv;
+      }
+      __syncthreads();
+      d = cmul(d, spiv);
+      if (own && t != k) {
+        cplx f = cmk(0.0, 0.0);
+#pragma unroll
+        for (int m = 0; m < GJ_NB; ++m)
+          if (m == mm) f = pr[m];
+#pragma unroll
+        for (int m = 0; m < GJ_NB; ++m) {
+          const cplx r = prow[m];
+          if (m == mm) {
+            pr[m] = cmk(-(f.x * r.x - f.y * r.y), -(f.x * r.y + f.y * r.x));
+          } else {
+            pr[m].x -= f.x * r.x - f.y * r.y;
+            pr[m].y -= f.x * r.y + f.y * r.x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < GJ_NB; ++m)
+      if (own && m < w) a[t + (long long)(k0 + m) * lda] = pr[m];
+    // ---- source row of this thread after the panel's interchanges
+    int src = t;
+    for (int m = w - 1; m >= 0; --m) {
+      const int kk = k0 + m, pp = perm[kk];
+      if (src == kk) src = pp;
+      else if (src == pp) src = kk;
+    }
+    const int blk = t - k0;   // position of this row inside the panel, if any
+    // ---- rank-w update of all other columns, GJ_CH at a time
+    const int nrest = n - w;
+    for (int c0 = 0; c0 < nrest; c0 += GJ_CH) {
+      cplx acc[GJ_CH];
+      const bool inblk = blk >= 0 && blk < w;
+#pragma unroll
+      for (int c = 0; c < GJ_CH; ++c) {
+        const int jj = c0 + c;
+        const int j = (jj < k0) ? jj : jj + w;
+        const cplx v = (own && jj < nrest) ? a[src + (long long)j * lda] : cmk(0.0, 0.0);
+        if (inblk) cb[blk][c] = v;
+        acc[c] = inblk ? cmk(0.0, 0.0) : v;
+      }
+      __syncthreads();
+      if (own) {
+#pragma unroll
+        for (int m = 0; m < GJ_NB; ++m) {
+          if (m < w) {
+            const cplx pm = pr[m];
+#pragma unroll
+            for (int c = 0; c < GJ_CH; ++c) {
+              const cplx b = cb[m][c];
+              acc[c].x += pm.x * b.x - pm.y * b.y;
+              acc[c].y += pm.x * b.y + pm.y * b.x;
+            }
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < GJ_CH; ++c) {
+          const int jj = c0 + c;
+          if (jj < nrest) a[t + (long long)((jj < k0) ? jj : jj + w) * lda] = acc[c];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // undo the row interchanges on the columns of the inverse (own row only)
+  if (own)
+    for (int k = n - 1; k >= 0; --k) {
+      const int p = perm[k];
+      if (p != k) {
+        cplx x = a[t + (long long)k * lda];
+        a[t + (long long)k * lda] = a[t + (long long)p * lda];
+        a[t + (long long)p * lda] = x;
+      }
+    }
+  for (int k = t; k < n; k += GJ_THREADS) perm_all[(long long)blockIdx.x * n + k] = perm[k];
+  if (t == 0) {
+    det[blockIdx.x] = d;
+    info[blockIdx.x] = inf;
+  }
+}
+
 __global__ void __launch_bounds__(NT)
 pfaffian_kernel(int n, const cplx* __restrict__ Aall, int lda, long long sA,
                 cplx* __restrict__ work, double* __restrict__ logabs, cplx* __restrict__ phase,
@@ -205,11 +398,196 @@ pfaffian_kernel(int n, const cplx* __restrict__ Aall, int lda, long long sA,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Blocked Parlett-Reid Pfaffian (n <= PF_THREADS), one workgroup per matrix,
+// thread i owns row i of the full skew copy W.  Pair steps are grouped in
+// panels of PF_PS: within a panel the rank-2 updates
+//   W(0:kk,0:kk) += x y^T - y x^T,  x = W(:,k)/W(kk,k), y = W(:,kk)
+// are kept as pending vectors X(i,s), Y(i,s) in the registers of row i; the
+// columns a step needs are formed on the fly (global column + pending terms)
+// and the symmetric interchange moves the pending rows with the matrix rows.
+// After the panel the leading block receives one rank-2*PF_PS update.  Pivoting
+// is that of ZSKTF2: IZAMAX over W(0:k-1, k), first index on ties.
+// ---------------------------------------------------------------------------
+constexpr int PF_THREADS = 512;
+constexpr int PF_PS = 8;
+constexpr int PF_CH = 16;
+
+__global__ void __launch_bounds__(PF_THREADS)
+pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long sA,
+                        cplx* __restrict__ work, double* __restrict__ logabs,
+                        cplx* __restrict__ phase, int* __restrict__ zero) {
+  __shared__ cplx xk[PF_PS], yk[PF_PS];
+  __shared__ cplx swx[2][PF_PS], swy[2][PF_PS];
+  __shared__ cplx swc[2];
+  __shared__ cplx cx[PF_CH][PF_PS], cy[PF_CH][PF_PS];
+  __shared__ double wv[PF_THREADS / 64];
+  __shared__ int wi[PF_THREADS / 64];
+  __shared__ cplx spiv;
+  const int t = threadIdx.x;
+  const bool own = t < n;
+  const cplx* A = Aall + (long long)blockIdx.x * sA;
+  cplx* W = work + (long long)blockIdx.x * n * n;
+  const long long nn = (long long)n * n;
+  for (long long idx = t; idx < nn; idx += PF_THREADS) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    cplx v = cmk(0.0, 0.0);
+    if (i < j) v = A[i + (long long)j * lda];
+    else if (i > j) { v = A[j + (long long)i * lda]; v = cmk(-v.x, -v.y); }
+    W[idx] = v;
+  }
+  __syncthreads();
+  double la = 0.0;
+  cplx ph = cmk(1.0, 0.0);
+  int z = 0;
+  cplx X[PF_PS], Y[PF_PS];
+  int k_top = n - 1;
+  while (k_top >= 1) {
+#pragma unroll
+    for (int s2 = 0; s2 < PF_PS; ++s2) { X[s2] = cmk(0.0, 0.0); Y[s2] = cmk(0.0, 0.0); }
+    int ns = 0, m = k_top + 1;
+    for (int s = 0; s < PF_PS; ++s) {
+      const int k = k_top - 2 * s;
+      if (k < 1) break;
+      const int kk = k - 1;
+      // pending vectors of row k
+      if (t == k)
+#pragma unroll
+        for (int s2 = 0; s2 < PF_PS; ++s2) { xk[s2] = X[s2]; yk[s2] = Y[s2]; }
+      __syncthreads();
+      cplx cik = cmk(0.0, 0.0);
+      if (own && t < k) {
+        cik = W[t + (long long)k * n];
+#pragma unroll
+        for (int s2 = 0; s2 < PF_PS; ++s2)
+          if (s2 < s) {
+            const cplx a1 = cmul(X[s2], yk[s2]), a2 = cmul(Y[s2], xk[s2]);
+            cik.x += a1.x - a2.x;
+            cik.y += a1.y - a2.y;
+          }
+      }
+      double v = (own && t < k) ? cabs1(cik) : -1.0;
+      int vi = (own && t < k) ? t : n;
+      double mx;
+      int kp;
+      gj_argmax(v, vi, wv, wi, mx, kp);
+      if (!(mx > 0.0) || kp < 0 || kp >= k) { z = 1; break; }
+      if (kp != kk) {
+        if (t == kp) {
+#pragma unroll
+          for (int s2 = 0; s2 < PF_PS; ++s2) { swx[0][s2] = X[s2]; swy[0][s2] = Y[s2]; }
+          swc[0] = cik;
+        }
+        if (t == kk) {
+#pragma unroll
+          for (int s2 = 0; s2 < PF_PS; ++s2) { swx[1][s2] = X[s2]; swy[1][s2] = Y[s2]; }
+          swc[1] = cik;
+        }
+        // rows kp <-> kk of the leading columns
+        for (int j = t; j < k; j += PF_THREADS) {
+          cplx* r1 = W + kp + (long long)j * n;
+          cplx* r2 = W + kk + (long long)j * n;
+          const cplx x1 = *r1;
+          *r1 = *r2;
+          *r2 = x1;
+        }
+        __syncthreads();
+        if (t == kp) {
+#pragma unroll
+          for (int s2 = 0; s2 < PF_PS; ++s2) { X[s2] = swx[1][s2]; Y[s2] = swy[1][s2]; }
+          cik = swc[1];
+        }
+        if (t == kk) {
+#pragma unroll
+          for (int s2 = 0; s2 < PF_PS; ++s2) { X[s2] = swx[0][s2]; Y[s2] = swy[0][s2]; }
+          cik = swc[0];
+        }
+        // columns kp <-> kk of the leading rows
+        if (own && t < k) {
+          cplx* c1 = W + t + (long long)kp * n;
+          cplx* c2 = W + t + (long long)kk * n;
+          const cplx x1 = *c1;
+          *c1 = *c2;
+          *c2 = x1;
+        }
+        ph = cmk(-ph.x, -ph.y);
+        __syncthreads();
+      }
+      if (t == kk) {
+        spiv = cik;
+#pragma unroll
+        for (int s2 = 0; s2 < PF_PS; ++s2) { xk[s2] = X[s2]; yk[s2] = Y[s2]; }
+      }
+      __syncthreads();
+      const cplx p = spiv;
+      const double ap = sqrt(p.x * p.x + p.y * p.y);
+      la += log(ap);
+      ph = cmul(ph, cmk(p.x / ap, p.y / ap));
+      const cplx pinv = cinv(p);
+      cplx xn = cmk(0.0, 0.0), yn = cmk(0.0, 0.0);
+      if (own && t < kk) {
+        cplx ci = W[t + (long long)kk * n];
+#pragma unroll
+        for (int s2 = 0; s2 < PF_PS; ++s2)
+          if (s2 < s) {
+            const cplx a1 = cmul(X[s2], yk[s2]), a2 = cmul(Y[s2], xk[s2]);
+            ci.x += a1.x - a2.x;
+            ci.y += a1.y - a2.y;
+          }
+        xn = cmul(cik, pinv);
+        yn = ci;
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < PF_PS; ++s2)
+        if (s2 == s) { X[s2] = xn; Y[s2] = yn; }
+      ns = s + 1;
+      m = kk;
+      __syncthreads();
+    }
+    if (z) break;
+    // rank-2*ns update of the leading block [0, m)
+    for (int j0 = 0; j0 < m; j0 += PF_CH) {
+      if (t >= j0 && t < j0 + PF_CH && t < m)
+#pragma unroll
+        for (int s2 = 0; s2 < PF_PS; ++s2) { cx[t - j0][s2] = X[s2]; cy[t - j0][s2] = Y[s2]; }
+      __syncthreads();
+      if (own && t < m) {
+        for (int c = 0; c < PF_CH; ++c) {
+          const int j = j0 + c;
+          if (j >= m) break;
+          cplx acc = W[t + (long long)j * n];
+#pragma unroll
+          for (int s2 = 0; s2 < PF_PS; ++s2)
+            if (s2 < ns) {
+              const cplx a1 = cmul(X[s2], cy[c][s2]), a2 = cmul(Y[s2], cx[c][s2]);
+              acc.x += a1.x - a2.x;
+              acc.y += a1.y - a2.y;
+            }
+          W[t + (long long)j * n] = acc;
+        }
+      }
+      __syncthreads();
+    }
+    k_top = m - 1;
+  }
+  if (t == 0) {
+    logabs[blockIdx.x] = z ? -INFINITY : la;
+    phase[blockIdx.x] = z ? cmk(0.0, 0.0) : ph;
+    zero[blockIdx.x] = z;
+  }
+}
+
 }  // namespace
 
 int batched_inverse(hipStream_t s, int n, cplx* A, int lda, long long sA, int batch, cplx* det,
                     int* info, int* perm) {
   if (n <= 0 || batch <= 0) return TG_OK;
+  if (n <= GJ_THREADS) {
+    hipLaunchKernelGGL(gj_blocked_kernel, dim3(batch), dim3(GJ_THREADS), 0, s, n, A, lda, sA, det,
+                       info, perm);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+  }
   size_t shm = 2 * (size_t)n * sizeof(cplx) + (size_t)n * sizeof(int);
   hipLaunchKernelGGL(gj_inverse_kernel, dim3(batch), dim3(NT), shm, s, n, A, lda, sA, det, info,
                      perm);
@@ -223,6 +601,12 @@ int batched_pfaffian(hipStream_t s, int n, const cplx* A, int lda, long long sA,
   if (n % 2 != 0 || n == 0) {
     tg_set_error("batched_pfaffian: n must be even and positive here");
     return TG_E_ARG;
+  }
+  if (n <= PF_THREADS) {
+    hipLaunchKernelGGL(pfaffian_blocked_kernel, dim3(batch), dim3(PF_THREADS), 0, s, n, A, lda, sA,
+                       work, logabs, phase, zero);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
   }
   size_t shm = 2 * (size_t)n * sizeof(cplx);
   hipLaunchKernelGGL(pfaffian_kernel, dim3(batch), dim3(NT), shm, s, n, A, lda, sA, work, logabs,

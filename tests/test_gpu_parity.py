@@ -150,3 +150,32 @@ def test_projection_own_canonical_basis():
                                  float(g["valence_Z"]), float(g["valence_N"]))
     assert np.max(np.abs(r["H20v"] - g["field_H20v"])) < 1e-9
     ctx.close()
+
+
+def test_projection_emax4_vs_oracle():
+    """No-core e_max=4 basis (140 states, 5.5e5 stored TBMEs from the native
+    producer), 3x3 angles: the GPU path against the CPU oracle on the same
+    seeded inputs (blocked inverse and reduced Pfaffian at a mid size)."""
+    from taurus_vap_amd import hamiltonian as H, host, inputs as I
+    w = I.WORKLOADS["emax4"]
+    h = H.workload_hamiltonian(w, 1234)
+    b = h["basis"]
+    U, V = host.seed_state(w, b, 1234)
+    cb = host.construct_canonical_basis(U, V)
+    from taurus_vap_amd import gpu
+    ctx = gpu.Context(b.dim, b.sp_2mj, b.sp_2mt, b.sp_l, b.sp_tr, h["H0"], h["H1"], h["abcd"],
+                      h["H2"], h["trperm"])
+    r = ctx.project_wavefunction(U, V, cb["D"], cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"],
+                                 3, 3, 2 * np.pi, w.Z, w.N)
+    ctx.close()
+    ham = dict(H0=h["H0"], H1=h["H1"], abcd=h["abcd"].astype(np.int64) - 1, H2=h["H2"],
+               trperm=h["trperm"].astype(np.int64))
+    canon = dict(D=cb["D"].astype(complex), nocc=cb["nocc"], nemp=cb["nemp"], ovac=cb["ovac"],
+                 voveru=cb["voveru"])
+    ref = P.project_wavefunction(U, V, ham, dict(sp_2mj=b.sp_2mj, sp_tr=b.sp_tr - 1), 3, 3,
+                                 2 * np.pi, w.Z, w.N, canon=canon)
+    e, e_ref = (r["ener"] / r["over"]).real, (ref["ener"] / ref["over"]).real
+    print("emax4 E_gpu %.12f E_oracle %.12f" % (e, e_ref), flush=True)
+    assert abs(r["over"] - ref["over"]) < 1e-9 * abs(ref["over"])
+    assert abs(e - e_ref) < 1e-8 * abs(e_ref)
+    assert np.max(np.abs(r["H20v"] - ref["H20v"])) < 1e-9
