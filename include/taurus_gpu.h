@@ -112,15 +112,17 @@ int tg_calculate_overlap(int ndim, int nocc0, int nemp0, double ovac0, const dou
  * construct_canonical_basis (src/module_wavefunctions.f90:777-945, called at
  * module_projection.f90:219-234): D0 real ndim x ndim (bogo_zD0), nocc0, nemp0,
  * ovac0 and voveru0 real[ndim-nocc0].
+ * iprint > 0 also computes the parity overlaps rot_pari (:279-284) and pnp_pari.
  * facpi = proj_facpi; valence_Z/N; factor_delta (module_fields.f90:39).
  * Outputs:
- *   pnp_scalars  complex[6]: pnp_over, pnp_ener, pnp_prot, pnp_neut, pnp_prot2, pnp_neut2
+ *   pnp_scalars  complex[7]: pnp_over, pnp_ener, pnp_prot, pnp_neut, pnp_prot2,
+ *                pnp_neut2, pnp_pari (0 unless iprint > 0)
  *   pnp_A, pnp_H20, pnp_AH20 complex ndim x ndim (may be NULL)
  *   field_H20v   real[ndim*ndim]
  *   imag_grad    (may be NULL) the largest Im part checked by :751-755
  * Returns TG_E_IMAG_GRADIENT where the reference stops.
  */
-int tg_project_wavefunction(tg_ctx* ctx, const double* U0, const double* V0,
+int tg_project_wavefunction(tg_ctx* ctx, int iprint, const double* U0, const double* V0,
                             const double* D0, int nocc0, int nemp0, double ovac0,
                             const double* voveru0, int Mphip, int Mphin, double facpi,
                             double valence_Z, double valence_N, double factor_delta,
@@ -136,7 +138,7 @@ int tg_project_wavefunction(tg_ctx* ctx, const double* U0, const double* V0,
 int tg_set_state(tg_ctx* ctx, const double* U0, const double* V0, const double* D0,
                  int nocc0, int nemp0, double ovac0, const double* voveru0, int Mphip,
                  int Mphin, double facpi, double valence_Z, double valence_N,
-                 double factor_delta);
+                 double factor_delta, int iprint);
 int tg_project_resident(tg_ctx* ctx);
 int tg_sync(tg_ctx* ctx);
 int tg_get_results(tg_ctx* ctx, double* pnp_scalars, double* pnp_A, double* pnp_H20,

@@ -220,6 +220,8 @@ do iter = 1, iter_max
     call dump_c0('pnp_neut', pnp_neut)
     call dump_c0('pnp_prot2', pnp_prot2)
     call dump_c0('pnp_neut2', pnp_neut2)
+    call dump_c0('pnp_pari', pnp_pari)
+    call dump_i0('iter_print', iter_print)
     call dump_c2('pnp_A', pnp_A, ndim, ndim)
     call dump_c2('pnp_H20', pnp_H20, ndim, ndim)
     call dump_c2('pnp_AH20', pnp_AH20, ndim, ndim)

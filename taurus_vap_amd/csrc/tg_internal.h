@@ -164,6 +164,8 @@ struct tg_ctx {
   DevBuf d_int;                                   // int work
   DevBuf d_pfw;                                   // pfaffian work
   int have_state = 0;
+  int iprint = 0;                                 // parity overlaps (rot_pari) wanted
+  DevBuf d_psign;                                 // (-1)^l per single-particle state
   // on-device gradient step (calculate_gradient, gradient_type 0)
   int grad_on = 0;
   double grad_eta = 0, grad_mu = 0;

@@ -95,6 +95,14 @@ __global__ void build_uv_phi_kernel(int n, int na, const double* __restrict__ R6
   Vphi[a * stride + r] = v;
 }
 
+// Ds(i,j) = sgn(i) D(i,j)
+__global__ void row_sign_kernel(int n, const double* __restrict__ sgn, const double* __restrict__ D,
+                                double* __restrict__ Ds) {
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= (long long)n * n) return;
+  Ds[idx] = sgn[idx % n] * D[idx];
+}
+
 __global__ void conj_kernel(long long tot, cplx* __restrict__ X) {
   long long idx = (long long)blockIdx.x * NT + threadIdx.x;
   if (idx < tot) X[idx].y = -X[idx].y;
@@ -270,11 +278,11 @@ __global__ void build_mreg_kernel(int n2, int nb0, int nb1, int nreg, int na,
 // per-angle in: over[a], scal[a*9+k]; writes f1/f2 per angle and the 6 sums
 __global__ void __launch_bounds__(NT)
 gauge_scalars_kernel(int na, int n, cplx H0, const cplx* __restrict__ w, const cplx* __restrict__ over,
-                     const cplx* __restrict__ sc, cplx* __restrict__ f12, cplx* __restrict__ ener,
-                     cplx* __restrict__ pnp) {
-  __shared__ cplx red[6][NT];
-  cplx acc[6];
-  for (int k = 0; k < 6; ++k) acc[k] = cmk(0.0, 0.0);
+                     const cplx* __restrict__ sc, const cplx* __restrict__ pari,
+                     cplx* __restrict__ f12, cplx* __restrict__ ener, cplx* __restrict__ pnp) {
+  __shared__ cplx red[7][NT];
+  cplx acc[7];
+  for (int k = 0; k < 7; ++k) acc[k] = cmk(0.0, 0.0);
   for (int a = threadIdx.x; a < na; a += NT) {
     const cplx* s = sc + (long long)a * 9;
     const cplx prot = s[0], neut = s[1];
@@ -292,15 +300,16 @@ gauge_scalars_kernel(int na, int n, cplx H0, const cplx* __restrict__ w, const c
     cfma(acc[3], neut, f1);
     cfma(acc[4], prot2, f1);
     cfma(acc[5], neut2, f1);
+    if (pari) cfma(acc[6], pari[a], w[a]);   // pnp_pari += rot_pari * weight
   }
-  for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = acc[k];
+  for (int k = 0; k < 7; ++k) red[k][threadIdx.x] = acc[k];
   __syncthreads();
   for (int s = NT / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s)
-      for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = cadd(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+      for (int k = 0; k < 7; ++k) red[k][threadIdx.x] = cadd(red[k][threadIdx.x], red[k][threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x < 6) pnp[threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 7) pnp[threadIdx.x] = red[threadIdx.x][0];
 }
 
 // pnp_A = sum f1 A, pnp_H20 = sum f1 H20, pnp_AH20 = sum f2 A   (angles in order)
@@ -469,6 +478,17 @@ int project_device(tg_ctx* c) {
       TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, V + r0, n, 0, U + r0, n, 0, 0.0, R6 + (4 + sp) * nn, n, 0, 1));
       TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, D + r0, n, 0, D + r0, n, 0, 0.0, R6 + (6 + sp) * nn, n, 0, 1));
     }
+    if (c->iprint) {
+      // D0^T diag((-1)^l Pi_p) D0 and D0^T diag((-1)^l Pi_n) D0 (generate_rotation_parity :607-625)
+      double* Ds = R6 + 10 * nn;
+      hipLaunchKernelGGL(row_sign_kernel, dim3(nblk(nn)), dim3(NT), 0, s, n, c->d_psign.as<double>(),
+                         D, Ds);
+      TG_HIP(hipGetLastError());
+      for (int sp = 0; sp < 2; ++sp) {
+        const int r0 = sp * h;
+        TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, D + r0, n, 0, Ds + r0, n, 0, 0.0, R6 + (8 + sp) * nn, n, 0, 1));
+      }
+    }
   }
   const double* Pp = R6 + 6 * nn;
   const double* Pn = R6 + 7 * nn;
@@ -481,8 +501,8 @@ int project_device(tg_ctx* c) {
   cplx* Akeep = work + wsz;                  // A_phi of every angle
   cplx* Hkeep = Akeep + (size_t)A * nn;      // H20_phi of every angle
   // scalars: ph (2A) | traces (9A) | over (A) | ener (A) | f12 (2A) | pnp (8) | pre (A) | det (A)
-  //          | phase (A) | w (A) | logabs (A doubles) | imag (doubles)
-  TG_TRY(c->d_scal.ensure(sizeof(cplx) * (19 * (size_t)A + 8) + sizeof(double) * A +
+  //          | phase (A) | w (A) | pari (A) | logabs (A doubles) | imag (doubles)
+  TG_TRY(c->d_scal.ensure(sizeof(cplx) * (20 * (size_t)A + 8) + sizeof(double) * A +
                           sizeof(double) * (nn / NT + 2)));
   cplx* d_ph = c->d_scal.as<cplx>();
   cplx* d_tr = d_ph + 2 * A;
@@ -494,7 +514,8 @@ int project_device(tg_ctx* c) {
   cplx* d_det = d_pre + A;
   cplx* d_phase = d_det + A;
   cplx* d_w = d_phase + A;
-  double* d_logabs = reinterpret_cast<double*>(d_w + A);
+  cplx* d_pari = d_w + A;
+  double* d_logabs = reinterpret_cast<double*>(d_pari + A);
   double* d_imag = d_logabs + A;
   TG_TRY(c->d_int.ensure(sizeof(int) * ((size_t)A * (2 * n + 4) + 8)));
   int* d_info = c->d_int.as<int>();
@@ -517,7 +538,7 @@ int project_device(tg_ctx* c) {
   // ---- overlap set-up (per iteration, host scalars)
   const int nocc = c->nocc, nemp = c->nemp;
   const int nbloc = n - nocc - nemp;
-  const bool reduced = proj && nocc == 0 && nbloc > 0 && (nbloc % 2 == 0);
+  const bool reduced = nocc == 0 && nbloc > 0 && (nbloc % 2 == 0);
   const double ovac = (nbloc == 0) ? 0.0 : c->ovac;
   // signfac of :811-814 with state 0 = state 1
   const int n0 = n - nocc, n2 = n - nemp;
@@ -527,7 +548,8 @@ int project_device(tg_ctx* c) {
   const int signfac = sign_pow(e1) * sign_pow(e2) * sign_pow(e3);
   std::vector<double> Tsc, sg;
   double prod_sgn = 1.0;
-  if (reduced) {
+  const bool need_ovl = proj || c->iprint;
+  if (reduced && need_ovl) {
     Tsc.resize(nbloc);
     sg.resize(nbloc / 2);
     for (int p = 0; p < nbloc / 2; ++p) {
@@ -540,7 +562,7 @@ int project_device(tg_ctx* c) {
   TG_TRY(c->d_ovl.ensure(sizeof(double) * (2 * (size_t)n + 8)));
   double* d_T = c->d_ovl.as<double>();
   double* d_sg = d_T + n;
-  if (reduced) {
+  if (reduced && need_ovl) {
     TG_HIP(hipMemcpyAsync(d_T, Tsc.data(), sizeof(double) * nbloc, hipMemcpyHostToDevice, s));
     TG_HIP(hipMemcpyAsync(d_sg, sg.data(), sizeof(double) * (nbloc / 2), hipMemcpyHostToDevice, s));
     TG_HIP(hipStreamSynchronize(s));
@@ -616,36 +638,35 @@ int project_device(tg_ctx* c) {
                          c->factor_delta, X, Y, Z, H20);
       TG_HIP(hipGetLastError());
     }
-    // overlap
-    if (!proj) {
-      hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, cmk(1.0, 0.0),
-                         d_over + a0);
-      TG_HIP(hipGetLastError());
-    } else if (reduced) {
-      ScopedTiming tm(c, "overlap");
-      const int nb = nbloc;
-      const long long mm = (long long)nb * nb;
-      // reuse X/Y/Z/T as Q/JQ/K/pfaffian work (nb <= n)
-      cplx* Q = X;
-      cplx* JQ = Y;
-      cplx* K = Z;
-      hipLaunchKernelGGL(build_q_kernel, dim3(nblk(mm * na)), dim3(NT), 0, s, n, nemp, nb, na, Pp,
-                         Pn, d_T, d_sg, d_ph + 2 * a0, Q, JQ);
-      TG_HIP(hipGetLastError());
-      TG_TRY(gemm_zz(s, OP_T, OP_N, nb, nb, nb, one, Q, nb, mm, JQ, nb, mm, zero, K, nb, mm, na));
-      hipLaunchKernelGGL(add_j_kernel, dim3(nblk((long long)(nb / 2) * na)), dim3(NT), 0, s, nb, na,
-                         d_sg, K);
-      TG_HIP(hipGetLastError());
-      {
-        ScopedTiming tk(c, "kernel:pfaffian");
-        TG_TRY(batched_pfaffian(s, nb, K, nb, mm, na, T, d_logabs, d_phase, d_zero));
+    // overlaps <Phi|R|Phi> for the gauge rotation (rot_over) and, when printing,
+    // for parity x gauge (rot_pari, :279-284).  R_n = D0^T R D0 = ep*Pa + en*Pb.
+    auto overlap = [&](const double* Pa, const double* Pb, cplx* dst) -> int {
+      if (reduced) {
+        ScopedTiming tm(c, "overlap");
+        const int nb = nbloc;
+        const long long mm = (long long)nb * nb;
+        // reuse X/Y/Z/T as Q/JQ/K/pfaffian work (nb <= n)
+        cplx* Q = X;
+        cplx* JQ = Y;
+        cplx* K = Z;
+        hipLaunchKernelGGL(build_q_kernel, dim3(nblk(mm * na)), dim3(NT), 0, s, n, nemp, nb, na, Pa,
+                           Pb, d_T, d_sg, d_ph + 2 * a0, Q, JQ);
+        TG_HIP(hipGetLastError());
+        TG_TRY(gemm_zz(s, OP_T, OP_N, nb, nb, nb, one, Q, nb, mm, JQ, nb, mm, zero, K, nb, mm, na));
+        hipLaunchKernelGGL(add_j_kernel, dim3(nblk((long long)(nb / 2) * na)), dim3(NT), 0, s, nb,
+                           na, d_sg, K);
+        TG_HIP(hipGetLastError());
+        {
+          ScopedTiming tk(c, "kernel:pfaffian");
+          TG_TRY(batched_pfaffian(s, nb, K, nb, mm, na, T, d_logabs, d_phase, d_zero));
+        }
+        const double pre = signfac * prod_sgn * sign_pow(nb / 2);
+        hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, cmk(pre, 0.0), d_pre);
+        hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, d_logabs,
+                           d_phase, d_zero, d_pre, ovac, dst);
+        TG_HIP(hipGetLastError());
+        return TG_OK;
       }
-      const double pre = signfac * prod_sgn * sign_pow(nb / 2);
-      hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, cmk(pre, 0.0), d_pre);
-      hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, d_logabs,
-                         d_phase, d_zero, d_pre, ovac, d_over + a0);
-      TG_HIP(hipGetLastError());
-    } else {
       // reference sequence: R2 -> inverse + det -> M_reg -> Pfaffian(nreg)
       ScopedTiming tm(c, "overlap");
       const int nreg = 2 * nbloc;
@@ -655,10 +676,11 @@ int project_device(tg_ctx* c) {
       cplx* M = R2 + (size_t)na * m2;
       cplx* PW = M + (size_t)na * mr;
       double* d_vou = reinterpret_cast<double*>(PW + (size_t)na * mr);
-      TG_HIP(hipMemcpyAsync(d_vou, c->voveru.data() + nemp, sizeof(double) * nbloc,
-                            hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(build_rn2_kernel, dim3(nblk(m2 * na)), dim3(NT), 0, s, n, nemp, n2, na, Pp,
-                         Pn, d_ph + 2 * a0, R2);
+      if (nbloc > 0)
+        TG_HIP(hipMemcpyAsync(d_vou, c->voveru.data() + nemp, sizeof(double) * nbloc,
+                              hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(build_rn2_kernel, dim3(nblk(m2 * na)), dim3(NT), 0, s, n, nemp, n2, na, Pa,
+                         Pb, d_ph + 2 * a0, R2);
       TG_HIP(hipGetLastError());
       TG_TRY(batched_inverse(s, n2, R2, n2, m2, na, d_det, d_info, d_perm));
       if (nbloc > 0) {
@@ -678,9 +700,18 @@ int project_device(tg_ctx* c) {
       hipLaunchKernelGGL(scale_det_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, (double)signfac,
                          d_det, d_pre);
       hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, d_logabs,
-                         d_phase, d_zero, d_pre, ovac, d_over + a0);
+                         d_phase, d_zero, d_pre, ovac, dst);
       TG_HIP(hipGetLastError());
+      return TG_OK;
+    };
+    if (!proj) {
+      hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, cmk(1.0, 0.0),
+                         d_over + a0);
+      TG_HIP(hipGetLastError());
+    } else {
+      TG_TRY(overlap(Pp, Pn, d_over + a0));
     }
+    if (c->iprint) TG_TRY(overlap(R6 + 8 * nn, R6 + 9 * nn, d_pari + a0));
   }
 
   // ---- sum_gauge and calculate_projected_H20
@@ -690,7 +721,7 @@ int project_device(tg_ctx* c) {
   cplx* pAH = pH + nn;
   double* H20v = reinterpret_cast<double*>(pAH + nn);
   hipLaunchKernelGGL(gauge_scalars_kernel, dim3(1), dim3(NT), 0, s, A, n, c->H0, d_w, d_over, d_tr,
-                     d_f12, d_ener, d_pnp);
+                     c->iprint ? d_pari : nullptr, d_f12, d_ener, d_pnp);
   hipLaunchKernelGGL(gauge_matrices_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, A, d_f12, Akeep,
                      Hkeep, nn, pA, pH, pAH);
   hipLaunchKernelGGL(projected_h20_kernel, dim3(nblk(nn)), dim3(NT), 0, s, n, d_pnp, pA, pH, pAH,
@@ -714,7 +745,7 @@ int project_device(tg_ctx* c) {
 
 int upload_state(tg_ctx* c, const double* U0, const double* V0, const double* D0, int nocc0,
                  int nemp0, double ovac0, const double* voveru0, int Mphip, int Mphin,
-                 double facpi, double valZ, double valN, double factor_delta) {
+                 double facpi, double valZ, double valN, double factor_delta, int iprint) {
   const int n = c->n;
   if (!U0 || !V0 || !D0 || Mphip < 1 || Mphin < 1 || nocc0 < 0 || nemp0 < 0 ||
       nocc0 + nemp0 > n || (n - nocc0 > 0 && !voveru0)) {
@@ -725,7 +756,7 @@ int upload_state(tg_ctx* c, const double* U0, const double* V0, const double* D0
   TG_TRY(c->d_U.ensure(sizeof(double) * nn));
   TG_TRY(c->d_V.ensure(sizeof(double) * nn));
   TG_TRY(c->d_D.ensure(sizeof(double) * nn));
-  TG_TRY(c->d_real6.ensure(sizeof(double) * 8 * nn));
+  TG_TRY(c->d_real6.ensure(sizeof(double) * 11 * nn));
   TG_HIP(hipMemcpyAsync(c->d_U.p, U0, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
   TG_HIP(hipMemcpyAsync(c->d_V.p, V0, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
   TG_HIP(hipMemcpyAsync(c->d_D.p, D0, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
@@ -740,6 +771,7 @@ int upload_state(tg_ctx* c, const double* U0, const double* V0, const double* D0
   c->valZ = valZ;
   c->valN = valN;
   c->factor_delta = factor_delta;
+  c->iprint = iprint ? 1 : 0;
   // assign_angles (:112-185)
   c->A = Mphip * Mphin;
   c->phip.resize(c->A);
@@ -771,8 +803,8 @@ int fetch_results(tg_ctx* c, double* scal, double* pA, double* pH, double* pAH, 
   const int A = c->A;
   cplx* d_pnp = d_ph + 2 * A + 9 * A + A + A + 2 * A;
   cplx* d_w = d_pnp + 8 + 3 * A;
-  double* d_imag = reinterpret_cast<double*>(d_w + A) + A;
-  if (scal) TG_HIP(hipMemcpy(scal, d_pnp, sizeof(cplx) * 6, hipMemcpyDeviceToHost));
+  double* d_imag = reinterpret_cast<double*>(d_w + 2 * A) + A;
+  if (scal) TG_HIP(hipMemcpy(scal, d_pnp, sizeof(cplx) * 7, hipMemcpyDeviceToHost));
   if (pA) TG_HIP(hipMemcpy(pA, d_out, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
   if (pH) TG_HIP(hipMemcpy(pH, d_out + nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
   if (pAH) TG_HIP(hipMemcpy(pAH, d_out + 2 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
@@ -846,6 +878,13 @@ int tg_init(tg_ctx** out, int ndim, const int32_t* sp_2mj, const int32_t* sp_2mt
   r = c->d_H1.ensure(sizeof(cplx) * (size_t)ndim * ndim);
   if (r == TG_OK && hipMemcpy(c->d_H1.p, H1, sizeof(cplx) * (size_t)ndim * ndim,
                               hipMemcpyHostToDevice) != hipSuccess) r = TG_E_HIP;
+  if (r == TG_OK) {
+    std::vector<double> ps(ndim);
+    for (int i = 0; i < ndim; ++i) ps[i] = (sp_l[i] % 2 == 0) ? 1.0 : -1.0;
+    r = c->d_psign.ensure(sizeof(double) * ndim);
+    if (r == TG_OK && hipMemcpy(c->d_psign.p, ps.data(), sizeof(double) * ndim,
+                                hipMemcpyHostToDevice) != hipSuccess) r = TG_E_HIP;
+  }
   if (r == TG_OK) r = contraction_build(c, ntb, abcd, H2, h2_bytes, trperm);
   if (r != TG_OK) {
     tg_finalize(c);
@@ -861,7 +900,7 @@ int tg_finalize(tg_ctx* c) {
   (void)timing_collect(c);
   for (DevBuf* b : {&c->d_H1, &c->d_U, &c->d_V, &c->d_D, &c->d_ang, &c->d_real6, &c->d_work,
                     &c->d_blk, &c->d_scal, &c->d_out, &c->d_ovl, &c->d_int, &c->d_pfw, &c->d_c20,
-                    &c->d_Z})
+                    &c->d_Z, &c->d_psign})
     b->release();
   for (Channel* ch : {&c->ph, &c->pp})
     for (DevBuf* b : {&ch->d_perm, &ch->d_groups, &ch->d_W, &ch->d_tile_group, &ch->d_tile_n0})
@@ -1108,10 +1147,11 @@ int tg_calculate_overlap(int ndim, int nocc0, int nemp0, double ovac0, const dou
 
 int tg_set_state(tg_ctx* c, const double* U0, const double* V0, const double* D0, int nocc0,
                  int nemp0, double ovac0, const double* voveru0, int Mphip, int Mphin,
-                 double facpi, double valence_Z, double valence_N, double factor_delta) {
+                 double facpi, double valence_Z, double valence_N, double factor_delta,
+                 int iprint) {
   if (!c) return TG_E_ARG;
   return upload_state(c, U0, V0, D0, nocc0, nemp0, ovac0, voveru0, Mphip, Mphin, facpi, valence_Z,
-                      valence_N, factor_delta);
+                      valence_N, factor_delta, iprint);
 }
 
 int tg_project_resident(tg_ctx* c) {
@@ -1135,15 +1175,15 @@ int tg_get_results(tg_ctx* c, double* pnp_scalars, double* pnp_A, double* pnp_H2
   return fetch_results(c, pnp_scalars, pnp_A, pnp_H20, pnp_AH20, field_H20v, imag_grad);
 }
 
-int tg_project_wavefunction(tg_ctx* c, const double* U0, const double* V0, const double* D0,
-                            int nocc0, int nemp0, double ovac0, const double* voveru0, int Mphip,
-                            int Mphin, double facpi, double valence_Z, double valence_N,
-                            double factor_delta, double* pnp_scalars, double* pnp_A,
-                            double* pnp_H20, double* pnp_AH20, double* field_H20v,
-                            double* imag_grad) {
+int tg_project_wavefunction(tg_ctx* c, int iprint, const double* U0, const double* V0,
+                            const double* D0, int nocc0, int nemp0, double ovac0,
+                            const double* voveru0, int Mphip, int Mphin, double facpi,
+                            double valence_Z, double valence_N, double factor_delta,
+                            double* pnp_scalars, double* pnp_A, double* pnp_H20,
+                            double* pnp_AH20, double* field_H20v, double* imag_grad) {
   if (!c) return TG_E_ARG;
   TG_TRY(upload_state(c, U0, V0, D0, nocc0, nemp0, ovac0, voveru0, Mphip, Mphin, facpi, valence_Z,
-                      valence_N, factor_delta));
+                      valence_N, factor_delta, iprint));
   TG_TRY(tg_project_resident(c));
   return fetch_results(c, pnp_scalars, pnp_A, pnp_H20, pnp_AH20, field_H20v, imag_grad);
 }

@@ -62,8 +62,8 @@ def load_library():
     lib.tg_calculate_fields_diag.argtypes = [P] * 7
     lib.tg_zskpfa.argtypes = [I, I, P, I, P, P]
     lib.tg_calculate_overlap.argtypes = [I, I, I, D, P, P, I, I, D, P, P, P, P]
-    lib.tg_project_wavefunction.argtypes = [P, P, P, P, I, I, D, P, I, I, D, D, D, D, P, P, P, P, P, P]
-    lib.tg_set_state.argtypes = [P, P, P, P, I, I, D, P, I, I, D, D, D, D]
+    lib.tg_project_wavefunction.argtypes = [P, I, P, P, P, I, I, D, P, I, I, D, D, D, D, P, P, P, P, P, P]
+    lib.tg_set_state.argtypes = [P, P, P, P, I, I, D, P, I, I, D, D, D, D, I]
     lib.tg_project_resident.argtypes = [P]
     lib.tg_sync.argtypes = [P]
     lib.tg_get_results.argtypes = [P, P, P, P, P, P, P]
@@ -153,27 +153,28 @@ class Context:
 
     # project_wavefunction(iopt=0) + calculate_projected_H20 (module_projection.f90:201, 729)
     def project_wavefunction(self, U0, V0, D0, nocc0, nemp0, ovac0, voveru0, Mphip, Mphin,
-                             facpi, valence_Z, valence_N, factor_delta=1.0):
+                             facpi, valence_Z, valence_N, factor_delta=1.0, iprint=0):
         n = self.n
         U0, V0, D0 = (_c(x, np.float64) for x in (U0, V0, D0))
         vou = np.ascontiguousarray(np.asarray(voveru0, dtype=np.float64).reshape(-1))
         if vou.size == 0:
             vou = np.zeros(1)
-        scal = np.zeros(6, np.complex128)
+        scal = np.zeros(7, np.complex128)
         pA, pH, pAH = (np.zeros((n, n), np.complex128, order="F") for _ in range(3))
         H20v = np.zeros(n * n)
         imag = np.zeros(1)
         _check(load_library().tg_project_wavefunction(
-            self._h, _p(U0), _p(V0), _p(D0), int(nocc0), int(nemp0), float(ovac0), _p(vou),
+            self._h, int(iprint), _p(U0), _p(V0), _p(D0), int(nocc0), int(nemp0), float(ovac0), _p(vou),
             int(Mphip), int(Mphin), float(facpi), float(valence_Z), float(valence_N),
             float(factor_delta), _p(scal), _p(pA), _p(pH), _p(pAH), _p(H20v), _p(imag)),
             "tg_project_wavefunction")
         return dict(over=scal[0], ener=scal[1], prot=scal[2], neut=scal[3], prot2=scal[4],
-                    neut2=scal[5], A=pA, H20=pH, AH20=pAH, H20v=H20v, imag=float(imag[0]))
+                    neut2=scal[5], pari=scal[6], A=pA, H20=pH, AH20=pAH, H20v=H20v,
+                    imag=float(imag[0]))
 
     # resident-state API used for timing
     def set_state(self, U0, V0, D0, nocc0, nemp0, ovac0, voveru0, Mphip, Mphin, facpi,
-                  valence_Z, valence_N, factor_delta=1.0):
+                  valence_Z, valence_N, factor_delta=1.0, iprint=0):
         U0, V0, D0 = (_c(x, np.float64) for x in (U0, V0, D0))
         vou = np.ascontiguousarray(np.asarray(voveru0, dtype=np.float64).reshape(-1))
         if vou.size == 0:
@@ -182,7 +183,7 @@ class Context:
         _check(load_library().tg_set_state(self._h, _p(U0), _p(V0), _p(D0), int(nocc0), int(nemp0),
                                            float(ovac0), _p(vou), int(Mphip), int(Mphin),
                                            float(facpi), float(valence_Z), float(valence_N),
-                                           float(factor_delta)), "tg_set_state")
+                                           float(factor_delta), int(iprint)), "tg_set_state")
 
     def project_resident(self):
         _check(load_library().tg_project_resident(self._h), "tg_project_resident")
@@ -192,7 +193,7 @@ class Context:
 
     def get_results(self):
         n = self.n
-        scal = np.zeros(6, np.complex128)
+        scal = np.zeros(7, np.complex128)
         H20v = np.zeros(n * n)
         imag = np.zeros(1)
         _check(load_library().tg_get_results(self._h, _p(scal), None, None, None, _p(H20v), _p(imag)),

@@ -62,6 +62,7 @@ def pack_case(d: dict, ham: dict, angle_sel, prec: int) -> dict:
         rot_prot2=np.array(d["rot_prot2"]), rot_neut2=np.array(d["rot_neut2"]),
         pnp_over=d["pnp_over"][0], pnp_ener=d["pnp_ener"][0], pnp_prot=d["pnp_prot"][0],
         pnp_neut=d["pnp_neut"][0], pnp_prot2=d["pnp_prot2"][0], pnp_neut2=d["pnp_neut2"][0],
+        pnp_pari=d["pnp_pari"][0], iter_print=d["iter_print"][0],
         pnp_A=d["pnp_A"][0], pnp_H20=d["pnp_H20"][0], pnp_AH20=d["pnp_AH20"][0],
         field_H20v=d["field_H20v"][0],
         gradient_Zi=d["gradient_Zi"][0], gradient_norm=d["gradient_norm"][0],

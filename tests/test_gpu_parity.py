@@ -112,6 +112,33 @@ def test_projection_matches_reference(case, plain):
     ctx.close()
 
 
+def test_parity_projection_matches_reference():
+    """iprint > 0: parity overlaps of the projected state (rot_pari, pnp_pari,
+    module_projection.f90:279-284, 439) as the reference computed them for the
+    Mg24 deck (iter_print = 1); and the same against the oracle for sd/pf."""
+    g = load_golden("mg24_32")
+    assert int(g["iter_print"]) == 1
+    ctx = _ctx(g)
+    r = _project(ctx, g, iprint=1)
+    ctx.close()
+    assert abs(r["pari"] - g["pnp_pari"]) < 1e-10, (r["pari"], g["pnp_pari"])
+    for case in ("sd64", "pf64"):
+        g = load_golden(case)
+        cb = golden_canon(g)
+        n = int(g["ndim"])
+        par = np.where(g["sp_l"] % 2 == 0, 1.0, -1.0)
+        ref = 0j
+        for a in range(int(g["nangles"])):
+            R = P.rotation_gauge(float(g["phip"][a]), float(g["phin"][a]), n) * par[:, None]
+            o = P.calculate_overlap(cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"], cb["D"],
+                                    cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"], cb["D"], R)
+            ref += o * g["weip"][a] * g["wein"][a]
+        ctx = _ctx(g)
+        r = _project(ctx, g, iprint=1)
+        ctx.close()
+        assert abs(r["pari"] - ref) < 1e-9 * max(1.0, abs(ref)), (case, r["pari"], ref)
+
+
 def test_projection_fp32_storage_matches_stock_reference():
     """The stock reference stores TBMEs in real32 (rH2 = real32) and accumulates
     in real64; the FP32 device copy of the same values reproduces it."""
