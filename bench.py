@@ -5,7 +5,8 @@ Workload (default): e_max = 6 harmonic-oscillator basis (336 single-particle
 states), 15 x 15 Fomenko gauge angles, 10 protons / 12 neutrons, seeded
 synthetic JT-coupled interaction V ~ N(0, 2 MeV) exp(-(Na+Nb+Nc+Nd)/4)
 (12.9M stored m-scheme elements), SPEs hbar*omega (N + 3/2), seed state a real
-Thouless rotation (Z ~ N(0, 0.1)) of the lowest-filled Slater determinant.
+Thouless rotation (Z ~ N(0, 0.01)) of the lowest-filled Slater determinant,
+brought to <Z> = 10, <N> = 12.
 TBMEs stored in FP32, all arithmetic FP64.
 
 One step = one projected-gradient iteration of the hot path with the state
@@ -35,6 +36,11 @@ sys.path.insert(0, ROOT)
 METRIC = "PNVAP gradient iters/s (e_max=6, 15x15 angles) vs plain FP64 GPU, |dE|<1 keV"
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
+# Gauge integral over [0, 2 pi): the seed states carry proton-neutron mixing
+# (their Thouless matrix has proton-neutron entries), so check_symmetries keeps
+# the default proj_facpi = 2 pi (src/module_projection.f90:37,
+# src/module_constraints.f90:749-753), as in the reference's own runs of them.
+FACPI = 2.0 * math.pi
 
 
 def log(msg):
@@ -65,15 +71,17 @@ def contraction_sizes(h):
     return dict(wph=float(sum(v * v for v in ph.values())), wpp=float(sum(v * v for v in pp.values())))
 
 
-def run_gpu(args, ham, plain, fp32, steps, warmup, timing=True):
+def run_gpu(args, ham, plain, fp32, steps, warmup, timing=True, graph=False, conj_pairs=False):
     from taurus_vap_amd import host
     ctx, h, w, U, V = host.workload_context(args.workload, seed=args.seed, tbme_fp32=fp32,
-                                            plain=plain, ham=ham)
+                                            plain=plain, ham=ham, graph=graph,
+                                            conj_pairs=conj_pairs)
+    timing = timing and not graph     # per-kernel events run the eager launch sequence
     cb = host.construct_canonical_basis(U, V)
     n = h["basis"].dim
     M = args.angles if args.angles else w.M
     ctx.set_state(U, V, cb["D"], cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"], M, M,
-                  2.0 * math.pi, float(w.Z), float(w.N))
+                  FACPI, float(w.Z), float(w.N))
     # gradient step with two constraint arrays (the shape of Q20 + Jx)
     rng = np.random.default_rng(args.seed + 7)
     c20 = rng.normal(size=(2, n * n)) * 0.01
@@ -109,14 +117,15 @@ def run_gpu(args, ham, plain, fp32, steps, warmup, timing=True):
         tc1 = time.perf_counter()
         for _ in range(2):
             ctx.project_wavefunction(U, V, cb2["D"], cb2["nocc"], cb2["nemp"], cb2["ovac"],
-                                     cb2["voveru"], M, M, 2.0 * math.pi, float(w.Z), float(w.N))
+                                     cb2["voveru"], M, M, FACPI, float(w.Z), float(w.N))
         tc2 = time.perf_counter()
         boundary = dict(canonical_basis_ms=1e3 * (tc1 - tc0), host_call_ms=1e3 * (tc2 - tc1) / 2)
     ctx.close()
     E = (res["ener"] / res["over"]).real
     nbloc = n - cb["nocc"] - cb["nemp"]
     return dict(seconds=t1 - t0, steps=steps, E=E, Z=(res["prot"] / res["over"]).real,
-                N=(res["neut"] / res["over"]).real, gnorm=gnorm, timing=kt, n=n, A=M * M,
+                N=(res["neut"] / res["over"]).real, gnorm=gnorm, timing=kt, n=n,
+                A=((M * M + 1) // 2 if conj_pairs else M * M),   # angles per launch
                 nb=nbloc, nocc=cb["nocc"], nemp=cb["nemp"], UV=(U, V), boundary=boundary)
 
 
@@ -171,6 +180,19 @@ def cpu_baseline(args, ham, threads, UV):
                        "extrapolated to %d angles" % (t2, t4, min(g2, g4), fixed, per_angle, A))
 
 
+def copy_probe(mib: int):
+    """A plain copy of known size: reads and writes mib MiB per launch."""
+    import torch
+    nel = mib * (1 << 20) // 8
+    src = torch.ones(nel, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy probe: 3 copies of %d MiB (%d bytes read + %d written per launch)"
+        % (mib, nel * 8, nel * 8))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,12 +208,22 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1,
                     help="timed steps of the plain FP64 per-angle path (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--conj-pairs", action="store_true",
+                    help="compute one gauge angle of each (phi, -phi) pair (exact for real U, V)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the projection as a captured hipGraph (no per-kernel timing)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (calibration "
+                         "of the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the batched gauge-angle path is measured on a "
               "single MI355X (--gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     from taurus_vap_amd import hamiltonian as H, inputs as I
     w = I.WORKLOADS[args.workload]
@@ -202,7 +234,8 @@ def main():
     sizes = contraction_sizes(ham)
 
     main_run = run_gpu(args, ham, plain=(args.mode == "plain"), fp32=(args.tbme == "fp32"),
-                       steps=args.steps, warmup=args.warmup)
+                       steps=args.steps, warmup=args.warmup, graph=args.graph,
+                       conj_pairs=args.conj_pairs)
     ips = main_run["steps"] / main_run["seconds"]
     log("optimized: %.3f it/s, E = %.9f MeV, <Z> = %.6f, <N> = %.6f, |grad| = %.4e"
         % (ips, main_run["E"], main_run["Z"], main_run["N"], main_run["gnorm"]))
@@ -212,6 +245,7 @@ def main():
 
     # dominant timed kernel and its roofline
     n, A, nb = main_run["n"], main_run["A"], main_run["nb"]
+    M = args.angles if args.angles else w.M
     fl = algorithmic_flops(sizes, n, A, nb)
     kern = {k: v for k, v in main_run["timing"].items() if k.startswith("kernel:") and v[1] > 0}
     roofline = None
@@ -256,12 +290,13 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "%s: e_max=6 HO basis (%d sp states), %dx%d gauge angles, %dp/%dn, seeded "
+            "workload": "%s: HO basis of %d sp states, %dx%d gauge angles, %dp/%dn, seeded "
                         "synthetic JT interaction (%d stored m-scheme TBMEs), TBMEs %s / FP64 "
                         "arithmetic, state resident in HBM" % (
-                            args.workload, n, int(math.sqrt(A)), int(math.sqrt(A)), w.Z, w.N,
-                            len(ham["trperm"]), args.tbme.upper()),
-            "mode": args.mode,
+                            args.workload, n, M, M, w.Z, w.N, len(ham["trperm"]),
+                            args.tbme.upper()),
+            "mode": args.mode + ("+graph" if args.graph else "") +
+                    ("+conj_pairs" if args.conj_pairs else ""),
             "parallelism": "single GPU, gauge angles batched",
         },
         "roofline": roofline,

@@ -39,6 +39,10 @@ extern "C" {
 /* Context flags */
 #define TG_FLAG_PLAIN        1   /* one gauge angle per launch sequence, no batching */
 #define TG_FLAG_TBME_FP32    2   /* keep TBMEs in FP32 on the device (FP64 accumulation) */
+#define TG_FLAG_GRAPH        4   /* replay the projection as a captured hipGraph */
+#define TG_FLAG_CONJ_PAIRS   8   /* [0, 2 pi) grids: compute one angle of each (phi, -phi)
+                                    pair (per-angle quantities of real U, V at -phi are the
+                                    complex conjugates) and add it as 2 Re(.) */
 
 typedef struct tg_ctx tg_ctx;
 

@@ -266,6 +266,13 @@ int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const c
     hipLaunchKernelGGL(gather_kernel, dim3(nblk((long long)na * npp)), dim3(NT), 0, s, na, npp,
                        ppp, kRL, nn, two * na, na, Xpp);
   TG_HIP(hipGetLastError());
+  // first call for this batch size: pick the tile shapes (outputs recomputed below)
+  TG_TRY(gemm_tune_grouped(s, na, Xph, Cph, c->ph.d_W.p, c->w_fp32, c->ph.d_groups.as<GroupDesc>(),
+                           c->ph.d_tile_group.as<int>(), c->ph.d_tile_n0.as<int>(),
+                           c->ph.ntiles_total, 0));
+  TG_TRY(gemm_tune_grouped(s, two * na, Xpp, Cpp, c->pp.d_W.p, c->w_fp32,
+                           c->pp.d_groups.as<GroupDesc>(), c->pp.d_tile_group.as<int>(),
+                           c->pp.d_tile_n0.as<int>(), c->pp.ntiles_total, 1));
   {
     ScopedTiming tk(c, "kernel:grouped_ph");
     TG_TRY(gemm_grouped(s, na, Xph, Cph, c->ph.d_W.p, c->w_fp32, c->ph.d_groups.as<GroupDesc>(),
@@ -276,7 +283,7 @@ int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const c
     ScopedTiming tk(c, "kernel:grouped_pp");
     TG_TRY(gemm_grouped(s, two * na, Xpp, Cpp, c->pp.d_W.p, c->w_fp32,
                         c->pp.d_groups.as<GroupDesc>(), c->pp.d_tile_group.as<int>(),
-                        c->pp.d_tile_n0.as<int>(), 0, c->pp.ntiles_total));
+                        c->pp.d_tile_n0.as<int>(), 1, c->pp.ntiles_total));
   }
   hipLaunchKernelGGL(scatter_ph_kernel, dim3(nblk((long long)na * nph)), dim3(NT), 0, s, na, nph,
                      pph, Cph, nn, gamma);

@@ -7,6 +7,12 @@
 // (float, converted to double when staged): the contraction blocks keep their
 // matrix elements in FP32 while every product and sum is FP64.
 #include "tg_internal.h"
+#include "tg_mfma.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
 
 namespace {
 
@@ -204,11 +210,141 @@ int dispatch(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K,
 
 inline bool is_zero(cplx z) { return z.x == 0.0 && z.y == 0.0; }
 
+// Kernel family of the complex GEMMs: matrix-core (MFMA) tiles unless
+// TG_GEMM=vector; TG_GEMM_VARIANT / TG_GROUPED_VARIANT force one tile shape
+// (and switch the autotuner off for that family).
+struct GemmSel {
+  int mfma = 1, variant = -1, grouped = -1;
+  GemmSel() {
+    const char* e = std::getenv("TG_GEMM");
+    if (e && std::strcmp(e, "vector") == 0) mfma = 0;
+    const char* v = std::getenv("TG_GEMM_VARIANT");
+    if (v) variant = std::atoi(v);
+    const char* g = std::getenv("TG_GROUPED_VARIANT");
+    if (g) grouped = std::atoi(g);
+  }
+};
+const GemmSel& sel() {
+  static GemmSel s;
+  return s;
+}
+
+// Autotuned tile shape per GEMM signature (kind, ops, sizes, batch).  Filled by
+// gemm_tune_* (timed runs of every variant); signatures never tuned use tile 0.
+struct GemmKey {
+  int kind, opa, opb, M, N, K, batch;
+  bool operator<(const GemmKey& o) const {
+    const int a[7] = {kind, opa, opb, M, N, K, batch};
+    const int b[7] = {o.kind, o.opa, o.opb, o.M, o.N, o.K, o.batch};
+    for (int i = 0; i < 7; ++i)
+      if (a[i] != b[i]) return a[i] < b[i];
+    return false;
+  }
+};
+enum { K_ZZ = 0, K_DZ = 1, K_ZD = 2, K_GROUPED = 3 };
+std::map<GemmKey, int>& choices() {
+  static std::map<GemmKey, int> m;
+  return m;
+}
+int variant_for(const GemmKey& k) {
+  const int forced = (k.kind == K_GROUPED) ? sel().grouped : sel().variant;
+  if (forced >= 0) return forced;
+  auto it = choices().find(k);
+  return it == choices().end() ? 0 : it->second;
+}
+
+// Times `reps` runs of each variant on stream s and records the fastest.
+template <typename F>
+int tune(hipStream_t s, const GemmKey& key, int nvar, F run) {
+  if (!sel().mfma) return TG_OK;
+  const int forced = (key.kind == K_GROUPED) ? sel().grouped : sel().variant;
+  if (forced >= 0 || choices().count(key)) return TG_OK;
+  hipEvent_t e0, e1;
+  TG_HIP(hipEventCreate(&e0));
+  TG_HIP(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 1e30f;
+  int rc = TG_OK;
+  for (int v = 0; v < nvar && rc == TG_OK; ++v) {
+    rc = run(v);                                  // warm-up (first launch of the kernel)
+    if (rc != TG_OK) break;
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < 2 && rc == TG_OK; ++r) rc = run(v);
+    (void)hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = TG_E_HIP;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (rc == TG_OK && ms < best_ms) {
+      best_ms = ms;
+      best = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != TG_OK) return rc;
+  choices()[key] = best;
+  if (std::getenv("TG_GEMM_VERBOSE"))
+    fprintf(stderr, "[taurus_gpu] gemm kind %d ops %d%d M %d N %d K %d batch %d -> tile %d (%.3f ms)\n",
+            key.kind, key.opa, key.opb, key.M, key.N, key.K, key.batch, best, best_ms / 2);
+  return TG_OK;
+}
+
 }  // namespace
+
+int gemm_tuned_variant(int kind, TgOp opa, TgOp opb, int M, int N, int K, int batch) {
+  return variant_for(GemmKey{kind, (int)opa, (int)opb, M, N, K, batch});
+}
+
+int gemm_tune_zz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, const cplx* A, int lda,
+                 long long sA, const cplx* B, int ldb, long long sB, cplx* C, int ldc,
+                 long long sC, int batch) {
+  const GemmKey key{K_ZZ, (int)opa, (int)opb, M, N, K, batch};
+  return tune(s, key, TG_MFMA_VARIANTS, [&](int v) {
+    return mfma_gemm_zz(v, s, opa, opb, M, N, K, cmk(1, 0), A, lda, sA, B, ldb, sB, cmk(0, 0), 0,
+                        C, ldc, sC, batch);
+  });
+}
+
+int gemm_tune_dz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, const double* A,
+                 int lda, long long sA, const cplx* B, int ldb, long long sB, cplx* C, int ldc,
+                 long long sC, int batch) {
+  if (opa == OP_C) opa = OP_T;
+  const GemmKey key{K_DZ, (int)opa, (int)opb, M, N, K, batch};
+  return tune(s, key, TG_MFMA_VARIANTS, [&](int v) {
+    return mfma_gemm_dz(v, s, opa, opb, M, N, K, cmk(1, 0), A, lda, sA, B, ldb, sB, cmk(0, 0), 0,
+                        C, ldc, sC, batch);
+  });
+}
+
+int gemm_tune_zd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, const cplx* A, int lda,
+                 long long sA, const double* B, int ldb, long long sB, cplx* C, int ldc,
+                 long long sC, int batch) {
+  if (opb == OP_C) opb = OP_T;
+  const GemmKey key{K_ZD, (int)opa, (int)opb, M, N, K, batch};
+  return tune(s, key, TG_MFMA_VARIANTS, [&](int v) {
+    return mfma_gemm_zd(v, s, opa, opb, M, N, K, cmk(1, 0), A, lda, sA, B, ldb, sB, cmk(0, 0), 0,
+                        C, ldc, sC, batch);
+  });
+}
+
+int gemm_tune_grouped(hipStream_t s, int M, const cplx* X, cplx* C, const void* W, int w_is_fp32,
+                      const GroupDesc* groups_dev, const int* tile_group_dev,
+                      const int* tile_n0_dev, int ntiles_total, int tag) {
+  const GemmKey key{K_GROUPED, tag, 0, M, ntiles_total, 0, 1};
+  return tune(s, key, TG_MFMA_GROUPED_VARIANTS, [&](int v) {
+    return mfma_gemm_grouped(v, s, M, X, C, W, w_is_fp32, groups_dev, tile_group_dev, tile_n0_dev,
+                             ntiles_total);
+  });
+}
 
 int gemm_zz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
             const cplx* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
             cplx beta, cplx* C, int ldc, long long sC, int batch) {
+  if (M <= 0 || N <= 0 || batch <= 0) return TG_OK;
+  if (sel().mfma)
+    return mfma_gemm_zz(variant_for(GemmKey{K_ZZ, (int)opa, (int)opb, M, N, K, batch}), s, opa,
+                        opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
+                        !is_zero(beta), C, ldc, sC, batch);
   return dispatch<cplx, cplx, cplx>(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
                                     !is_zero(beta), C, ldc, sC, batch);
 }
@@ -216,7 +352,12 @@ int gemm_zz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
 int gemm_dz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
             const double* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
             cplx beta, cplx* C, int ldc, long long sC, int batch) {
+  if (M <= 0 || N <= 0 || batch <= 0) return TG_OK;
   if (opa == OP_C) opa = OP_T;
+  if (sel().mfma)
+    return mfma_gemm_dz(variant_for(GemmKey{K_DZ, (int)opa, (int)opb, M, N, K, batch}), s, opa,
+                        opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
+                        !is_zero(beta), C, ldc, sC, batch);
   return dispatch<double, cplx, cplx>(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
                                       !is_zero(beta), C, ldc, sC, batch);
 }
@@ -224,7 +365,12 @@ int gemm_dz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
 int gemm_zd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
             const cplx* A, int lda, long long sA, const double* B, int ldb, long long sB,
             cplx beta, cplx* C, int ldc, long long sC, int batch) {
+  if (M <= 0 || N <= 0 || batch <= 0) return TG_OK;
   if (opb == OP_C) opb = OP_T;
+  if (sel().mfma)
+    return mfma_gemm_zd(variant_for(GemmKey{K_ZD, (int)opa, (int)opb, M, N, K, batch}), s, opa,
+                        opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
+                        !is_zero(beta), C, ldc, sC, batch);
   return dispatch<cplx, double, cplx>(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
                                       !is_zero(beta), C, ldc, sC, batch);
 }
@@ -240,9 +386,12 @@ int gemm_dd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, double alpha
 
 int gemm_grouped(hipStream_t s, int M, const cplx* X, cplx* C, const void* W, int w_is_fp32,
                  const GroupDesc* groups_dev, const int* tile_group_dev, const int* tile_n0_dev,
-                 int ntiles_n, int ntiles_total) {
-  (void)ntiles_n;
+                 int tag, int ntiles_total) {
   if (M <= 0 || ntiles_total <= 0) return TG_OK;
+  if (sel().mfma)
+    return mfma_gemm_grouped(variant_for(GemmKey{K_GROUPED, tag, 0, M, ntiles_total, 0, 1}), s, M,
+                             X, C, W, w_is_fp32, groups_dev, tile_group_dev, tile_n0_dev,
+                             ntiles_total);
   dim3 grid(ntiles_total, (M + BM - 1) / BM, 1);
   if (w_is_fp32)
     hipLaunchKernelGGL(gemm_grouped_kernel<float>, grid, dim3(NT), 0, s, M, X, C,

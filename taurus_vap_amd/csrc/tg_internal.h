@@ -61,6 +61,9 @@ __host__ __device__ inline void cfma(cplx& a, cplx b, cplx c) {
 // ---------------------------------------------------------------------------
 // Device buffer helper
 // ---------------------------------------------------------------------------
+// Count of device (re)allocations made through DevBuf::ensure.
+unsigned long long devbuf_epoch();
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -100,9 +103,28 @@ struct GroupDesc {
   int size;       // s_g
   long long woff; // offset of W_g
 };
+// `tag` names the channel (0 = ph, 1 = pp) for the autotuner.
 int gemm_grouped(hipStream_t s, int M, const cplx* X, cplx* C, const void* W, int w_is_fp32,
                  const GroupDesc* groups_dev, const int* tile_group_dev, const int* tile_n0_dev,
-                 int ntiles_n, int ntiles_total);
+                 int tag, int ntiles_total);
+
+// Autotuner: time every matrix-core tile variant of one GEMM signature on the
+// given operands (C is overwritten) and keep the fastest for later calls with
+// that signature.  No effect with TG_GEMM=vector or a forced variant.
+int gemm_tune_zz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, const cplx* A, int lda,
+                 long long sA, const cplx* B, int ldb, long long sB, cplx* C, int ldc,
+                 long long sC, int batch);
+int gemm_tune_dz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, const double* A,
+                 int lda, long long sA, const cplx* B, int ldb, long long sB, cplx* C, int ldc,
+                 long long sC, int batch);
+int gemm_tune_zd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, const cplx* A, int lda,
+                 long long sA, const double* B, int ldb, long long sB, cplx* C, int ldc,
+                 long long sC, int batch);
+int gemm_tune_grouped(hipStream_t s, int M, const cplx* X, cplx* C, const void* W, int w_is_fp32,
+                      const GroupDesc* groups_dev, const int* tile_group_dev,
+                      const int* tile_n0_dev, int ntiles_total, int tag);
+// tile variant the dispatcher uses for a signature (kind: 0 zz, 1 dz, 2 zd, 3 grouped)
+int gemm_tuned_variant(int kind, TgOp opa, TgOp opb, int M, int N, int K, int batch);
 
 // ---------------------------------------------------------------------------
 // Linear algebra (tg_linalg.hip)
@@ -132,6 +154,16 @@ struct Channel {
   int ntiles_n = 0, ntiles_total = 0;
 };
 
+// Configuration a captured projection graph is valid for.
+struct GraphKey {
+  long long v[16];
+  bool operator==(const GraphKey& o) const {
+    for (int i = 0; i < 16; ++i)
+      if (v[i] != o.v[i]) return false;
+    return true;
+  }
+};
+
 struct tg_ctx {
   int n = 0;
   int flags = 0;
@@ -153,6 +185,8 @@ struct tg_ctx {
   std::vector<double> voveru;
   std::vector<double> phip, phin;
   std::vector<cplx> weight;                       // weip * wein
+  std::vector<double> mult;                       // 2 for a conjugate pair, else 1
+  int conj_pairs = 0;                             // TG_FLAG_CONJ_PAIRS
   // device work areas
   DevBuf d_U, d_V, d_D, d_ang;                    // real n*n, angles
   DevBuf d_real6;                                 // Kp Kn Lp Ln Mp Mn (6 n*n real)
@@ -164,6 +198,13 @@ struct tg_ctx {
   DevBuf d_int;                                   // int work
   DevBuf d_pfw;                                   // pfaffian work
   int have_state = 0;
+  int tuned_n = 0, tuned_chunk = 0, tuned_nb = -1;  // sizes the GEMM tiles were tuned for
+  int state_dirty = 0;                            // set-up of a new state not uploaded yet
+  // hipGraph replay of the projection (TG_FLAG_GRAPH)
+  int use_graph = 0, graph_valid = 0, graph_seen = 0;
+  GraphKey graph_key_v{}, graph_seen_key{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
   int iprint = 0;                                 // parity overlaps (rot_pari) wanted
   DevBuf d_psign;                                 // (-1)^l per single-particle state
   // on-device gradient step (calculate_gradient, gradient_type 0)

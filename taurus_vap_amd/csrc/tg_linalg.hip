@@ -9,6 +9,8 @@
 // (L2 / MALL resident), the pivot row and column of each step are staged in LDS.
 #include "tg_internal.h"
 
+#include <cstdlib>
+
 namespace {
 
 constexpr int NT = 256;
@@ -130,11 +132,13 @@ gj_inverse_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
 // rank-NB update,  c <- P c + (T - E) (P c)[panel rows],  where T are the
 // panel's transformed columns (the in-place inverse convention) and E the
 // identity columns of the panel.  Traffic per matrix drops from n passes over
-// the matrix to n/NB passes.
+// the matrix to n/NB passes.  The panel rows of the other columns are staged in
+// LDS a window at a time, so the update streams each thread's own row without
+// further synchronisation.
 // ---------------------------------------------------------------------------
 constexpr int GJ_THREADS = 512;
-constexpr int GJ_NB = 24;
-constexpr int GJ_CH = 16;
+constexpr int GJ_WIN = 320;   // columns staged per window (panel width x GJ_WIN complex in LDS)
+constexpr int GJ_U = 8;       // columns per step of a thread's streaming update
 
 __device__ inline void wave_argmax(double& v, int& i) {
   for (int off = 32; off > 0; off >>= 1) {
@@ -157,14 +161,15 @@ __device__ inline void gj_argmax(double v, int i, double* wv, int* wi, double& b
   __syncthreads();
 }
 
+template <int GJ_NB>
 __global__ void __launch_bounds__(GJ_THREADS)
 gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* __restrict__ det,
                   int* __restrict__ info, int* __restrict__ perm_all) {
-  __shared__ cplx prow[GJ_NB];
-  __shared__ cplx sw[2][GJ_NB];
-  __shared__ cplx cb[GJ_NB][GJ_CH];
-  __shared__ double wv[GJ_THREADS / 64];
-  __shared__ int wi[GJ_THREADS / 64];
+  __shared__ cplx sw[2][2][GJ_NB];
+  __shared__ cplx Pw[GJ_NB][GJ_WIN + GJ_U];   // GJ_U columns of slack: no clamping below
+  __shared__ int srcs[GJ_NB];
+  __shared__ double wv[2][GJ_THREADS / 64];
+  __shared__ int wi[2][GJ_THREADS / 64];
   __shared__ int perm[1024];
   __shared__ cplx spiv;
   const int t = threadIdx.x;
@@ -178,78 +183,76 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
 #pragma unroll
     for (int m = 0; m < GJ_NB; ++m)
       pr[m] = (own && m < w) ? a[t + (long long)(k0 + m) * lda] : cmk(0.0, 0.0);
-    // ---- panel reduction in registers
+    // ---- panel reduction in registers: two barriers per pivot (argmax, pivot
+    // rows); the LDS areas alternate between consecutive pivots
     for (int mm = 0; mm < w; ++mm) {
       const int k = k0 + mm;
+      const int buf = mm & 1;
       cplx pv = cmk(0.0, 0.0);
 #pragma unroll
       for (int m = 0; m < GJ_NB; ++m)
         if (m == mm) pv = pr[m];
       double v = (own && t >= k) ? cabs1(pv) : -1.0;
       int vi = (own && t >= k) ? t : n;
-      double mx;
-      int p;
-      gj_argmax(v, vi, wv, wi, mx, p);
+      wave_argmax(v, vi);
+      if ((t & 63) == 0) {
+        wv[buf][t >> 6] = v;
+        wi[buf][t >> 6] = vi;
+      }
+      __syncthreads();
+      double mx = wv[buf][0];
+      int p = wi[buf][0];
+      for (int q = 1; q < GJ_THREADS / 64; ++q)
+        if (wv[buf][q] > mx || (wv[buf][q] == mx && wi[buf][q] < p)) {
+          mx = wv[buf][q];
+          p = wi[buf][q];
+        }
       const bool singular = !(mx > 0.0) || p < k || p >= n;
       if (singular) {
         if (inf == 0) inf = k + 1;
-        p = k;
-      }
-      if (t == 0) perm[k] = p;
-      if (p != k) {
-        if (t == p)
-#pragma unroll
-          for (int m = 0; m < GJ_NB; ++m) sw[0][m] = pr[m];
-        if (t == k)
-#pragma unroll
-          for (int m = 0; m < GJ_NB; ++m) sw[1][m] = pr[m];
-        __syncthreads();
-        if (t == p)
-#pragma unroll
-          for (int m = 0; m < GJ_NB; ++m) pr[m] = sw[1][m];
-        if (t == k)
-#pragma unroll
-          for (int m = 0; m < GJ_NB; ++m) pr[m] = sw[0][m];
-        d = cmk(-d.x, -d.y);
-      }
-      if (singular) {
+        if (t == 0) perm[k] = k;
         d = cmk(0.0, 0.0);
-        __syncthreads();
         continue;
       }
-      if (t == k) {
-        cplx piv = cmk(0.0, 0.0);
+      if (t == 0) perm[k] = p;
+      if (t == p)
 #pragma unroll
-        for (int m = 0; m < GJ_NB; ++m)
-          if (m == mm) piv = pr[m];
-        const cplx pinv = cinv(piv);
+        for (int m = 0; m < GJ_NB; ++m) sw[buf][0][m] = pr[m];
+      if (t == k && p != k)
 #pragma unroll
-        for (int m = 0; m < GJ_NB; ++m) {
-          pr[m] = (m == mm) ? pinv : cmul(pr[m], pinv);
-          prow[m] = pr[m];
-        }
-        spiv = piv;
-      }
+        for (int m = 0; m < GJ_NB; ++m) sw[buf][1][m] = pr[m];
       __syncthreads();
-      d = cmul(d, spiv);
-      if (own && t != k) {
+      const cplx piv = sw[buf][0][mm];
+      const cplx pinv = cinv(piv);
+      if (p != k) d = cmk(-d.x, -d.y);
+      d = cmul(d, piv);
+      if (t == p && p != k)
+#pragma unroll
+        for (int m = 0; m < GJ_NB; ++m) pr[m] = sw[buf][1][m];
+      if (own) {
+        // pivot row k: r / pivot (pivot column 1 / pivot); other rows:
+        // row - (f / pivot) r (pivot column -f / pivot).  One code path:
+        // g = -1/pivot and a zero base for row k, g = f/pivot otherwise.
         cplx f = cmk(0.0, 0.0);
 #pragma unroll
         for (int m = 0; m < GJ_NB; ++m)
           if (m == mm) f = pr[m];
+        const bool pivrow = (t == k);
+        const cplx g = pivrow ? cmk(-pinv.x, -pinv.y) : cmul(f, pinv);
 #pragma unroll
         for (int m = 0; m < GJ_NB; ++m) {
-          const cplx r = prow[m];
           if (m == mm) {
-            pr[m] = cmk(-(f.x * r.x - f.y * r.y), -(f.x * r.y + f.y * r.x));
+            pr[m] = cmk(-g.x, -g.y);
           } else {
-            pr[m].x -= f.x * r.x - f.y * r.y;
-            pr[m].y -= f.x * r.y + f.y * r.x;
+            const cplx r = sw[buf][0][m];
+            const cplx b = pivrow ? cmk(0.0, 0.0) : pr[m];
+            pr[m].x = b.x - (g.x * r.x - g.y * r.y);
+            pr[m].y = b.y - (g.x * r.y + g.y * r.x);
           }
         }
       }
-      __syncthreads();
     }
+    __syncthreads();   // perm of the panel visible to every thread
 #pragma unroll
     for (int m = 0; m < GJ_NB; ++m)
       if (own && m < w) a[t + (long long)(k0 + m) * lda] = pr[m];
@@ -261,37 +264,56 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
       else if (src == pp) src = kk;
     }
     const int blk = t - k0;   // position of this row inside the panel, if any
-    // ---- rank-w update of all other columns, GJ_CH at a time
+    const bool inblk = own && blk >= 0 && blk < w;
+    if (inblk) srcs[blk] = src;
+    __syncthreads();
+    // ---- interchanges and rank-w update of the other columns, GJ_WIN at a time.
+    // Rows moved out of the panel come from panel positions (each position is
+    // swapped once, with a row at or below it), so after the panel source rows
+    // are staged in LDS every thread only reads and writes its own row.
     const int nrest = n - w;
-    for (int c0 = 0; c0 < nrest; c0 += GJ_CH) {
-      cplx acc[GJ_CH];
-      const bool inblk = blk >= 0 && blk < w;
-#pragma unroll
-      for (int c = 0; c < GJ_CH; ++c) {
+    for (int c0 = 0; c0 < nrest; c0 += GJ_WIN) {
+      const int cw = min(GJ_WIN, nrest - c0);
+      for (int e = t; e < w * cw; e += GJ_THREADS) {
+        const int m = e % w, c = e / w;
         const int jj = c0 + c;
-        const int j = (jj < k0) ? jj : jj + w;
-        const cplx v = (own && jj < nrest) ? a[src + (long long)j * lda] : cmk(0.0, 0.0);
-        if (inblk) cb[blk][c] = v;
-        acc[c] = inblk ? cmk(0.0, 0.0) : v;
+        Pw[m][c] = a[srcs[m] + (long long)((jj < k0) ? jj : jj + w) * lda];
       }
       __syncthreads();
+      if (own && !inblk && src != t)
+        for (int c = 0; c < cw; ++c) {
+          const int jj = c0 + c;
+          const long long j = (jj < k0) ? jj : jj + w;
+          a[t + j * lda] = a[src + j * lda];
+        }
+      __syncthreads();
       if (own) {
+        for (int c = 0; c < cw; c += GJ_U) {
+          cplx acc[GJ_U];
 #pragma unroll
-        for (int m = 0; m < GJ_NB; ++m) {
-          if (m < w) {
-            const cplx pm = pr[m];
+          for (int u = 0; u < GJ_U; ++u) {
+            const int jj = c0 + c + u;
+            acc[u] = (!inblk && c + u < cw) ? a[t + (long long)((jj < k0) ? jj : jj + w) * lda]
+                                            : cmk(0.0, 0.0);
+          }
 #pragma unroll
-            for (int c = 0; c < GJ_CH; ++c) {
-              const cplx b = cb[m][c];
-              acc[c].x += pm.x * b.x - pm.y * b.y;
-              acc[c].y += pm.x * b.y + pm.y * b.x;
+          for (int m = 0; m < GJ_NB; ++m) {
+            if (m < w) {
+              const cplx pm = pr[m];
+              const cplx* prow = &Pw[m][c];     // columns past cw are discarded
+#pragma unroll
+              for (int u = 0; u < GJ_U; ++u) {
+                const cplx b = prow[u];
+                acc[u].x += pm.x * b.x - pm.y * b.y;
+                acc[u].y += pm.x * b.y + pm.y * b.x;
+              }
             }
           }
-        }
 #pragma unroll
-        for (int c = 0; c < GJ_CH; ++c) {
-          const int jj = c0 + c;
-          if (jj < nrest) a[t + (long long)((jj < k0) ? jj : jj + w) * lda] = acc[c];
+          for (int u = 0; u < GJ_U; ++u) {
+            const int jj = c0 + c + u;
+            if (c + u < cw) a[t + (long long)((jj < k0) ? jj : jj + w) * lda] = acc[u];
+          }
         }
       }
       __syncthreads();
@@ -583,8 +605,17 @@ int batched_inverse(hipStream_t s, int n, cplx* A, int lda, long long sA, int ba
                     int* info, int* perm) {
   if (n <= 0 || batch <= 0) return TG_OK;
   if (n <= GJ_THREADS) {
-    hipLaunchKernelGGL(gj_blocked_kernel, dim3(batch), dim3(GJ_THREADS), 0, s, n, A, lda, sA, det,
-                       info, perm);
+    // panel width: 16 or 24 columns (TG_GJ_NB), 16 by default
+    static const int nb = [] {
+      const char* e = std::getenv("TG_GJ_NB");
+      return (e && std::atoi(e) == 24) ? 24 : 16;
+    }();
+    if (nb == 24)
+      hipLaunchKernelGGL(gj_blocked_kernel<24>, dim3(batch), dim3(GJ_THREADS), 0, s, n, A, lda,
+                         sA, det, info, perm);
+    else
+      hipLaunchKernelGGL(gj_blocked_kernel<16>, dim3(batch), dim3(GJ_THREADS), 0, s, n, A, lda,
+                         sA, det, info, perm);
     TG_HIP(hipGetLastError());
     return TG_OK;
   }

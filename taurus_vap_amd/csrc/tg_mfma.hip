@@ -1,0 +1,359 @@
+// FP64 matrix-core GEMMs (v_mfma_f64_16x16x4f64) for gfx950.
+//
+// C[b] = alpha * op(A[b]) op(B[b]) + beta * C[b], column-major, with complex
+// operands split into real and imaginary planes in LDS:
+//   complex x complex : Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br  (4 MFMA)
+//   real x complex    : Cr += A Br,          Ci += A Bi          (2 MFMA)
+//   complex x real    : Cr += Ar B,          Ci += Ai B          (2 MFMA)
+// A workgroup of WM x WN waves owns a BM x BN tile of C; each wave owns a
+// (BM/WM) x (BN/WN) sub-tile of 16 x 16 MFMA blocks.  K advances in chunks of 16
+// staged through LDS; the next chunk is loaded into registers while the
+// current one is multiplied.  Each operand is kept in LDS with its contiguous
+// global dimension fastest, so the staging stores are unit-stride.
+//
+// MFMA 16x16x4 f64 operand layout (lane l = 0..63):
+//   A(16x4): lane l holds A[l % 16][l / 16];  B(4x16): lane l holds B[l / 16][l % 16];
+//   D(16x16): lane l, register r holds D[l / 16 + 4 * r][l % 16].
+//
+// Batched launches are mapped XCD-aware: workgroup ids are dealt round-robin
+// to the 8 XCDs, so all tiles of one batch element are given ids congruent
+// modulo 8 and share that XCD's L2.
+#include "tg_internal.h"
+#include "tg_mfma.h"
+
+namespace {
+
+typedef double __attribute__((ext_vector_type(4))) d4;
+
+__device__ inline d4 mfma(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <typename T> struct IsC { static constexpr bool v = false; };
+template <> struct IsC<cplx> { static constexpr bool v = true; };
+
+__device__ inline void split(cplx v, double& r, double& i) { r = v.x; i = v.y; }
+__device__ inline void split(double v, double& r, double& i) { r = v; i = 0.0; }
+__device__ inline void split(float v, double& r, double& i) { r = (double)v; i = 0.0; }
+
+template <typename T> __device__ inline T zero_v();
+template <> __device__ inline double zero_v<double>() { return 0.0; }
+template <> __device__ inline float zero_v<float>() { return 0.0f; }
+template <> __device__ inline cplx zero_v<cplx>() { return cmk(0.0, 0.0); }
+
+constexpr int XCDS = 8;
+
+template <typename TA, typename TB, int OPA, int OPB, int BM, int BN, int WM, int WN>
+struct Tile {
+  static constexpr int BK = 16;
+  static constexpr int NW = WM * WN, NTH = 64 * NW;
+  static constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
+  static constexpr bool CA = IsC<TA>::v, CB = IsC<TB>::v;
+  // A(i,k): OP_N -> contiguous in i: LDS [k][i]; else contiguous in k: LDS [i][k]
+  static constexpr bool AK = (OPA == OP_N);
+  // B(k,j): OP_N -> contiguous in k: LDS [j][k]; else contiguous in j: LDS [k][j]
+  static constexpr bool BJ = (OPB == OP_N);
+  static constexpr int PADR = 4;                   // pad of rows of length BM / BN
+  static constexpr int LDA_ = AK ? BM + PADR : BK + 1;
+  static constexpr int LDB_ = BJ ? BK + 1 : BN + PADR;
+  static constexpr int SA = AK ? BK * LDA_ : BM * LDA_;
+  static constexpr int SB = BJ ? BN * LDB_ : BK * LDB_;
+  static constexpr int NA = BM * BK / NTH, NB = BN * BK / NTH;
+  static_assert(BM % (16 * WM) == 0 && BN % (16 * WN) == 0, "tile");
+  static_assert((BM * BK) % NTH == 0 && (BN * BK) % NTH == 0, "staging");
+
+  __device__ static inline int ia(int i, int k) { return AK ? k * LDA_ + i : i * LDA_ + k; }
+  __device__ static inline int ib(int k, int j) { return BJ ? j * LDB_ + k : k * LDB_ + j; }
+
+  struct Smem {
+    double ar[SA];
+    double ai[CA ? SA : 1];
+    double br[SB];
+    double bi[CB ? SB : 1];
+  };
+
+  // element (i, k) of op(A) and (k, j) of op(B) handled by staging slot e
+  __device__ static inline void slot_a(int e, int& i, int& k) {
+    if (AK) { i = e % BM; k = e / BM; } else { k = e % BK; i = e / BK; }
+  }
+  __device__ static inline void slot_b(int e, int& k, int& j) {
+    if (BJ) { k = e % BK; j = e / BK; } else { j = e % BN; k = e / BN; }
+  }
+
+  __device__ static void run(int M, int N, int K, cplx alpha, const TA* __restrict__ A, int lda,
+                             const TB* __restrict__ B, int ldb, cplx beta, bool use_beta,
+                             cplx* __restrict__ C, int ldc, int m0, int n0, Smem& sm) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm0 = (w % WM) * (BM / WM), wn0 = (w / WM) * (BN / WN);
+    const int lr = lane & 15, lk = lane >> 4;
+
+    d4 cr[TM][TN], ci[TM][TN];
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TN; ++y) {
+        cr[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+        ci[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+      }
+
+    TA ra[NA];
+    TB rb[NB];
+    auto load = [&](int k0) {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) {
+        int i, k;
+        slot_a(t + r * NTH, i, k);
+        const int gi = m0 + i, gk = k0 + k;
+        TA v = zero_v<TA>();
+        if (gi < M && gk < K) v = AK ? A[gi + (long long)gk * lda] : A[gk + (long long)gi * lda];
+        ra[r] = v;
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        int k, j;
+        slot_b(t + r * NTH, k, j);
+        const int gk = k0 + k, gj = n0 + j;
+        TB v = zero_v<TB>();
+        if (gk < K && gj < N) v = BJ ? B[gk + (long long)gj * ldb] : B[gj + (long long)gk * ldb];
+        rb[r] = v;
+      }
+    };
+    auto store = [&]() {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) {
+        int i, k;
+        slot_a(t + r * NTH, i, k);
+        double vr, vi;
+        split(ra[r], vr, vi);
+        sm.ar[ia(i, k)] = vr;
+        if (CA) sm.ai[ia(i, k)] = (OPA == OP_C) ? -vi : vi;
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        int k, j;
+        slot_b(t + r * NTH, k, j);
+        double vr, vi;
+        split(rb[r], vr, vi);
+        sm.br[ib(k, j)] = vr;
+        if (CB) sm.bi[ib(k, j)] = (OPB == OP_C) ? -vi : vi;
+      }
+    };
+
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+      store();
+      __syncthreads();
+      if (k0 + BK < K) load(k0 + BK);
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        double ar[TM], ai[TM], br[TN], bi[TN];
+#pragma unroll
+        for (int x = 0; x < TM; ++x) {
+          const int idx = ia(wm0 + 16 * x + lr, kk + lk);
+          ar[x] = sm.ar[idx];
+          if (CA) ai[x] = sm.ai[idx];
+        }
+#pragma unroll
+        for (int y = 0; y < TN; ++y) {
+          const int idx = ib(kk + lk, wn0 + 16 * y + lr);
+          br[y] = sm.br[idx];
+          if (CB) bi[y] = sm.bi[idx];
+        }
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+          for (int y = 0; y < TN; ++y) {
+            if (CA && CB) {
+              cr[x][y] = mfma(ar[x], br[y], cr[x][y]);
+              cr[x][y] = mfma(-ai[x], bi[y], cr[x][y]);
+              ci[x][y] = mfma(ar[x], bi[y], ci[x][y]);
+              ci[x][y] = mfma(ai[x], br[y], ci[x][y]);
+            } else if (CA) {
+              cr[x][y] = mfma(ar[x], br[y], cr[x][y]);
+              ci[x][y] = mfma(ai[x], br[y], ci[x][y]);
+            } else {
+              cr[x][y] = mfma(ar[x], br[y], cr[x][y]);
+              ci[x][y] = mfma(ar[x], bi[y], ci[x][y]);
+            }
+          }
+      }
+      __syncthreads();
+    }
+
+    // epilogue: lane holds rows lk + 4*r of column lr of each 16x16 block
+#pragma unroll
+    for (int y = 0; y < TN; ++y) {
+      const int gj = n0 + wn0 + 16 * y + lr;
+      if (gj >= N) continue;
+#pragma unroll
+      for (int x = 0; x < TM; ++x) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gi = m0 + wm0 + 16 * x + lk + 4 * r;
+          if (gi >= M) continue;
+          cplx* cp = C + gi + (long long)gj * ldc;
+          cplx v = cmul(alpha, cmk(cr[x][y][r], ci[x][y][r]));
+          if (use_beta) v = cadd(v, cmul(beta, *cp));
+          *cp = v;
+        }
+      }
+    }
+  }
+};
+
+// Batched: grid = XCDS * ceil(batch / XCDS) * tiles_per_matrix workgroups.
+template <typename TA, typename TB, int OPA, int OPB, int BM, int BN, int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN)
+mfma_batched_kernel(int M, int N, int K, cplx alpha, const TA* __restrict__ A, int lda,
+                    long long sA, const TB* __restrict__ B, int ldb, long long sB, cplx beta,
+                    int use_beta, cplx* __restrict__ C, int ldc, long long sC, int batch,
+                    int tiles_m, int tiles) {
+  typedef Tile<TA, TB, OPA, OPB, BM, BN, WM, WN> T;
+  __shared__ typename T::Smem sm;
+  const int L = blockIdx.x;
+  const int xcd = L % XCDS, q = L / XCDS;
+  const int slot = q / tiles, tile = q % tiles;
+  const long long b = xcd + (long long)XCDS * slot;
+  if (b >= batch) return;
+  const int tm = tile % tiles_m, tn = tile / tiles_m;
+  T::run(M, N, K, alpha, A + b * sA, lda, B + b * sB, ldb, beta, use_beta != 0, C + b * sC, ldc,
+         tm * BM, tn * BN, sm);
+}
+
+// Grouped contraction: C_g (M x s_g) = X_g (M x s_g) W_g (s_g x s_g), W real (float/double).
+template <typename TW, int BM, int BN, int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN)
+mfma_grouped_kernel(int M, const cplx* __restrict__ X, cplx* __restrict__ C,
+                    const TW* __restrict__ W, const GroupDesc* __restrict__ groups,
+                    const int* __restrict__ tile_group, const int* __restrict__ tile_n0) {
+  typedef Tile<cplx, TW, OP_N, OP_N, BM, BN, WM, WN> T;
+  __shared__ typename T::Smem sm;
+  const int tile = blockIdx.x;
+  const GroupDesc g = groups[tile_group[tile]];
+  const int n0 = tile_n0[tile];
+  T::run(M, g.size, g.size, cmk(1.0, 0.0), X + (long long)g.off * M, M, W + g.woff, g.size,
+         cmk(0.0, 0.0), false, C + (long long)g.off * M, M, blockIdx.y * BM, n0, sm);
+}
+
+template <typename TA, typename TB, int OPA, int OPB, int BM, int BN, int WM, int WN>
+int launch_batched(hipStream_t s, int M, int N, int K, cplx alpha, const TA* A, int lda,
+                   long long sA, const TB* B, int ldb, long long sB, cplx beta, int use_beta,
+                   cplx* C, int ldc, long long sC, int batch) {
+  const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+  const int tiles = tiles_m * tiles_n;
+  const long long slots = (batch + XCDS - 1) / XCDS;
+  const long long grid = (long long)XCDS * slots * tiles;
+  if (grid > 0x7fffffffLL) {
+    tg_set_error("mfma gemm: grid too large");
+    return TG_E_ARG;
+  }
+  hipLaunchKernelGGL((mfma_batched_kernel<TA, TB, OPA, OPB, BM, BN, WM, WN>), dim3((unsigned)grid),
+                     dim3(64 * WM * WN), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
+                     use_beta, C, ldc, sC, batch, tiles_m, tiles);
+  TG_HIP(hipGetLastError());
+  return TG_OK;
+}
+
+// tile variants: 0 = 64x64 (2x2 waves), 1 = 48x48 (1x3 waves), 2 = 112x112 (1x7 waves),
+// 3 = 96x64 (2x2 waves)
+template <typename TA, typename TB, int OPA, int OPB>
+int launch_variant(int variant, hipStream_t s, int M, int N, int K, cplx alpha, const TA* A,
+                   int lda, long long sA, const TB* B, int ldb, long long sB, cplx beta,
+                   int use_beta, cplx* C, int ldc, long long sC, int batch) {
+  switch (variant) {
+    case 0:
+      return launch_batched<TA, TB, OPA, OPB, 64, 64, 2, 2>(s, M, N, K, alpha, A, lda, sA, B, ldb,
+                                                            sB, beta, use_beta, C, ldc, sC, batch);
+    case 1:
+      return launch_batched<TA, TB, OPA, OPB, 48, 48, 1, 3>(s, M, N, K, alpha, A, lda, sA, B, ldb,
+                                                            sB, beta, use_beta, C, ldc, sC, batch);
+    case 2:
+      return launch_batched<TA, TB, OPA, OPB, 112, 112, 1, 7>(s, M, N, K, alpha, A, lda, sA, B,
+                                                              ldb, sB, beta, use_beta, C, ldc, sC,
+                                                              batch);
+    case 3:
+      return launch_batched<TA, TB, OPA, OPB, 96, 64, 2, 2>(s, M, N, K, alpha, A, lda, sA, B, ldb,
+                                                            sB, beta, use_beta, C, ldc, sC, batch);
+  }
+  tg_set_error("mfma gemm: bad tile variant");
+  return TG_E_ARG;
+}
+
+template <typename TA, typename TB>
+int dispatch(int variant, hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
+             const TA* A, int lda, long long sA, const TB* B, int ldb, long long sB, cplx beta,
+             int use_beta, cplx* C, int ldc, long long sC, int batch) {
+#define TG_CASE(OA, OB)                                                                        \
+  if (opa == OA && opb == OB)                                                                  \
+    return launch_variant<TA, TB, OA, OB>(variant, s, M, N, K, alpha, A, lda, sA, B, ldb, sB,  \
+                                          beta, use_beta, C, ldc, sC, batch);
+  TG_CASE(OP_N, OP_N) TG_CASE(OP_N, OP_T) TG_CASE(OP_T, OP_N) TG_CASE(OP_T, OP_T)
+  if (IsC<TA>::v) { TG_CASE(OP_C, OP_N) TG_CASE(OP_C, OP_T) }
+  if (IsC<TB>::v) { TG_CASE(OP_N, OP_C) TG_CASE(OP_T, OP_C) }
+  if (IsC<TA>::v && IsC<TB>::v) { TG_CASE(OP_C, OP_C) }
+#undef TG_CASE
+  tg_set_error("mfma gemm: unsupported op combination");
+  return TG_E_ARG;
+}
+
+}  // namespace
+
+int mfma_gemm_zz(int variant, hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
+                 const cplx* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
+                 cplx beta, int use_beta, cplx* C, int ldc, long long sC, int batch) {
+  return dispatch<cplx, cplx>(variant, s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
+                              use_beta, C, ldc, sC, batch);
+}
+
+int mfma_gemm_dz(int variant, hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
+                 const double* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
+                 cplx beta, int use_beta, cplx* C, int ldc, long long sC, int batch) {
+  if (opa == OP_C) opa = OP_T;
+  return dispatch<double, cplx>(variant, s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
+                                use_beta, C, ldc, sC, batch);
+}
+
+int mfma_gemm_zd(int variant, hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
+                 const cplx* A, int lda, long long sA, const double* B, int ldb, long long sB,
+                 cplx beta, int use_beta, cplx* C, int ldc, long long sC, int batch) {
+  if (opb == OP_C) opb = OP_T;
+  return dispatch<cplx, double>(variant, s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
+                                use_beta, C, ldc, sC, batch);
+}
+
+int mfma_gemm_grouped(int variant, hipStream_t s, int M, const cplx* X, cplx* C, const void* W,
+                      int w_is_fp32, const GroupDesc* groups_dev, const int* tile_group_dev,
+                      const int* tile_n0_dev, int ntiles_total) {
+  if (M <= 0 || ntiles_total <= 0) return TG_OK;
+  // the tile lists are built for 64-column tiles of each group
+  if (variant == 0) {
+    dim3 grid(ntiles_total, (M + 63) / 64, 1);
+    if (w_is_fp32)
+      hipLaunchKernelGGL((mfma_grouped_kernel<float, 64, 64, 2, 2>), grid, dim3(256), 0, s, M, X,
+                         C, (const float*)W, groups_dev, tile_group_dev, tile_n0_dev);
+    else
+      hipLaunchKernelGGL((mfma_grouped_kernel<double, 64, 64, 2, 2>), grid, dim3(256), 0, s, M, X,
+                         C, (const double*)W, groups_dev, tile_group_dev, tile_n0_dev);
+  } else if (variant == 1) {
+    dim3 grid(ntiles_total, (M + 127) / 128, 1);
+    if (w_is_fp32)
+      hipLaunchKernelGGL((mfma_grouped_kernel<float, 128, 64, 4, 1>), grid, dim3(256), 0, s, M, X,
+                         C, (const float*)W, groups_dev, tile_group_dev, tile_n0_dev);
+    else
+      hipLaunchKernelGGL((mfma_grouped_kernel<double, 128, 64, 4, 1>), grid, dim3(256), 0, s, M,
+                         X, C, (const double*)W, groups_dev, tile_group_dev, tile_n0_dev);
+  } else if (variant == 2) {
+    dim3 grid(ntiles_total, (M + 255) / 256, 1);
+    if (w_is_fp32)
+      hipLaunchKernelGGL((mfma_grouped_kernel<float, 256, 64, 4, 2>), grid, dim3(512), 0, s, M, X,
+                         C, (const float*)W, groups_dev, tile_group_dev, tile_n0_dev);
+    else
+      hipLaunchKernelGGL((mfma_grouped_kernel<double, 256, 64, 4, 2>), grid, dim3(512), 0, s, M,
+                         X, C, (const double*)W, groups_dev, tile_group_dev, tile_n0_dev);
+  } else {
+    tg_set_error("mfma grouped gemm: bad tile variant");
+    return TG_E_ARG;
+  }
+  TG_HIP(hipGetLastError());
+  return TG_OK;
+}

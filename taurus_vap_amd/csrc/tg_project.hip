@@ -34,12 +34,18 @@
 
 #include "tg_internal.h"
 
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
 // ---------------------------------------------------------------------------
 // errors / buffers
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
 void tg_set_error(const std::string& msg) { g_err = msg; }
 extern "C" const char* tg_last_error(void) { return g_err.c_str(); }
+
+static unsigned long long g_devbuf_epoch = 0;
 
 int DevBuf::ensure(size_t nbytes) {
   if (nbytes <= bytes && p) return TG_OK;
@@ -53,8 +59,10 @@ int DevBuf::ensure(size_t nbytes) {
     return TG_E_NOMEM;
   }
   bytes = nbytes;
+  ++g_devbuf_epoch;
   return TG_OK;
 }
+unsigned long long devbuf_epoch() { return g_devbuf_epoch; }
 void DevBuf::release() {
   if (p) (void)hipFree(p);
   p = nullptr;
@@ -279,7 +287,8 @@ __global__ void build_mreg_kernel(int n2, int nb0, int nb1, int nreg, int na,
 __global__ void __launch_bounds__(NT)
 gauge_scalars_kernel(int na, int n, cplx H0, const cplx* __restrict__ w, const cplx* __restrict__ over,
                      const cplx* __restrict__ sc, const cplx* __restrict__ pari,
-                     cplx* __restrict__ f12, cplx* __restrict__ ener, cplx* __restrict__ pnp) {
+                     const double* __restrict__ mult, cplx* __restrict__ f12,
+                     cplx* __restrict__ ener, cplx* __restrict__ pnp) {
   __shared__ cplx red[7][NT];
   cplx acc[7];
   for (int k = 0; k < 7; ++k) acc[k] = cmk(0.0, 0.0);
@@ -294,13 +303,16 @@ gauge_scalars_kernel(int na, int n, cplx H0, const cplx* __restrict__ w, const c
     f12[2 * a] = f1;
     f12[2 * a + 1] = cmul(f1, e);
     ener[a] = e;
-    acc[0] = cadd(acc[0], f1);
-    cfma(acc[1], e, f1);
-    cfma(acc[2], prot, f1);
-    cfma(acc[3], neut, f1);
-    cfma(acc[4], prot2, f1);
-    cfma(acc[5], neut2, f1);
-    if (pari) cfma(acc[6], pari[a], w[a]);   // pnp_pari += rot_pari * weight
+    // a conjugate pair adds t + conj(t) = 2 Re(t)
+    const bool pair = mult[a] > 1.5;
+    auto add = [&](cplx& s, cplx t) { s = pair ? cmk(s.x + 2.0 * t.x, s.y) : cadd(s, t); };
+    add(acc[0], f1);
+    add(acc[1], cmul(e, f1));
+    add(acc[2], cmul(prot, f1));
+    add(acc[3], cmul(neut, f1));
+    add(acc[4], cmul(prot2, f1));
+    add(acc[5], cmul(neut2, f1));
+    if (pari) add(acc[6], cmul(pari[a], w[a]));   // pnp_pari += rot_pari * weight
   }
   for (int k = 0; k < 7; ++k) red[k][threadIdx.x] = acc[k];
   __syncthreads();
@@ -314,6 +326,7 @@ gauge_scalars_kernel(int na, int n, cplx H0, const cplx* __restrict__ w, const c
 
 // pnp_A = sum f1 A, pnp_H20 = sum f1 H20, pnp_AH20 = sum f2 A   (angles in order)
 __global__ void gauge_matrices_kernel(long long nn, int na, const cplx* __restrict__ f12,
+                                      const double* __restrict__ mult,
                                       const cplx* __restrict__ Aphi, const cplx* __restrict__ H20,
                                       long long stride, cplx* __restrict__ pA,
                                       cplx* __restrict__ pH, cplx* __restrict__ pAH) {
@@ -323,9 +336,16 @@ __global__ void gauge_matrices_kernel(long long nn, int na, const cplx* __restri
   for (int a = 0; a < na; ++a) {
     const cplx f1 = f12[2 * a], f2 = f12[2 * a + 1];
     const cplx av = Aphi[a * stride + idx], hv = H20[a * stride + idx];
-    cfma(sa, av, f1);
-    cfma(sh, hv, f1);
-    cfma(sah, av, f2);
+    if (mult[a] > 1.5) {        // conjugate pair: 2 Re(.)
+      const cplx t1 = cmul(av, f1), t2 = cmul(hv, f1), t3 = cmul(av, f2);
+      sa.x += 2.0 * t1.x;
+      sh.x += 2.0 * t2.x;
+      sah.x += 2.0 * t3.x;
+    } else {
+      cfma(sa, av, f1);
+      cfma(sh, hv, f1);
+      cfma(sah, av, f2);
+    }
   }
   pA[idx] = sa;
   pH[idx] = sh;
@@ -367,19 +387,39 @@ projected_h20_kernel(int n, const cplx* __restrict__ pnp, const cplx* __restrict
 // overlap assembly: o = pre * phase * exp(logabs + lnorm)
 __global__ void overlap_finish_kernel(int na, const double* __restrict__ logabs,
                                       const cplx* __restrict__ phase, const int* __restrict__ zero,
-                                      const cplx* __restrict__ pre, double lnorm,
+                                      const cplx* __restrict__ pre, const double* __restrict__ lnorm,
                                       cplx* __restrict__ over) {
   int a = blockIdx.x * NT + threadIdx.x;
   if (a >= na) return;
   if (zero[a]) { over[a] = cmk(0.0, 0.0); return; }
-  const double m = exp(logabs[a] + lnorm);
+  const double m = exp(logabs[a] + lnorm[0]);
   const cplx z = cmul(pre[a], phase[a]);
   over[a] = cmk(m * z.x, m * z.y);
 }
 
-__global__ void fill_cplx_kernel(int na, cplx v, cplx* __restrict__ x) {
-  int a = blockIdx.x * NT + threadIdx.x;
+__global__ void fill_cplx_kernel(long long na, cplx v, cplx* __restrict__ x) {
+  long long a = (long long)blockIdx.x * NT + threadIdx.x;
   if (a < na) x[a] = v;
+}
+
+__global__ void fill_from_kernel(int na, const cplx* __restrict__ v, cplx* __restrict__ x) {
+  int a = blockIdx.x * NT + threadIdx.x;
+  if (a < na) x[a] = v[0];
+}
+
+// Pfaffian of an empty block: log|pf| = 0, phase 1
+__global__ void unit_pfaffian_kernel(int na, double* __restrict__ logabs, cplx* __restrict__ phase,
+                                     int* __restrict__ zero) {
+  int a = blockIdx.x * NT + threadIdx.x;
+  if (a >= na) return;
+  logabs[a] = 0.0;
+  phase[a] = cmk(1.0, 0.0);
+  zero[a] = 0;
+}
+
+__global__ void copy_real_kernel(long long nn, const double* __restrict__ src, double* __restrict__ dst) {
+  long long i = (long long)blockIdx.x * NT + threadIdx.x;
+  if (i < nn) dst[i] = src[i];
 }
 
 // pre[a] = signfac * det[a] for the direct overlap
@@ -456,12 +496,174 @@ int sign_pow(long long e) { return (e % 2 == 0) ? 1 : -1; }
 enum { W_UPHI, W_VPHI, W_UT, W_VT, W_RHO, W_KLR, W_KRL, W_G, W_H, W_DLR, W_DRL, W_T, W_X, W_Y,
        W_Z, W_COUNT };
 
-int project_device(tg_ctx* c) {
-  const int n = c->n, h = n / 2, A = c->A;
-  const long long nn = (long long)n * n;
+// Device pointers and host scalars of one projection of the resident state
+// (pure host arithmetic on the context; no HIP calls).
+struct RunPlan {
+  int n, h, A, chunk;
+  long long nn;
+  bool proj, need_ovl, reduced;
+  int nocc, nemp, nbloc, n0, n2, signfac;
+  double pre;                               // reduced-overlap prefactor
+  cplx *work, *Akeep, *Hkeep;
+  cplx *d_ph, *d_tr, *d_over, *d_ener, *d_f12, *d_pnp, *d_pre, *d_det, *d_phase, *d_w, *d_pari;
+  double *d_logabs, *d_imag, *d_mult;
+  int *d_info, *d_zero, *d_perm;
+  double *d_T, *d_sg, *d_ovac, *d_vou;      // overlap set-up uploaded with the state
+  cplx* d_pre0;
+  size_t work_bytes, scal_bytes, int_bytes, ovl_bytes, pfw_bytes;
+};
+
+RunPlan make_plan(tg_ctx* c) {
+  RunPlan P{};
+  P.n = c->n;
+  P.h = P.n / 2;
+  P.A = c->A;
+  P.nn = (long long)P.n * P.n;
+  P.chunk = c->plain ? 1 : P.A;
+  P.proj = std::max(c->Mphip, c->Mphin) > 1;
+  P.need_ovl = P.proj || c->iprint;
+  P.nocc = c->nocc;
+  P.nemp = c->nemp;
+  P.nbloc = P.n - P.nocc - P.nemp;
+  P.reduced = P.nocc == 0 && P.nbloc > 0 && (P.nbloc % 2 == 0);
+  // signfac of :811-814 with state 0 = state 1
+  P.n0 = P.n - P.nocc;
+  P.n2 = P.n - P.nemp;
+  const long long e1 = (long long)P.n2 * (P.n2 + 1) / 2;
+  const long long e2 = (long long)P.nocc * (P.nocc - 1) / 2;
+  const long long e3 = (long long)P.nocc * (P.nocc + 1) + (long long)P.nocc * (P.n0 + P.n2) +
+                       (long long)P.nocc * P.n0;
+  P.signfac = sign_pow(e1) * sign_pow(e2) * sign_pow(e3);
+  double prod_sgn = 1.0;
+  if (P.reduced)
+    for (int q = 0; q < P.nbloc / 2; ++q)
+      if (c->voveru[P.nemp + 2 * q] < 0) prod_sgn = -prod_sgn;
+  P.pre = P.signfac * prod_sgn * sign_pow(P.nbloc / 2);
+  const int A = P.A, n = P.n;
+  const long long nn = P.nn;
+  const size_t wsz = (size_t)W_COUNT * P.chunk * nn;
+  P.work_bytes = sizeof(cplx) * (wsz + 2 * (size_t)A * nn);
+  // scalars: ph (2A) | traces (9A) | over (A) | ener (A) | f12 (2A) | pnp (8) | pre (A) | det (A)
+  //          | phase (A) | w (A) | pari (A) | logabs (A doubles) | imag (doubles)
+  P.scal_bytes = sizeof(cplx) * (20 * (size_t)A + 8) + sizeof(double) * A +
+                 sizeof(double) * (nn / NT + 2) + sizeof(double) * A;
+  P.int_bytes = sizeof(int) * ((size_t)A * (2 * n + 4) + 8);
+  // overlap set-up: T (n) | sg (n) | ovac (2) | pre0 (cplx) | v/u (n)
+  P.ovl_bytes = sizeof(double) * (3 * (size_t)n + 8);
+  P.pfw_bytes = 0;
+  if (P.need_ovl && !P.reduced) {
+    const int nreg = 2 * P.nbloc;
+    const long long m2 = (long long)P.n2 * P.n2, mr = (long long)nreg * nreg;
+    P.pfw_bytes = sizeof(cplx) * (size_t)P.chunk * (m2 + 2 * mr);
+  }
+  P.work = c->d_work.as<cplx>();
+  P.Akeep = P.work + wsz;
+  P.Hkeep = P.Akeep + (size_t)A * nn;
+  P.d_ph = c->d_scal.as<cplx>();
+  P.d_tr = P.d_ph + 2 * A;
+  P.d_over = P.d_tr + 9 * A;
+  P.d_ener = P.d_over + A;
+  P.d_f12 = P.d_ener + A;
+  P.d_pnp = P.d_f12 + 2 * A;
+  P.d_pre = P.d_pnp + 8;
+  P.d_det = P.d_pre + A;
+  P.d_phase = P.d_det + A;
+  P.d_w = P.d_phase + A;
+  P.d_pari = P.d_w + A;
+  P.d_logabs = reinterpret_cast<double*>(P.d_pari + A);
+  P.d_imag = P.d_logabs + A;
+  P.d_mult = P.d_imag + (nn / NT + 2);
+  P.d_info = c->d_int.as<int>();
+  P.d_zero = P.d_info + A;
+  P.d_perm = P.d_zero + A;
+  P.d_T = c->d_ovl.as<double>();
+  P.d_sg = P.d_T + n;
+  P.d_ovac = P.d_sg + n;
+  P.d_pre0 = reinterpret_cast<cplx*>(P.d_ovac + 2);
+  P.d_vou = P.d_ovac + 4;
+  return P;
+}
+
+// Allocations, uploads of the state-dependent set-up (angles, weights, overlap
+// scaling, ln-norm and prefactor) and, on the first call at a size, the GEMM
+// tile autotuning.  Runs eagerly, never under stream capture.
+int project_upload(tg_ctx* c) {
+  RunPlan P = make_plan(c);
+  const int n = P.n, A = P.A;
+  const long long nn = P.nn;
   hipStream_t s = c->stream;
-  const int chunk = c->plain ? 1 : A;
-  const bool proj = std::max(c->Mphip, c->Mphin) > 1;
+  TG_TRY(c->d_work.ensure(P.work_bytes));
+  TG_TRY(c->d_scal.ensure(P.scal_bytes));
+  TG_TRY(c->d_int.ensure(P.int_bytes));
+  TG_TRY(c->d_ovl.ensure(P.ovl_bytes));
+  TG_TRY(c->d_out.ensure(sizeof(cplx) * 3 * nn + sizeof(double) * nn));
+  if (P.pfw_bytes) TG_TRY(c->d_pfw.ensure(P.pfw_bytes));
+  P = make_plan(c);   // pointers of the (re)allocated buffers
+
+  std::vector<cplx> hph(2 * A), hw(A);
+  for (int a = 0; a < A; ++a) {
+    hph[2 * a] = cmk(cos(c->phip[a]), sin(c->phip[a]));
+    hph[2 * a + 1] = cmk(cos(c->phin[a]), sin(c->phin[a]));
+    hw[a] = c->weight[a];
+  }
+  std::vector<double> ovl(3 * (size_t)n + 8, 0.0);
+  double* Tsc = ovl.data();
+  double* sg = Tsc + n;
+  if (P.reduced)
+    for (int q = 0; q < P.nbloc / 2; ++q) {
+      const double x = c->voveru[P.nemp + 2 * q];
+      sg[q] = (x < 0) ? -1.0 : 1.0;
+      Tsc[2 * q] = Tsc[2 * q + 1] = sqrt(fabs(x));
+    }
+  sg[n] = (P.nbloc == 0) ? 0.0 : c->ovac;                 // d_ovac
+  sg[n + 2] = P.pre;                                      // d_pre0 (re, im = 0)
+  for (int i = 0; i < P.nbloc; ++i) sg[n + 4 + i] = c->voveru[P.nemp + i];   // d_vou
+  TG_HIP(hipMemcpyAsync(P.d_ph, hph.data(), sizeof(cplx) * 2 * A, hipMemcpyHostToDevice, s));
+  TG_HIP(hipMemcpyAsync(P.d_w, hw.data(), sizeof(cplx) * A, hipMemcpyHostToDevice, s));
+  TG_HIP(hipMemcpyAsync(P.d_mult, c->mult.data(), sizeof(double) * A, hipMemcpyHostToDevice, s));
+  TG_HIP(hipMemcpyAsync(P.d_T, ovl.data(), sizeof(double) * ovl.size(), hipMemcpyHostToDevice, s));
+  TG_HIP(hipStreamSynchronize(s));   // host vectors go out of scope
+
+  // first iteration at this size: autotune the tile shapes of the batched GEMMs
+  // of the loop body on zeroed scratch operands (W_T, W_X -> W_Y)
+  const int chunk = P.chunk;
+  if (c->tuned_n != n || c->tuned_chunk != chunk || c->tuned_nb != P.nbloc) {
+    auto Wb = [&](int k) { return P.work + (size_t)k * chunk * nn; };
+    const double* U = c->d_U.as<double>();
+    cplx* Ta = Wb(W_T);
+    cplx* Xa = Wb(W_X);
+    cplx* Ya = Wb(W_Y);
+    const long long tot = nn * chunk;
+    hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(tot)), dim3(NT), 0, s, tot, cmk(0, 0), Ta);
+    hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(tot)), dim3(NT), 0, s, tot, cmk(0, 0), Xa);
+    TG_HIP(hipGetLastError());
+    TG_TRY(gemm_tune_zz(s, OP_N, OP_N, n, n, n, Xa, n, nn, Ta, n, nn, Ya, n, nn, chunk));
+    TG_TRY(gemm_tune_zz(s, OP_T, OP_N, n, n, n, Xa, n, nn, Ta, n, nn, Ya, n, nn, chunk));
+    TG_TRY(gemm_tune_dz(s, OP_N, OP_N, n, n, n, U, n, 0, Xa, n, nn, Ya, n, nn, chunk));
+    TG_TRY(gemm_tune_dz(s, OP_N, OP_T, n, n, n, U, n, 0, Xa, n, nn, Ya, n, nn, chunk));
+    TG_TRY(gemm_tune_zd(s, OP_N, OP_T, n, n, n, Xa, n, nn, U, n, 0, Ya, n, nn, chunk));
+    if (P.reduced && P.need_ovl) {
+      const int nb = P.nbloc;
+      const long long mm = (long long)nb * nb;
+      TG_TRY(gemm_tune_zz(s, OP_T, OP_N, nb, nb, nb, Xa, nb, mm, Ta, nb, mm, Ya, nb, mm, chunk));
+    }
+    c->tuned_n = n;
+    c->tuned_chunk = chunk;
+    c->tuned_nb = P.nbloc;
+  }
+  c->state_dirty = 0;
+  return TG_OK;
+}
+
+// The projection proper: kernel launches only (capturable into a hipGraph).
+// Buffers and the uploaded set-up come from project_upload.
+int project_run(tg_ctx* c) {
+  const RunPlan P = make_plan(c);
+  const int n = P.n, h = P.h, A = P.A;
+  const long long nn = P.nn;
+  hipStream_t s = c->stream;
+  const int chunk = P.chunk;
+  const bool proj = P.proj;
   const double* U = c->d_U.as<double>();
   const double* V = c->d_V.as<double>();
   const double* D = c->d_D.as<double>();
@@ -493,80 +695,32 @@ int project_device(tg_ctx* c) {
   const double* Pp = R6 + 6 * nn;
   const double* Pn = R6 + 7 * nn;
 
-  // ---- work areas
-  const size_t wsz = (size_t)W_COUNT * chunk * nn;
-  TG_TRY(c->d_work.ensure(sizeof(cplx) * (wsz + 2 * (size_t)A * nn)));
-  cplx* work = c->d_work.as<cplx>();
-  auto Wb = [&](int k) { return work + (size_t)k * chunk * nn; };
-  cplx* Akeep = work + wsz;                  // A_phi of every angle
-  cplx* Hkeep = Akeep + (size_t)A * nn;      // H20_phi of every angle
-  // scalars: ph (2A) | traces (9A) | over (A) | ener (A) | f12 (2A) | pnp (8) | pre (A) | det (A)
-  //          | phase (A) | w (A) | pari (A) | logabs (A doubles) | imag (doubles)
-  TG_TRY(c->d_scal.ensure(sizeof(cplx) * (20 * (size_t)A + 8) + sizeof(double) * A +
-                          sizeof(double) * (nn / NT + 2)));
-  cplx* d_ph = c->d_scal.as<cplx>();
-  cplx* d_tr = d_ph + 2 * A;
-  cplx* d_over = d_tr + 9 * A;
-  cplx* d_ener = d_over + A;
-  cplx* d_f12 = d_ener + A;
-  cplx* d_pnp = d_f12 + 2 * A;
-  cplx* d_pre = d_pnp + 8;
-  cplx* d_det = d_pre + A;
-  cplx* d_phase = d_det + A;
-  cplx* d_w = d_phase + A;
-  cplx* d_pari = d_w + A;
-  double* d_logabs = reinterpret_cast<double*>(d_pari + A);
-  double* d_imag = d_logabs + A;
-  TG_TRY(c->d_int.ensure(sizeof(int) * ((size_t)A * (2 * n + 4) + 8)));
-  int* d_info = c->d_int.as<int>();
-  int* d_zero = d_info + A;
-  int* d_perm = d_zero + A;
-
-  // angles, phases and weights
-  {
-    std::vector<cplx> hph(2 * A), hw(A);
-    for (int a = 0; a < A; ++a) {
-      hph[2 * a] = cmk(cos(c->phip[a]), sin(c->phip[a]));
-      hph[2 * a + 1] = cmk(cos(c->phin[a]), sin(c->phin[a]));
-      hw[a] = c->weight[a];
-    }
-    TG_HIP(hipMemcpyAsync(d_ph, hph.data(), sizeof(cplx) * 2 * A, hipMemcpyHostToDevice, s));
-    TG_HIP(hipMemcpyAsync(d_w, hw.data(), sizeof(cplx) * A, hipMemcpyHostToDevice, s));
-    TG_HIP(hipStreamSynchronize(s));   // host vectors go out of scope
-  }
-
-  // ---- overlap set-up (per iteration, host scalars)
-  const int nocc = c->nocc, nemp = c->nemp;
-  const int nbloc = n - nocc - nemp;
-  const bool reduced = nocc == 0 && nbloc > 0 && (nbloc % 2 == 0);
-  const double ovac = (nbloc == 0) ? 0.0 : c->ovac;
-  // signfac of :811-814 with state 0 = state 1
-  const int n0 = n - nocc, n2 = n - nemp;
-  const long long e1 = (long long)n2 * (n2 + 1) / 2;
-  const long long e2 = (long long)nocc * (nocc - 1) / 2;
-  const long long e3 = (long long)nocc * (nocc + 1) + (long long)nocc * (n0 + n2) + (long long)nocc * n0;
-  const int signfac = sign_pow(e1) * sign_pow(e2) * sign_pow(e3);
-  std::vector<double> Tsc, sg;
-  double prod_sgn = 1.0;
-  const bool need_ovl = proj || c->iprint;
-  if (reduced && need_ovl) {
-    Tsc.resize(nbloc);
-    sg.resize(nbloc / 2);
-    for (int p = 0; p < nbloc / 2; ++p) {
-      const double x = c->voveru[nemp + 2 * p];
-      sg[p] = (x < 0) ? -1.0 : 1.0;
-      prod_sgn *= sg[p];
-      Tsc[2 * p] = Tsc[2 * p + 1] = sqrt(fabs(x));
-    }
-  }
-  TG_TRY(c->d_ovl.ensure(sizeof(double) * (2 * (size_t)n + 8)));
-  double* d_T = c->d_ovl.as<double>();
-  double* d_sg = d_T + n;
-  if (reduced && need_ovl) {
-    TG_HIP(hipMemcpyAsync(d_T, Tsc.data(), sizeof(double) * nbloc, hipMemcpyHostToDevice, s));
-    TG_HIP(hipMemcpyAsync(d_sg, sg.data(), sizeof(double) * (nbloc / 2), hipMemcpyHostToDevice, s));
-    TG_HIP(hipStreamSynchronize(s));
-  }
+  auto Wb = [&](int k) { return P.work + (size_t)k * chunk * nn; };
+  cplx* Akeep = P.Akeep;                     // A_phi of every angle
+  cplx* Hkeep = P.Hkeep;                     // H20_phi of every angle
+  cplx* d_ph = P.d_ph;
+  cplx* d_tr = P.d_tr;
+  cplx* d_over = P.d_over;
+  cplx* d_ener = P.d_ener;
+  cplx* d_f12 = P.d_f12;
+  cplx* d_pnp = P.d_pnp;
+  cplx* d_pre = P.d_pre;
+  cplx* d_det = P.d_det;
+  cplx* d_phase = P.d_phase;
+  cplx* d_w = P.d_w;
+  cplx* d_pari = P.d_pari;
+  double* d_logabs = P.d_logabs;
+  double* d_imag = P.d_imag;
+  int* d_info = P.d_info;
+  int* d_zero = P.d_zero;
+  int* d_perm = P.d_perm;
+  const int nemp = P.nemp;
+  const int nbloc = P.nbloc;
+  const bool reduced = P.reduced;
+  const int n2 = P.n2;
+  const int signfac = P.signfac;
+  double* d_T = P.d_T;
+  double* d_sg = P.d_sg;
 
   // ---- loop over angle chunks
   for (int a0 = 0; a0 < A; a0 += chunk) {
@@ -660,10 +814,9 @@ int project_device(tg_ctx* c) {
           ScopedTiming tk(c, "kernel:pfaffian");
           TG_TRY(batched_pfaffian(s, nb, K, nb, mm, na, T, d_logabs, d_phase, d_zero));
         }
-        const double pre = signfac * prod_sgn * sign_pow(nb / 2);
-        hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, cmk(pre, 0.0), d_pre);
+        hipLaunchKernelGGL(fill_from_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, P.d_pre0, d_pre);
         hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, d_logabs,
-                           d_phase, d_zero, d_pre, ovac, dst);
+                           d_phase, d_zero, d_pre, P.d_ovac, dst);
         TG_HIP(hipGetLastError());
         return TG_OK;
       }
@@ -671,14 +824,10 @@ int project_device(tg_ctx* c) {
       ScopedTiming tm(c, "overlap");
       const int nreg = 2 * nbloc;
       const long long m2 = (long long)n2 * n2, mr = (long long)nreg * nreg;
-      TG_TRY(c->d_pfw.ensure(sizeof(cplx) * (size_t)na * (m2 + 2 * mr) + sizeof(double) * 2 * n));
       cplx* R2 = c->d_pfw.as<cplx>();
       cplx* M = R2 + (size_t)na * m2;
       cplx* PW = M + (size_t)na * mr;
-      double* d_vou = reinterpret_cast<double*>(PW + (size_t)na * mr);
-      if (nbloc > 0)
-        TG_HIP(hipMemcpyAsync(d_vou, c->voveru.data() + nemp, sizeof(double) * nbloc,
-                              hipMemcpyHostToDevice, s));
+      const double* d_vou = P.d_vou;
       hipLaunchKernelGGL(build_rn2_kernel, dim3(nblk(m2 * na)), dim3(NT), 0, s, n, nemp, n2, na, Pa,
                          Pb, d_ph + 2 * a0, R2);
       TG_HIP(hipGetLastError());
@@ -689,18 +838,14 @@ int project_device(tg_ctx* c) {
         TG_HIP(hipGetLastError());
         TG_TRY(batched_pfaffian(s, nreg, M, nreg, mr, na, PW, d_logabs, d_phase, d_zero));
       } else {
-        std::vector<double> zl(na, 0.0);
-        std::vector<cplx> on(na, cmk(1, 0));
-        std::vector<int> zz(na, 0);
-        TG_HIP(hipMemcpyAsync(d_logabs, zl.data(), sizeof(double) * na, hipMemcpyHostToDevice, s));
-        TG_HIP(hipMemcpyAsync(d_phase, on.data(), sizeof(cplx) * na, hipMemcpyHostToDevice, s));
-        TG_HIP(hipMemcpyAsync(d_zero, zz.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
-        TG_HIP(hipStreamSynchronize(s));
+        hipLaunchKernelGGL(unit_pfaffian_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, d_logabs,
+                           d_phase, d_zero);
+        TG_HIP(hipGetLastError());
       }
       hipLaunchKernelGGL(scale_det_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, (double)signfac,
                          d_det, d_pre);
       hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, d_logabs,
-                         d_phase, d_zero, d_pre, ovac, dst);
+                         d_phase, d_zero, d_pre, P.d_ovac, dst);
       TG_HIP(hipGetLastError());
       return TG_OK;
     };
@@ -715,15 +860,14 @@ int project_device(tg_ctx* c) {
   }
 
   // ---- sum_gauge and calculate_projected_H20
-  TG_TRY(c->d_out.ensure(sizeof(cplx) * 3 * nn + sizeof(double) * nn));
   cplx* pA = c->d_out.as<cplx>();
   cplx* pH = pA + nn;
   cplx* pAH = pH + nn;
   double* H20v = reinterpret_cast<double*>(pAH + nn);
   hipLaunchKernelGGL(gauge_scalars_kernel, dim3(1), dim3(NT), 0, s, A, n, c->H0, d_w, d_over, d_tr,
-                     c->iprint ? d_pari : nullptr, d_f12, d_ener, d_pnp);
-  hipLaunchKernelGGL(gauge_matrices_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, A, d_f12, Akeep,
-                     Hkeep, nn, pA, pH, pAH);
+                     c->iprint ? d_pari : nullptr, P.d_mult, d_f12, d_ener, d_pnp);
+  hipLaunchKernelGGL(gauge_matrices_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, A, d_f12, P.d_mult,
+                     Akeep, Hkeep, nn, pA, pH, pAH);
   hipLaunchKernelGGL(projected_h20_kernel, dim3(nblk(nn)), dim3(NT), 0, s, n, d_pnp, pA, pH, pAH,
                      H20v, d_imag);
   TG_HIP(hipGetLastError());
@@ -738,8 +882,79 @@ int project_device(tg_ctx* c) {
     hipLaunchKernelGGL(gradient_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, c->grad_eta, c->grad_mu,
                        c->grad_ncons, lam, c->d_c20.as<double>(), H20v, Zprev, Zcur, part);
     TG_HIP(hipGetLastError());
-    TG_HIP(hipMemcpyAsync(Zprev, Zcur, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(copy_real_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, Zcur, Zprev);
+    TG_HIP(hipGetLastError());
   }
+  return TG_OK;
+}
+
+// Graph key: everything project_run passes by value or sizes by, plus the
+// allocation epoch (a reallocated buffer invalidates captured pointers).
+GraphKey graph_key(const tg_ctx* c) {
+  GraphKey k{};
+  double fd = c->factor_delta, eta = c->grad_eta, mu = c->grad_mu;
+  long long b1, b2, b3;
+  std::memcpy(&b1, &fd, 8);
+  std::memcpy(&b2, &eta, 8);
+  std::memcpy(&b3, &mu, 8);
+  const long long vals[16] = {c->n, c->A, c->Mphip, c->Mphin, c->plain, c->nocc, c->nemp,
+                              c->iprint, c->grad_on, c->grad_ncons, b1, b2, b3,
+                              (long long)devbuf_epoch(), c->w_fp32, 1};
+  for (int i = 0; i < 16; ++i) k.v[i] = vals[i];
+  return k;
+}
+
+void graph_release(tg_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_valid = 0;
+}
+
+// One projection of the resident state.  With TG_FLAG_GRAPH the kernel
+// sequence is captured into a hipGraph the second time a configuration is seen
+// (the first, eager run allocates and autotunes) and replayed afterwards; a new
+// state of the same configuration only re-uploads its set-up.
+int project_device(tg_ctx* c) {
+  if (c->state_dirty) TG_TRY(project_upload(c));
+  if (!c->use_graph || c->timing) return project_run(c);
+  const GraphKey k = graph_key(c);
+  if (c->graph_valid && k == c->graph_key_v) {
+    TG_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    return TG_OK;
+  }
+  if (!(c->graph_seen && k == c->graph_seen_key)) {
+    c->graph_seen = 1;
+    c->graph_seen_key = k;
+    return project_run(c);
+  }
+  graph_release(c);
+  TG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int r = project_run(c);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (r != TG_OK || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (r != TG_OK) return r;
+    tg_set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    return TG_E_HIP;
+  }
+  c->graph = g;
+  if (hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    c->graph_exec = nullptr;
+    graph_release(c);
+    tg_set_error("hipGraphInstantiate failed");
+    return TG_E_HIP;
+  }
+  c->graph_valid = 1;
+  c->graph_key_v = k;
+  if (std::getenv("TG_GRAPH_VERBOSE")) {
+    size_t nodes = 0;
+    (void)hipGraphGetNodes(g, nullptr, &nodes);
+    fprintf(stderr, "[taurus_gpu] projection captured: %zu graph nodes\n", nodes);
+  }
+  TG_HIP(hipGraphLaunch(c->graph_exec, c->stream));
   return TG_OK;
 }
 
@@ -772,24 +987,39 @@ int upload_state(tg_ctx* c, const double* U0, const double* V0, const double* D0
   c->valN = valN;
   c->factor_delta = factor_delta;
   c->iprint = iprint ? 1 : 0;
-  // assign_angles (:112-185)
-  c->A = Mphip * Mphin;
-  c->phip.resize(c->A);
-  c->phin.resize(c->A);
-  c->weight.resize(c->A);
-  int i = 0;
+  // assign_angles (:112-185).  With TG_FLAG_CONJ_PAIRS and the full [0, 2 pi)
+  // grid, the grid is closed under phi -> -phi (mod 2 pi): for real U, V every
+  // per-angle quantity at -phi is the complex conjugate of that at phi, and so
+  // is the weight, so only one angle of each pair is computed and enters the
+  // gauge sums twice, as 2 Re(.); self-conjugate angles enter once.
+  const bool pairs = c->conj_pairs && Mphip * Mphin > 1 &&
+                     fabs(facpi - 2.0 * M_PI) <= 1e-12 * 2.0 * M_PI;
+  auto partner = [](int j, int M) { return (M % 2 == 1) ? (j == 1 ? 1 : M - j + 2) : M - j + 1; };
+  c->phip.clear();
+  c->phin.clear();
+  c->weight.clear();
+  c->mult.clear();
   for (int jp = 1; jp <= Mphip; ++jp)
     for (int jn = 1; jn <= Mphin; ++jn) {
+      double mult = 1.0;
+      if (pairs) {
+        const int i0 = (jp - 1) * Mphin + (jn - 1);
+        const int i1 = (partner(jp, Mphip) - 1) * Mphin + (partner(jn, Mphin) - 1);
+        if (i1 < i0) continue;        // computed as the partner of i1
+        if (i1 > i0) mult = 2.0;
+      }
       const double pp = (Mphip % 2 == 1) ? facpi * (jp - 1.0) / Mphip : facpi * (jp - 0.5) / Mphip;
       const double pn = (Mphin % 2 == 1) ? facpi * (jn - 1.0) / Mphin : facpi * (jn - 0.5) / Mphin;
-      c->phip[i] = pp;
-      c->phin[i] = pn;
+      c->phip.push_back(pp);
+      c->phin.push_back(pn);
       const cplx wp = cmk(cos(-pp * valZ) / Mphip, sin(-pp * valZ) / Mphip);
       const cplx wn = cmk(cos(-pn * valN) / Mphin, sin(-pn * valN) / Mphin);
-      c->weight[i] = cmul(wp, wn);
-      ++i;
+      c->weight.push_back(cmul(wp, wn));
+      c->mult.push_back(mult);
     }
+  c->A = (int)c->phip.size();
   c->have_state = 1;
+  c->state_dirty = 1;
   return TG_OK;
 }
 
@@ -799,11 +1029,9 @@ int fetch_results(tg_ctx* c, double* scal, double* pA, double* pH, double* pAH, 
   const size_t nn = (size_t)n * n;
   TG_HIP(hipStreamSynchronize(c->stream));
   const cplx* d_out = c->d_out.as<cplx>();
-  cplx* d_ph = c->d_scal.as<cplx>();
-  const int A = c->A;
-  cplx* d_pnp = d_ph + 2 * A + 9 * A + A + A + 2 * A;
-  cplx* d_w = d_pnp + 8 + 3 * A;
-  double* d_imag = reinterpret_cast<double*>(d_w + 2 * A) + A;
+  const RunPlan P = make_plan(c);
+  const cplx* d_pnp = P.d_pnp;
+  const double* d_imag = P.d_imag;
   if (scal) TG_HIP(hipMemcpy(scal, d_pnp, sizeof(cplx) * 7, hipMemcpyDeviceToHost));
   if (pA) TG_HIP(hipMemcpy(pA, d_out, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
   if (pH) TG_HIP(hipMemcpy(pH, d_out + nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
@@ -856,6 +1084,8 @@ int tg_init(tg_ctx** out, int ndim, const int32_t* sp_2mj, const int32_t* sp_2mt
   c->flags = flags;
   c->plain = (flags & TG_FLAG_PLAIN) ? 1 : 0;
   c->w_fp32 = (flags & TG_FLAG_TBME_FP32) ? 1 : 0;
+  c->use_graph = (flags & TG_FLAG_GRAPH) ? 1 : 0;
+  c->conj_pairs = (flags & TG_FLAG_CONJ_PAIRS) ? 1 : 0;
   c->sp_2mj.assign(sp_2mj, sp_2mj + ndim);
   c->sp_2mt.assign(sp_2mt, sp_2mt + ndim);
   c->sp_l.assign(sp_l, sp_l + ndim);
@@ -898,6 +1128,7 @@ int tg_finalize(tg_ctx* c) {
   if (!c) return TG_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)timing_collect(c);
+  graph_release(c);
   for (DevBuf* b : {&c->d_H1, &c->d_U, &c->d_V, &c->d_D, &c->d_ang, &c->d_real6, &c->d_work,
                     &c->d_blk, &c->d_scal, &c->d_out, &c->d_ovl, &c->d_int, &c->d_pfw, &c->d_c20,
                     &c->d_Z, &c->d_psign})

@@ -27,6 +27,8 @@ LIBPATH = os.environ.get("TAURUS_GPU_LIB", os.path.join(HERE, "libtaurus_gpu.so"
 
 TG_FLAG_PLAIN = 1
 TG_FLAG_TBME_FP32 = 2
+TG_FLAG_GRAPH = 4
+TG_FLAG_CONJ_PAIRS = 8
 
 EXPORTS = [
     "tg_last_error", "tg_init", "tg_finalize", "tg_set_mode", "tg_calculate_fields",
@@ -96,7 +98,7 @@ class Context:
     """Device context: basis + Hamiltonian uploaded once (tg_init)."""
 
     def __init__(self, ndim, sp_2mj, sp_2mt, sp_l, sp_tr, H0, H1, abcd, H2, trperm,
-                 plain=False, tbme_fp32=None):
+                 plain=False, tbme_fp32=None, graph=False, conj_pairs=False):
         lib = load_library()
         self.n = int(ndim)
         H2 = np.ascontiguousarray(H2)
@@ -104,7 +106,8 @@ class Context:
             raise ValueError("H2 must be float32 or float64")
         if tbme_fp32 is None:
             tbme_fp32 = H2.dtype == np.float32
-        flags = (TG_FLAG_PLAIN if plain else 0) | (TG_FLAG_TBME_FP32 if tbme_fp32 else 0)
+        flags = ((TG_FLAG_PLAIN if plain else 0) | (TG_FLAG_TBME_FP32 if tbme_fp32 else 0)
+                 | (TG_FLAG_GRAPH if graph else 0) | (TG_FLAG_CONJ_PAIRS if conj_pairs else 0))
         self._keep = [np.ascontiguousarray(x, dtype=np.int32) for x in (sp_2mj, sp_2mt, sp_l, sp_tr)]
         H1 = _c(H1, np.complex128)
         abcd = np.ascontiguousarray(np.asarray(abcd, dtype=np.int16).reshape(-1))
@@ -198,8 +201,8 @@ class Context:
         imag = np.zeros(1)
         _check(load_library().tg_get_results(self._h, _p(scal), None, None, None, _p(H20v), _p(imag)),
                "tg_get_results")
-        return dict(over=scal[0], ener=scal[1], prot=scal[2], neut=scal[3], H20v=H20v,
-                    imag=float(imag[0]))
+        return dict(over=scal[0], ener=scal[1], prot=scal[2], neut=scal[3], prot2=scal[4],
+                    neut2=scal[5], pari=scal[6], H20v=H20v, imag=float(imag[0]))
 
     # calculate_gradient (gradient_type 0) on the device after each resident step
     def set_gradient(self, gradient_eta, gradient_mu, constraint_20=None, lagrange_lambda1=None):

@@ -135,13 +135,14 @@ def seed_state(w: I.Workload, basis: I.Basis, seed: int):
 
 
 def workload_context(name: str, seed: int = 1234, tbme_fp32: bool = True, plain: bool = False,
-                     ham: dict | None = None):
+                     ham: dict | None = None, graph: bool = False, conj_pairs: bool = False):
     """Device context + seed state of a named synthetic workload."""
     w = I.WORKLOADS[name]
     h = ham if ham is not None else H.workload_hamiltonian(w, seed)
     b = h["basis"]
     H2 = h["H2"].astype(np.float32) if tbme_fp32 else h["H2"]
     ctx = gpu.Context(b.dim, b.sp_2mj, b.sp_2mt, b.sp_l, b.sp_tr, h["H0"], h["H1"], h["abcd"], H2,
-                      h["trperm"], plain=plain, tbme_fp32=tbme_fp32)
+                      h["trperm"], plain=plain, tbme_fp32=tbme_fp32, graph=graph,
+                      conj_pairs=conj_pairs)
     U, V = seed_state(w, b, seed)
     return ctx, h, w, U, V

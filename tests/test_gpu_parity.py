@@ -17,11 +17,11 @@ pytestmark = pytest.mark.gpu
 CASES = ["sd64", "sd32", "pf64", "sd64_m1", "sd64_m4", "mg24_32"]
 
 
-def _ctx(g, plain=False, fp32=None):
+def _ctx(g, plain=False, fp32=None, conj_pairs=False):
     from taurus_vap_amd import gpu
     return gpu.Context(int(g["ndim"]), g["sp_2mj"], g["sp_2mt"], g["sp_l"], g["sp_tr"],
                        complex(g["H0"]), g["H1"], g["abcd"], g["H2"], g["trperm"], plain=plain,
-                       tbme_fp32=fp32)
+                       tbme_fp32=fp32, conj_pairs=conj_pairs)
 
 
 def _project(ctx, g, **kw):
@@ -95,10 +95,13 @@ def test_overlap_matches_reference(case):
 
 @pytest.mark.parametrize("plain", [False, True])
 @pytest.mark.parametrize("case", CASES)
-def test_projection_matches_reference(case, plain):
+@pytest.mark.parametrize("conj_pairs", [False, True])
+def test_projection_matches_reference(case, plain, conj_pairs):
+    """All gauge angles, or (conj_pairs) one angle of each (phi, -phi) pair
+    counted as 2 Re(.), against the reference's full-grid sums."""
     g = load_golden(case)
-    print("project", case, "plain" if plain else "batched", flush=True)
-    ctx = _ctx(g, plain=plain)
+    print("project", case, "plain" if plain else "batched", "pairs" if conj_pairs else "", flush=True)
+    ctx = _ctx(g, plain=plain, conj_pairs=conj_pairs)
     r = _project(ctx, g)
     over = g["pnp_over"]
     assert abs(r["over"] - over) < 1e-10 * max(1.0, abs(over))
@@ -118,10 +121,11 @@ def test_parity_projection_matches_reference():
     Mg24 deck (iter_print = 1); and the same against the oracle for sd/pf."""
     g = load_golden("mg24_32")
     assert int(g["iter_print"]) == 1
-    ctx = _ctx(g)
-    r = _project(ctx, g, iprint=1)
-    ctx.close()
-    assert abs(r["pari"] - g["pnp_pari"]) < 1e-10, (r["pari"], g["pnp_pari"])
+    for pairs in (False, True):
+        ctx = _ctx(g, conj_pairs=pairs)
+        r = _project(ctx, g, iprint=1)
+        ctx.close()
+        assert abs(r["pari"] - g["pnp_pari"]) < 1e-10, (pairs, r["pari"], g["pnp_pari"])
     for case in ("sd64", "pf64"):
         g = load_golden(case)
         cb = golden_canon(g)
@@ -206,3 +210,56 @@ def test_projection_emax4_vs_oracle():
     assert abs(r["over"] - ref["over"]) < 1e-9 * abs(ref["over"])
     assert abs(e - e_ref) < 1e-8 * abs(e_ref)
     assert np.max(np.abs(r["H20v"] - ref["H20v"])) < 1e-9
+
+
+def test_adjoint_gradient_emax4_finite_difference():
+    """e_max=4 no-core basis (140 states), 11x11 gauge angles, FP64 TBMEs: the
+    GPU projected gradient field_H20v (the adjoint dE_proj/dZ of
+    calculate_projected_H20) against central finite differences (h = 1e-6) of
+    the GPU projected energy along real skew Thouless directions
+    (evolve_wavefunction, src/module_constraints.f90:497-590): one single-pair
+    direction Z = e_ij - e_ji (dE/dt = 2 H20v_ij) and two dense random skew
+    directions (dE/dt = sum_kl H20v_kl Z_kl).  The canonical basis of every
+    displaced state is rebuilt on the host, as the reference does per
+    iteration."""
+    from taurus_vap_amd import gpu, hamiltonian as H, host, inputs as I
+    w = I.WORKLOADS["emax4"]
+    h = H.workload_hamiltonian(w, 1234)
+    b = h["basis"]
+    n = b.dim
+    U0, V0 = host.seed_state(w, b, 1234)
+    ctx = gpu.Context(b.dim, b.sp_2mj, b.sp_2mt, b.sp_l, b.sp_tr, h["H0"], h["H1"], h["abcd"],
+                      h["H2"], h["trperm"])
+    M = w.M
+
+    def project(U, V):
+        cb = host.construct_canonical_basis(U, V)
+        return ctx.project_wavefunction(U, V, cb["D"], cb["nocc"], cb["nemp"], cb["ovac"],
+                                        cb["voveru"], M, M, 2 * np.pi, w.Z, w.N)
+
+    def energy(U, V):
+        r = project(U, V)
+        return (r["ener"] / r["over"]).real
+
+    r0 = project(U0, V0)
+    G = r0["H20v"].reshape((n, n), order="F")
+    assert np.max(np.abs(G + G.T)) < 1e-12
+    hstep = 1e-6
+    rng = np.random.default_rng(5)
+    dirs = []
+    Z = np.zeros((n, n)); Z[3, 90], Z[90, 3] = 1.0, -1.0
+    dirs.append(Z)
+    for _ in range(2):
+        X = rng.normal(size=(n, n))
+        dirs.append(np.triu(X, 1) - np.triu(X, 1).T)
+    try:
+        for Z in dirs:
+            Up, Vp = host.evolve(U0, V0, hstep * Z)
+            Um, Vm = host.evolve(U0, V0, -hstep * Z)
+            fd = (energy(Up, Vp) - energy(Um, Vm)) / (2 * hstep)
+            pred = float(np.sum(G * Z))
+            print("emax4 adjoint: fd %.9e  H20v.Z %.9e" % (fd, pred), flush=True)
+            # rounding of E (~1e-12 relative of ~1e2-1e3 MeV) over 2h bounds the FD error
+            assert abs(fd - pred) < 1e-5 * abs(pred) + 2e-3, (fd, pred)
+    finally:
+        ctx.close()
