@@ -169,6 +169,8 @@ struct tg_ctx {
   int flags = 0;
   int plain = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;                  // overlaps, concurrent with the Thouless chain
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // basis
   std::vector<int> sp_2mj, sp_2mt, sp_l, sp_tr;   // sp_tr 0-based here
   cplx H0;
@@ -227,7 +229,8 @@ struct ScopedTiming {
   tg_ctx* c;
   hipEvent_t e0 = nullptr;
   const char* key;
-  ScopedTiming(tg_ctx* c_, const char* k);
+  hipStream_t s;
+  ScopedTiming(tg_ctx* c_, const char* k, hipStream_t st = nullptr);   // default: c->stream
   ~ScopedTiming();
 };
 int timing_collect(tg_ctx* c);

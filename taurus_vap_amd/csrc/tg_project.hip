@@ -437,19 +437,21 @@ namespace {
 
 }  // namespace
 
-ScopedTiming::ScopedTiming(tg_ctx* c_, const char* k) : c(c_), key(k) {
-  if (c->timing && hipEventCreate(&e0) == hipSuccess) (void)hipEventRecord(e0, c->stream);
+ScopedTiming::ScopedTiming(tg_ctx* c_, const char* k, hipStream_t st)
+    : c(c_), key(k), s(st ? st : c_->stream) {
+  if (c->timing && hipEventCreate(&e0) == hipSuccess) (void)hipEventRecord(e0, s);
 }
 ScopedTiming::~ScopedTiming() {
   if (!c->timing || !e0) return;
   hipEvent_t e1 = nullptr;
   if (hipEventCreate(&e1) != hipSuccess) return;
-  (void)hipEventRecord(e1, c->stream);
+  (void)hipEventRecord(e1, s);
   c->pending.push_back({std::string(key), e0, e1});
 }
 int timing_collect(tg_ctx* c) {
   if (c->pending.empty()) return TG_OK;
   TG_HIP(hipStreamSynchronize(c->stream));
+  TG_HIP(hipStreamSynchronize(c->stream2));
   for (auto& p : c->pending) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, p.e0, p.e1) == hipSuccess) {
@@ -550,11 +552,18 @@ RunPlan make_plan(tg_ctx* c) {
   P.int_bytes = sizeof(int) * ((size_t)A * (2 * n + 4) + 8);
   // overlap set-up: T (n) | sg (n) | ovac (2) | pre0 (cplx) | v/u (n)
   P.ovl_bytes = sizeof(double) * (3 * (size_t)n + 8);
+  // overlap work (its own buffers: the overlap runs on the second stream,
+  // concurrently with the Thouless / field / H20 chain)
+  //   reduced: Q | JQ | K | Pfaffian work, chunk * nb^2 complex each
+  //   direct:  R2 | M_reg | Pfaffian work | det (chunk) | info (chunk int) | perm (chunk * n int)
   P.pfw_bytes = 0;
-  if (P.need_ovl && !P.reduced) {
+  if (P.need_ovl && P.reduced) {
+    P.pfw_bytes = sizeof(cplx) * 4 * (size_t)P.chunk * P.nbloc * P.nbloc;
+  } else if (P.need_ovl) {
     const int nreg = 2 * P.nbloc;
     const long long m2 = (long long)P.n2 * P.n2, mr = (long long)nreg * nreg;
-    P.pfw_bytes = sizeof(cplx) * (size_t)P.chunk * (m2 + 2 * mr);
+    P.pfw_bytes = sizeof(cplx) * ((size_t)P.chunk * (m2 + 2 * mr) + P.chunk) +
+                  sizeof(int) * ((size_t)P.chunk * (P.n + 1));
   }
   P.work = c->d_work.as<cplx>();
   P.Akeep = P.work + wsz;
@@ -694,6 +703,10 @@ int project_run(tg_ctx* c) {
   }
   const double* Pp = R6 + 6 * nn;
   const double* Pn = R6 + 7 * nn;
+  // the overlaps depend only on R6 and the set-up: fork them to the second stream
+  hipStream_t s2 = c->stream2;
+  TG_HIP(hipEventRecord(c->ev_fork, s));
+  TG_HIP(hipStreamWaitEvent(s2, c->ev_fork, 0));
 
   auto Wb = [&](int k) { return P.work + (size_t)k * chunk * nn; };
   cplx* Akeep = P.Akeep;                     // A_phi of every angle
@@ -744,6 +757,76 @@ int project_run(tg_ctx* c) {
     cplx* Z = Wb(W_Z);
     cplx* H20 = Hkeep + (size_t)a0 * nn;
 
+    // overlaps <Phi|R|Phi> for the gauge rotation (rot_over) and, when printing,
+    // for parity x gauge (rot_pari, :279-284).  R_n = D0^T R D0 = ep*Pa + en*Pb.
+    auto overlap = [&](const double* Pa, const double* Pb, cplx* dst) -> int {
+      if (reduced) {
+        ScopedTiming tm(c, "overlap", s2);
+        const int nb = nbloc;
+        const long long mm = (long long)nb * nb;
+        cplx* Q = c->d_pfw.as<cplx>();
+        cplx* JQ = Q + (size_t)chunk * mm;
+        cplx* K = JQ + (size_t)chunk * mm;
+        cplx* PW = K + (size_t)chunk * mm;
+        hipLaunchKernelGGL(build_q_kernel, dim3(nblk(mm * na)), dim3(NT), 0, s2, n, nemp, nb, na, Pa,
+                           Pb, d_T, d_sg, d_ph + 2 * a0, Q, JQ);
+        TG_HIP(hipGetLastError());
+        TG_TRY(gemm_zz(s2, OP_T, OP_N, nb, nb, nb, one, Q, nb, mm, JQ, nb, mm, zero, K, nb, mm, na));
+        hipLaunchKernelGGL(add_j_kernel, dim3(nblk((long long)(nb / 2) * na)), dim3(NT), 0, s2, nb,
+                           na, d_sg, K);
+        TG_HIP(hipGetLastError());
+        {
+          ScopedTiming tk(c, "kernel:pfaffian", s2);
+          TG_TRY(batched_pfaffian(s2, nb, K, nb, mm, na, PW, d_logabs, d_phase, d_zero));
+        }
+        hipLaunchKernelGGL(fill_from_kernel, dim3(nblk(na)), dim3(NT), 0, s2, na, P.d_pre0, d_pre);
+        hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s2, na, d_logabs,
+                           d_phase, d_zero, d_pre, P.d_ovac, dst);
+        TG_HIP(hipGetLastError());
+        return TG_OK;
+      }
+      // reference sequence: R2 -> inverse + det -> M_reg -> Pfaffian(nreg)
+      ScopedTiming tm(c, "overlap", s2);
+      const int nreg = 2 * nbloc;
+      const long long m2 = (long long)n2 * n2, mr = (long long)nreg * nreg;
+      cplx* R2 = c->d_pfw.as<cplx>();
+      cplx* M = R2 + (size_t)chunk * m2;
+      cplx* PW = M + (size_t)chunk * mr;
+      cplx* det2 = PW + (size_t)chunk * mr;
+      int* info2 = reinterpret_cast<int*>(det2 + chunk);
+      int* perm2 = info2 + chunk;
+      const double* d_vou = P.d_vou;
+      hipLaunchKernelGGL(build_rn2_kernel, dim3(nblk(m2 * na)), dim3(NT), 0, s2, n, nemp, n2, na, Pa,
+                         Pb, d_ph + 2 * a0, R2);
+      TG_HIP(hipGetLastError());
+      TG_TRY(batched_inverse(s2, n2, R2, n2, m2, na, det2, info2, perm2));
+      if (nbloc > 0) {
+        hipLaunchKernelGGL(build_mreg_kernel, dim3(nblk(mr * na)), dim3(NT), 0, s2, n2, nbloc, nbloc,
+                           nreg, na, d_vou, d_vou, R2, M);
+        TG_HIP(hipGetLastError());
+        TG_TRY(batched_pfaffian(s2, nreg, M, nreg, mr, na, PW, d_logabs, d_phase, d_zero));
+      } else {
+        hipLaunchKernelGGL(unit_pfaffian_kernel, dim3(nblk(na)), dim3(NT), 0, s2, na, d_logabs,
+                           d_phase, d_zero);
+        TG_HIP(hipGetLastError());
+      }
+      hipLaunchKernelGGL(scale_det_kernel, dim3(nblk(na)), dim3(NT), 0, s2, na, (double)signfac,
+                         det2, d_pre);
+      hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s2, na, d_logabs,
+                         d_phase, d_zero, d_pre, P.d_ovac, dst);
+      TG_HIP(hipGetLastError());
+      return TG_OK;
+    };
+    if (!proj) {
+      hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, cmk(1.0, 0.0),
+                         d_over + a0);
+      TG_HIP(hipGetLastError());
+    } else {
+      TG_TRY(overlap(Pp, Pn, d_over + a0));
+    }
+    if (c->iprint) TG_TRY(overlap(R6 + 8 * nn, R6 + 9 * nn, d_pari + a0));
+
+
     // Thouless matrix A_phi (calculate_thouless :684-718)
     {
       ScopedTiming tm(c, "thouless_inverse");
@@ -792,72 +875,11 @@ int project_run(tg_ctx* c) {
                          c->factor_delta, X, Y, Z, H20);
       TG_HIP(hipGetLastError());
     }
-    // overlaps <Phi|R|Phi> for the gauge rotation (rot_over) and, when printing,
-    // for parity x gauge (rot_pari, :279-284).  R_n = D0^T R D0 = ep*Pa + en*Pb.
-    auto overlap = [&](const double* Pa, const double* Pb, cplx* dst) -> int {
-      if (reduced) {
-        ScopedTiming tm(c, "overlap");
-        const int nb = nbloc;
-        const long long mm = (long long)nb * nb;
-        // reuse X/Y/Z/T as Q/JQ/K/pfaffian work (nb <= n)
-        cplx* Q = X;
-        cplx* JQ = Y;
-        cplx* K = Z;
-        hipLaunchKernelGGL(build_q_kernel, dim3(nblk(mm * na)), dim3(NT), 0, s, n, nemp, nb, na, Pa,
-                           Pb, d_T, d_sg, d_ph + 2 * a0, Q, JQ);
-        TG_HIP(hipGetLastError());
-        TG_TRY(gemm_zz(s, OP_T, OP_N, nb, nb, nb, one, Q, nb, mm, JQ, nb, mm, zero, K, nb, mm, na));
-        hipLaunchKernelGGL(add_j_kernel, dim3(nblk((long long)(nb / 2) * na)), dim3(NT), 0, s, nb,
-                           na, d_sg, K);
-        TG_HIP(hipGetLastError());
-        {
-          ScopedTiming tk(c, "kernel:pfaffian");
-          TG_TRY(batched_pfaffian(s, nb, K, nb, mm, na, T, d_logabs, d_phase, d_zero));
-        }
-        hipLaunchKernelGGL(fill_from_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, P.d_pre0, d_pre);
-        hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, d_logabs,
-                           d_phase, d_zero, d_pre, P.d_ovac, dst);
-        TG_HIP(hipGetLastError());
-        return TG_OK;
-      }
-      // reference sequence: R2 -> inverse + det -> M_reg -> Pfaffian(nreg)
-      ScopedTiming tm(c, "overlap");
-      const int nreg = 2 * nbloc;
-      const long long m2 = (long long)n2 * n2, mr = (long long)nreg * nreg;
-      cplx* R2 = c->d_pfw.as<cplx>();
-      cplx* M = R2 + (size_t)na * m2;
-      cplx* PW = M + (size_t)na * mr;
-      const double* d_vou = P.d_vou;
-      hipLaunchKernelGGL(build_rn2_kernel, dim3(nblk(m2 * na)), dim3(NT), 0, s, n, nemp, n2, na, Pa,
-                         Pb, d_ph + 2 * a0, R2);
-      TG_HIP(hipGetLastError());
-      TG_TRY(batched_inverse(s, n2, R2, n2, m2, na, d_det, d_info, d_perm));
-      if (nbloc > 0) {
-        hipLaunchKernelGGL(build_mreg_kernel, dim3(nblk(mr * na)), dim3(NT), 0, s, n2, nbloc, nbloc,
-                           nreg, na, d_vou, d_vou, R2, M);
-        TG_HIP(hipGetLastError());
-        TG_TRY(batched_pfaffian(s, nreg, M, nreg, mr, na, PW, d_logabs, d_phase, d_zero));
-      } else {
-        hipLaunchKernelGGL(unit_pfaffian_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, d_logabs,
-                           d_phase, d_zero);
-        TG_HIP(hipGetLastError());
-      }
-      hipLaunchKernelGGL(scale_det_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, (double)signfac,
-                         d_det, d_pre);
-      hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, d_logabs,
-                         d_phase, d_zero, d_pre, P.d_ovac, dst);
-      TG_HIP(hipGetLastError());
-      return TG_OK;
-    };
-    if (!proj) {
-      hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, cmk(1.0, 0.0),
-                         d_over + a0);
-      TG_HIP(hipGetLastError());
-    } else {
-      TG_TRY(overlap(Pp, Pn, d_over + a0));
-    }
-    if (c->iprint) TG_TRY(overlap(R6 + 8 * nn, R6 + 9 * nn, d_pari + a0));
   }
+
+  // join the overlaps of the second stream
+  TG_HIP(hipEventRecord(c->ev_join, s2));
+  TG_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
 
   // ---- sum_gauge and calculate_projected_H20
   cplx* pA = c->d_out.as<cplx>();
@@ -1100,9 +1122,12 @@ int tg_init(tg_ctx** out, int ndim, const int32_t* sp_2mj, const int32_t* sp_2mt
   }
   c->H0 = cmk(H0_re, H0_im);
   int r = TG_OK;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete c;
-    tg_set_error("tg_init: hipStreamCreate failed");
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    tg_finalize(c);
+    tg_set_error("tg_init: stream / event creation failed");
     return TG_E_HIP;
   }
   r = c->d_H1.ensure(sizeof(cplx) * (size_t)ndim * ndim);
@@ -1136,6 +1161,10 @@ int tg_finalize(tg_ctx* c) {
   for (Channel* ch : {&c->ph, &c->pp})
     for (DevBuf* b : {&ch->d_perm, &ch->d_groups, &ch->d_W, &ch->d_tile_group, &ch->d_tile_n0})
       b->release();
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return TG_OK;
