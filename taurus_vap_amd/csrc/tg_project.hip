@@ -132,19 +132,32 @@ __global__ void add_h1_kernel(long long nn, int na, const cplx* __restrict__ H1,
   h[idx] = cadd(G[idx], H1[idx % nn]);
 }
 
-// H20 = X - X^T + fd * (Y - Z)   (calculate_H20, module_fields.f90:366-391)
-__global__ void h20_combine_kernel(int n, int na, double fd, const cplx* __restrict__ X,
-                                   const cplx* __restrict__ Y, const cplx* __restrict__ Z,
-                                   cplx* __restrict__ H20) {
+// H20 = G - G^T per angle, through 32 x 32 LDS tiles (both reads coalesced).
+// With G = U~^T (h V~ + fd/2 DeltaLR U~) - fd/2 V~^T DeltaRL V~ this is
+// calculate_H20 (module_fields.f90:366-391): U~^T h V~ - (.)^T + fd (U~^T DeltaLR U~
+// - V~^T DeltaRL V~), the two Delta terms being skew.
+__global__ void __launch_bounds__(256)
+skew_part_kernel(int n, const cplx* __restrict__ G, cplx* __restrict__ H) {
+  __shared__ cplx ta[32][33], tb[32][33];
   const long long nn = (long long)n * n;
-  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
-  if (idx >= nn * na) return;
-  const long long a = idx / nn;
-  const int r = (int)(idx % nn);
-  const int i = r % n, j = r / n;
-  const cplx x = X[idx], xt = X[a * nn + j + (long long)i * n];
-  const cplx y = Y[idx], z = Z[idx];
-  H20[idx] = cmk(x.x - xt.x + fd * (y.x - z.x), x.y - xt.y + fd * (y.y - z.y));
+  const long long a = blockIdx.z;
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+  const cplx* g = G + a * nn;
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, j = j0 + r;           // G(i, j): column j, row i (coalesced in i)
+    ta[r][tx] = (i < n && j < n) ? g[i + (long long)j * n] : cmk(0.0, 0.0);
+    const int i2 = j0 + tx, j2 = i0 + r;         // G(j0 + tx, i0 + r)
+    tb[r][tx] = (i2 < n && j2 < n) ? g[i2 + (long long)j2 * n] : cmk(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, j = j0 + r;
+    if (i < n && j < n) {
+      const cplx x = ta[r][tx], xt = tb[tx][r];   // G(i, j), G(j, i)
+      H[a * nn + i + (long long)j * n] = cmk(x.x - xt.x, x.y - xt.y);
+    }
+  }
 }
 
 // Per-angle traces (calculate_particle_number :78-143, calculate_expectval_energy
@@ -751,7 +764,6 @@ int project_run(tg_ctx* c) {
     cplx* hsp = Wb(W_H);
     cplx* dLR = Wb(W_DLR);
     cplx* dRL = Wb(W_DRL);
-    cplx* T = Wb(W_T);
     cplx* X = Wb(W_X);
     cplx* Y = Wb(W_Y);
     cplx* Z = Wb(W_Z);
@@ -862,17 +874,18 @@ int project_run(tg_ctx* c) {
     hipLaunchKernelGGL(traces_kernel, dim3(na), dim3(NT), 0, s, n, c->d_H1.as<cplx>(), rho, kLR,
                        kRL, G, dLR, nn, d_tr + 9 * (size_t)a0);
     TG_HIP(hipGetLastError());
-    // H20 (calculate_H20 with U~, V~)
+    // H20 (calculate_H20 with U~, V~) as G - G^T,
+    //   X1 = h V~ + fd/2 DeltaLR U~,  X2 = -fd/2 DeltaRL V~,  G = U~^T X1 + V~^T X2
     {
       ScopedTiming tm(c, "gemm_h20");
-      TG_TRY(gemm_zz(s, OP_T, OP_N, n, n, n, one, Ut, n, nn, hsp, n, nn, zero, T, n, nn, na));
-      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, T, n, nn, Vt, n, nn, zero, X, n, nn, na));
-      TG_TRY(gemm_zz(s, OP_T, OP_N, n, n, n, one, Ut, n, nn, dLR, n, nn, zero, T, n, nn, na));
-      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, T, n, nn, Ut, n, nn, zero, Y, n, nn, na));
-      TG_TRY(gemm_zz(s, OP_T, OP_N, n, n, n, one, Vt, n, nn, dRL, n, nn, zero, T, n, nn, na));
-      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, T, n, nn, Vt, n, nn, zero, Z, n, nn, na));
-      hipLaunchKernelGGL(h20_combine_kernel, dim3(nblk(tot)), dim3(NT), 0, s, n, na,
-                         c->factor_delta, X, Y, Z, H20);
+      const cplx hfd = cmk(0.5 * c->factor_delta, 0.0), mhfd = cmk(-0.5 * c->factor_delta, 0.0);
+      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, hsp, n, nn, Vt, n, nn, zero, X, n, nn, na));
+      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, hfd, dLR, n, nn, Ut, n, nn, one, X, n, nn, na));
+      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, mhfd, dRL, n, nn, Vt, n, nn, zero, Y, n, nn, na));
+      TG_TRY(gemm_zz(s, OP_T, OP_N, n, n, n, one, Ut, n, nn, X, n, nn, zero, Z, n, nn, na));
+      TG_TRY(gemm_zz(s, OP_T, OP_N, n, n, n, one, Vt, n, nn, Y, n, nn, one, Z, n, nn, na));
+      hipLaunchKernelGGL(skew_part_kernel, dim3((n + 31) / 32, (n + 31) / 32, na), dim3(256), 0, s,
+                         n, Z, H20);
       TG_HIP(hipGetLastError());
     }
   }
