@@ -52,10 +52,15 @@ __host__ __device__ inline cplx cinv(cplx a) {
     return cmk(r / d, -1.0 / d);
   }
 }
-// a += b*c
+// a += b*c  (four FMAs)
 __host__ __device__ inline void cfma(cplx& a, cplx b, cplx c) {
   a.x = fma(b.x, c.x, fma(-b.y, c.y, a.x));
   a.y = fma(b.x, c.y, fma(b.y, c.x, a.y));
+}
+// a -= b*c  (four FMAs)
+__host__ __device__ inline void cfms(cplx& a, cplx b, cplx c) {
+  a.x = fma(-b.x, c.x, fma(b.y, c.y, a.x));
+  a.y = fma(-b.x, c.y, fma(-b.y, c.x, a.y));
 }
 
 // ---------------------------------------------------------------------------

@@ -97,8 +97,7 @@ gj_inverse_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
       } else {
         cplx v = (j == k) ? cmk(0.0, 0.0) : *e;
         const cplx f = colk[i], r = rowk[j];
-        v.x -= f.x * r.x - f.y * r.y;
-        v.y -= f.x * r.y + f.y * r.x;
+        cfms(v, f, r);
         *e = v;
       }
     }
@@ -161,18 +160,19 @@ __device__ inline void gj_argmax(double v, int i, double* wv, int* wi, double& b
   __syncthreads();
 }
 
-template <int GJ_NB>
-__global__ void __launch_bounds__(GJ_THREADS)
+template <int GJ_NB, int RP, int TH>
+__global__ void __launch_bounds__(TH * RP)
 gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* __restrict__ det,
                   int* __restrict__ info, int* __restrict__ perm_all) {
   __shared__ cplx sw[2][2][GJ_NB];
   __shared__ cplx Pw[GJ_NB][GJ_WIN + GJ_U];   // GJ_U columns of slack: no clamping below
   __shared__ int srcs[GJ_NB];
-  __shared__ double wv[2][GJ_THREADS / 64];
-  __shared__ int wi[2][GJ_THREADS / 64];
+  __shared__ double wv[2][TH * RP / 64];
+  __shared__ int wi[2][TH * RP / 64];
   __shared__ int perm[1024];
-  __shared__ cplx spiv;
-  const int t = threadIdx.x;
+  // RP threads per row: thread tt handles row t; the copies (h = 1..RP-1) mirror
+  // the panel reduction and share the column updates
+  const int tt = threadIdx.x, t = tt % TH, h = tt / TH;
   const bool own = t < n;
   cplx* a = Aall + (long long)blockIdx.x * sA;
   cplx d = cmk(1.0, 0.0);
@@ -192,17 +192,17 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
 #pragma unroll
       for (int m = 0; m < GJ_NB; ++m)
         if (m == mm) pv = pr[m];
-      double v = (own && t >= k) ? cabs1(pv) : -1.0;
-      int vi = (own && t >= k) ? t : n;
+      double v = (own && h == 0 && t >= k) ? cabs1(pv) : -1.0;
+      int vi = (own && h == 0 && t >= k) ? t : n;
       wave_argmax(v, vi);
-      if ((t & 63) == 0) {
-        wv[buf][t >> 6] = v;
-        wi[buf][t >> 6] = vi;
+      if ((tt & 63) == 0) {
+        wv[buf][tt >> 6] = v;
+        wi[buf][tt >> 6] = vi;
       }
       __syncthreads();
       double mx = wv[buf][0];
       int p = wi[buf][0];
-      for (int q = 1; q < GJ_THREADS / 64; ++q)
+      for (int q = 1; q < TH * RP / 64; ++q)
         if (wv[buf][q] > mx || (wv[buf][q] == mx && wi[buf][q] < p)) {
           mx = wv[buf][q];
           p = wi[buf][q];
@@ -210,15 +210,15 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
       const bool singular = !(mx > 0.0) || p < k || p >= n;
       if (singular) {
         if (inf == 0) inf = k + 1;
-        if (t == 0) perm[k] = k;
+        if (tt == 0) perm[k] = k;
         d = cmk(0.0, 0.0);
         continue;
       }
-      if (t == 0) perm[k] = p;
-      if (t == p)
+      if (tt == 0) perm[k] = p;
+      if (h == 0 && t == p)
 #pragma unroll
         for (int m = 0; m < GJ_NB; ++m) sw[buf][0][m] = pr[m];
-      if (t == k && p != k)
+      if (h == 0 && t == k && p != k)
 #pragma unroll
         for (int m = 0; m < GJ_NB; ++m) sw[buf][1][m] = pr[m];
       __syncthreads();
@@ -246,8 +246,9 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
           } else {
             const cplx r = sw[buf][0][m];
             const cplx b = pivrow ? cmk(0.0, 0.0) : pr[m];
-            pr[m].x = b.x - (g.x * r.x - g.y * r.y);
-            pr[m].y = b.y - (g.x * r.y + g.y * r.x);
+            cplx v = b;
+            cfms(v, g, r);
+            pr[m] = v;
           }
         }
       }
@@ -255,7 +256,7 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
     __syncthreads();   // perm of the panel visible to every thread
 #pragma unroll
     for (int m = 0; m < GJ_NB; ++m)
-      if (own && m < w) a[t + (long long)(k0 + m) * lda] = pr[m];
+      if (own && h == 0 && m < w) a[t + (long long)(k0 + m) * lda] = pr[m];
     // ---- source row of this thread after the panel's interchanges
     int src = t;
     for (int m = w - 1; m >= 0; --m) {
@@ -265,7 +266,7 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
     }
     const int blk = t - k0;   // position of this row inside the panel, if any
     const bool inblk = own && blk >= 0 && blk < w;
-    if (inblk) srcs[blk] = src;
+    if (inblk && h == 0) srcs[blk] = src;
     __syncthreads();
     // ---- interchanges and rank-w update of the other columns, GJ_WIN at a time.
     // Rows moved out of the panel come from panel positions (each position is
@@ -274,21 +275,21 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
     const int nrest = n - w;
     for (int c0 = 0; c0 < nrest; c0 += GJ_WIN) {
       const int cw = min(GJ_WIN, nrest - c0);
-      for (int e = t; e < w * cw; e += GJ_THREADS) {
+      for (int e = tt; e < w * cw; e += TH * RP) {
         const int m = e % w, c = e / w;
         const int jj = c0 + c;
         Pw[m][c] = a[srcs[m] + (long long)((jj < k0) ? jj : jj + w) * lda];
       }
       __syncthreads();
       if (own && !inblk && src != t)
-        for (int c = 0; c < cw; ++c) {
+        for (int c = h; c < cw; c += RP) {
           const int jj = c0 + c;
           const long long j = (jj < k0) ? jj : jj + w;
           a[t + j * lda] = a[src + j * lda];
         }
       __syncthreads();
       if (own) {
-        for (int c = 0; c < cw; c += GJ_U) {
+        for (int c = h * GJ_U; c < cw; c += GJ_U * RP) {
           cplx acc[GJ_U];
 #pragma unroll
           for (int u = 0; u < GJ_U; ++u) {
@@ -304,8 +305,7 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
 #pragma unroll
               for (int u = 0; u < GJ_U; ++u) {
                 const cplx b = prow[u];
-                acc[u].x += pm.x * b.x - pm.y * b.y;
-                acc[u].y += pm.x * b.y + pm.y * b.x;
+                cfma(acc[u], pm, b);
               }
             }
           }
@@ -320,7 +320,7 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
     }
   }
   // undo the row interchanges on the columns of the inverse (own row only)
-  if (own)
+  if (own && h == 0)
     for (int k = n - 1; k >= 0; --k) {
       const int p = perm[k];
       if (p != k) {
@@ -329,8 +329,8 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
         a[t + (long long)p * lda] = x;
       }
     }
-  for (int k = t; k < n; k += GJ_THREADS) perm_all[(long long)blockIdx.x * n + k] = perm[k];
-  if (t == 0) {
+  for (int k = tt; k < n; k += TH * RP) perm_all[(long long)blockIdx.x * n + k] = perm[k];
+  if (tt == 0) {
     det[blockIdx.x] = d;
     info[blockIdx.x] = inf;
   }
@@ -407,8 +407,8 @@ pfaffian_kernel(int n, const cplx* __restrict__ Aall, int lda, long long sA,
       const cplx xi = xs[i], yj = ys[j], yi = ys[i], xj = xs[j];
       cplx* e = W + i + (long long)j * n;
       cplx v = *e;
-      v.x += (xi.x * yj.x - xi.y * yj.y) - (yi.x * xj.x - yi.y * xj.y);
-      v.y += (xi.x * yj.y + xi.y * yj.x) - (yi.x * xj.y + yi.y * xj.x);
+      cfma(v, xi, yj);
+      cfms(v, yi, xj);
       *e = v;
     }
     __syncthreads();
@@ -483,9 +483,8 @@ pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long
 #pragma unroll
         for (int s2 = 0; s2 < PF_PS; ++s2)
           if (s2 < s) {
-            const cplx a1 = cmul(X[s2], yk[s2]), a2 = cmul(Y[s2], xk[s2]);
-            cik.x += a1.x - a2.x;
-            cik.y += a1.y - a2.y;
+            cfma(cik, X[s2], yk[s2]);
+            cfms(cik, Y[s2], xk[s2]);
           }
       }
       double v = (own && t < k) ? cabs1(cik) : -1.0;
@@ -552,9 +551,8 @@ pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long
 #pragma unroll
         for (int s2 = 0; s2 < PF_PS; ++s2)
           if (s2 < s) {
-            const cplx a1 = cmul(X[s2], yk[s2]), a2 = cmul(Y[s2], xk[s2]);
-            ci.x += a1.x - a2.x;
-            ci.y += a1.y - a2.y;
+            cfma(ci, X[s2], yk[s2]);
+            cfms(ci, Y[s2], xk[s2]);
           }
         xn = cmul(cik, pinv);
         yn = ci;
@@ -581,9 +579,8 @@ pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long
 #pragma unroll
           for (int s2 = 0; s2 < PF_PS; ++s2)
             if (s2 < ns) {
-              const cplx a1 = cmul(X[s2], cy[c][s2]), a2 = cmul(Y[s2], cx[c][s2]);
-              acc.x += a1.x - a2.x;
-              acc.y += a1.y - a2.y;
+              cfma(acc, X[s2], cy[c][s2]);
+              cfms(acc, Y[s2], cx[c][s2]);
             }
           W[t + (long long)j * n] = acc;
         }
@@ -606,16 +603,18 @@ int batched_inverse(hipStream_t s, int n, cplx* A, int lda, long long sA, int ba
   if (n <= 0 || batch <= 0) return TG_OK;
   if (n <= GJ_THREADS) {
     // panel width: 16 or 24 columns (TG_GJ_NB), 16 by default
-    static const int nb = [] {
-      const char* e = std::getenv("TG_GJ_NB");
-      return (e && std::atoi(e) == 24) ? 24 : 16;
+    // TG_GJ=1: one thread per row; default: two threads per row (the second
+    // mirrors the panel reduction and takes half of the column updates)
+    static const int rp = [] {
+      const char* e = std::getenv("TG_GJ");
+      return (e && std::atoi(e) == 1) ? 1 : 2;
     }();
-    if (nb == 24)
-      hipLaunchKernelGGL(gj_blocked_kernel<24>, dim3(batch), dim3(GJ_THREADS), 0, s, n, A, lda,
+    if (rp == 2 && n <= 384)
+      hipLaunchKernelGGL((gj_blocked_kernel<16, 2, 384>), dim3(batch), dim3(768), 0, s, n, A, lda,
                          sA, det, info, perm);
     else
-      hipLaunchKernelGGL(gj_blocked_kernel<16>, dim3(batch), dim3(GJ_THREADS), 0, s, n, A, lda,
-                         sA, det, info, perm);
+      hipLaunchKernelGGL((gj_blocked_kernel<16, 1, GJ_THREADS>), dim3(batch), dim3(GJ_THREADS), 0, s,
+                         n, A, lda, sA, det, info, perm);
     TG_HIP(hipGetLastError());
     return TG_OK;
   }
