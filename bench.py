@@ -210,6 +210,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--conj-pairs", action="store_true",
                     help="compute one gauge angle of each (phi, -phi) pair (exact for real U, V)")
+    ap.add_argument("--pair-steps", type=int, default=5,
+                    help="timed steps of the conjugate-pair variant reported beside value (0: skip)")
     ap.add_argument("--graph", action="store_true",
                     help="replay the projection as a captured hipGraph (no per-kernel timing)")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -269,6 +271,17 @@ def main():
         plain = dict(value=None, E=pr["E"])
     dE_keV = abs(main_run["E"] - plain["E"]) * 1e3
 
+    # the same iteration computing one angle of each (phi, -phi) pair (exact for
+    # real U, V: the other angle's quantities are complex conjugates)
+    pairs = None
+    if not args.conj_pairs and args.pair_steps > 0:
+        cp = run_gpu(args, ham, plain=(args.mode == "plain"), fp32=(args.tbme == "fp32"),
+                     steps=args.pair_steps, warmup=args.warmup, timing=False, conj_pairs=True)
+        pairs = dict(value=cp["steps"] / cp["seconds"], unit="iterations/s",
+                     ms_per_step=1e3 * cp["seconds"] / cp["steps"], angles_per_launch=cp["A"],
+                     dE_vs_all_angles_keV=abs(cp["E"] - main_run["E"]) * 1e3)
+        log("conjugate pairs: %.3f it/s, E = %.9f MeV" % (pairs["value"], cp["E"]))
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -307,6 +320,7 @@ def main():
         "host_boundary": main_run["boundary"],
         "dE_vs_fp64_keV": dE_keV,
         "E_proj_MeV": main_run["E"],
+        "conj_pairs": pairs,
     }
     print(json.dumps(out), flush=True)
 

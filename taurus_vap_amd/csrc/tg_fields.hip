@@ -151,7 +151,11 @@ int contraction_build(tg_ctx* c, int64_t ntb, const int16_t* abcd, const void* H
     const GroupDesc& G = c->ph.groups[g];
     Wph[(size_t)(G.woff + (long long)(pi - G.off) + (long long)(po - G.off) * G.size)] += v;
   };
-  auto add_pp = [&](int x, int y, int u, int w, double v) {   // x>y, u>w
+  auto add_pp = [&](int x, int y, int u, int w, double v) {
+    // stored pairs have x > y and u > w; Delta and kappa are skew, so a pair
+    // in the other order enters with the opposite sign
+    if (x < y) { std::swap(x, y); v = -v; }
+    if (u < w) { std::swap(u, w); v = -v; }
     int po = pos_pp[x + y * n], pi = pos_pp[u + w * n];
     if (po < 0 || pi < 0) { bad = true; return; }
     int g = gpp[po];
@@ -168,6 +172,11 @@ int contraction_build(tg_ctx* c, int64_t ntb, const int16_t* abcd, const void* H
       tg_set_error("tg_init: TBME index out of range");
       return TG_E_ARG;
     }
+    // internal single-particle order
+    ia = c->sp_iperm[ia];
+    ib = c->sp_iperm[ib];
+    ic = c->sp_iperm[ic];
+    id = c->sp_iperm[id];
     double h = (h2_bytes == 4) ? (double)((const float*)H2)[kk] : ((const double*)H2)[kk];
     const int perm = trperm[kk];
     for (int it = 0; it < 2; ++it) {

@@ -177,7 +177,8 @@ struct tg_ctx {
   hipStream_t stream2 = nullptr;                  // overlaps, concurrent with the Thouless chain
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // basis
-  std::vector<int> sp_2mj, sp_2mt, sp_l, sp_tr;   // sp_tr 0-based here
+  std::vector<int> sp_2mj, sp_2mt, sp_l, sp_tr;   // internal order; sp_tr 0-based here
+  std::vector<int> sp_perm, sp_iperm;             // internal -> reference index, and back
   cplx H0;
   DevBuf d_H1;                                    // complex n*n
   // contraction channels

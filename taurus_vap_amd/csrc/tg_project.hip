@@ -993,6 +993,21 @@ int project_device(tg_ctx* c) {
   return TG_OK;
 }
 
+// dst(i, j) = src(perm i, perm j): a single-particle matrix into the internal order
+void permute_in(const tg_ctx* c, const cplx* src, cplx* dst) {
+  const int n = c->n;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i)
+      dst[i + (size_t)j * n] = src[c->sp_perm[i] + (size_t)c->sp_perm[j] * n];
+}
+// dst(perm i, perm j) = src(i, j): back to the reference order
+void permute_out(const tg_ctx* c, const cplx* src, cplx* dst) {
+  const int n = c->n;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i)
+      dst[c->sp_perm[i] + (size_t)c->sp_perm[j] * n] = src[i + (size_t)j * n];
+}
+
 int upload_state(tg_ctx* c, const double* U0, const double* V0, const double* D0, int nocc0,
                  int nemp0, double ovac0, const double* voveru0, int Mphip, int Mphin,
                  double facpi, double valZ, double valN, double factor_delta, int iprint) {
@@ -1007,9 +1022,20 @@ int upload_state(tg_ctx* c, const double* U0, const double* V0, const double* D0
   TG_TRY(c->d_V.ensure(sizeof(double) * nn));
   TG_TRY(c->d_D.ensure(sizeof(double) * nn));
   TG_TRY(c->d_real6.ensure(sizeof(double) * 11 * nn));
-  TG_HIP(hipMemcpyAsync(c->d_U.p, U0, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
-  TG_HIP(hipMemcpyAsync(c->d_V.p, V0, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
-  TG_HIP(hipMemcpyAsync(c->d_D.p, D0, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+  // rows (single-particle index) into the internal order
+  std::vector<double> h((size_t)3 * nn);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      const size_t src = (size_t)c->sp_perm[i] + (size_t)j * n, dst = (size_t)i + (size_t)j * n;
+      h[dst] = U0[src];
+      h[nn + dst] = V0[src];
+      h[2 * nn + dst] = D0[src];
+    }
+  TG_HIP(hipMemcpyAsync(c->d_U.p, h.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+  TG_HIP(hipMemcpyAsync(c->d_V.p, h.data() + nn, sizeof(double) * nn, hipMemcpyHostToDevice,
+                        c->stream));
+  TG_HIP(hipMemcpyAsync(c->d_D.p, h.data() + 2 * nn, sizeof(double) * nn, hipMemcpyHostToDevice,
+                        c->stream));
   TG_HIP(hipStreamSynchronize(c->stream));
   c->nocc = nocc0;
   c->nemp = nemp0;
@@ -1121,17 +1147,44 @@ int tg_init(tg_ctx** out, int ndim, const int32_t* sp_2mj, const int32_t* sp_2mt
   c->w_fp32 = (flags & TG_FLAG_TBME_FP32) ? 1 : 0;
   c->use_graph = (flags & TG_FLAG_GRAPH) ? 1 : 0;
   c->conj_pairs = (flags & TG_FLAG_CONJ_PAIRS) ? 1 : 0;
-  c->sp_2mj.assign(sp_2mj, sp_2mj + ndim);
-  c->sp_2mt.assign(sp_2mt, sp_2mt + ndim);
-  c->sp_l.assign(sp_l, sp_l + ndim);
-  c->sp_tr.resize(ndim);
-  for (int i = 0; i < ndim; ++i) {
-    c->sp_tr[i] = sp_tr[i] - 1;
-    if (c->sp_tr[i] < 0 || c->sp_tr[i] >= ndim) {
+  // Internal single-particle order: by (2mt, 2mj, l mod 2, reference index).
+  // Every class of the contraction blocks then occupies contiguous runs of
+  // rows in each column of the densities and fields, and the protons stay in
+  // the first half.  U, V, D0, H1 and the TBME indices are permuted on the way
+  // in and the single-particle fields on the way out; quasiparticle-basis
+  // quantities (A, H20, gradient) are unaffected.
+  for (int i = 0; i < ndim; ++i)
+    if (sp_tr[i] < 1 || sp_tr[i] > ndim) {
       delete c;
       tg_set_error("tg_init: sp_tr out of range");
       return TG_E_ARG;
     }
+  c->sp_perm.resize(ndim);
+  for (int i = 0; i < ndim; ++i) c->sp_perm[i] = i;
+  std::stable_sort(c->sp_perm.begin(), c->sp_perm.end(), [&](int x, int y) {
+    if (sp_2mt[x] != sp_2mt[y]) return sp_2mt[x] < sp_2mt[y];
+    if (sp_2mj[x] != sp_2mj[y]) return sp_2mj[x] < sp_2mj[y];
+    return (sp_l[x] & 1) < (sp_l[y] & 1);
+  });
+  c->sp_iperm.resize(ndim);
+  for (int i = 0; i < ndim; ++i) c->sp_iperm[c->sp_perm[i]] = i;
+  int nprot = 0;
+  for (int i = 0; i < ndim; ++i) nprot += sp_2mt[i] < 0;
+  if (2 * nprot != ndim) {
+    delete c;
+    tg_set_error("tg_init: the basis must hold ndim/2 proton states");
+    return TG_E_ARG;
+  }
+  c->sp_2mj.resize(ndim);
+  c->sp_2mt.resize(ndim);
+  c->sp_l.resize(ndim);
+  c->sp_tr.resize(ndim);
+  for (int i = 0; i < ndim; ++i) {
+    const int o = c->sp_perm[i];
+    c->sp_2mj[i] = sp_2mj[o];
+    c->sp_2mt[i] = sp_2mt[o];
+    c->sp_l[i] = sp_l[o];
+    c->sp_tr[i] = c->sp_iperm[sp_tr[o] - 1];
   }
   c->H0 = cmk(H0_re, H0_im);
   int r = TG_OK;
@@ -1144,11 +1197,15 @@ int tg_init(tg_ctx** out, int ndim, const int32_t* sp_2mj, const int32_t* sp_2mt
     return TG_E_HIP;
   }
   r = c->d_H1.ensure(sizeof(cplx) * (size_t)ndim * ndim);
-  if (r == TG_OK && hipMemcpy(c->d_H1.p, H1, sizeof(cplx) * (size_t)ndim * ndim,
-                              hipMemcpyHostToDevice) != hipSuccess) r = TG_E_HIP;
+  if (r == TG_OK) {
+    std::vector<cplx> h1((size_t)ndim * ndim);
+    permute_in(c, reinterpret_cast<const cplx*>(H1), h1.data());
+    if (hipMemcpy(c->d_H1.p, h1.data(), sizeof(cplx) * h1.size(), hipMemcpyHostToDevice) !=
+        hipSuccess) r = TG_E_HIP;
+  }
   if (r == TG_OK) {
     std::vector<double> ps(ndim);
-    for (int i = 0; i < ndim; ++i) ps[i] = (sp_l[i] % 2 == 0) ? 1.0 : -1.0;
+    for (int i = 0; i < ndim; ++i) ps[i] = (c->sp_l[i] % 2 == 0) ? 1.0 : -1.0;
     r = c->d_psign.ensure(sizeof(double) * ndim);
     if (r == TG_OK && hipMemcpy(c->d_psign.p, ps.data(), sizeof(double) * ndim,
                                 hipMemcpyHostToDevice) != hipSuccess) r = TG_E_HIP;
@@ -1200,18 +1257,21 @@ int tg_calculate_fields(tg_ctx* c, const double* rhoLR, const double* kappaLR,
   const size_t nn = (size_t)n * n;
   TG_TRY(c->d_work.ensure(sizeof(cplx) * 8 * nn));
   cplx* w = c->d_work.as<cplx>();
-  TG_HIP(hipMemcpyAsync(w, rhoLR, sizeof(cplx) * nn, hipMemcpyHostToDevice, c->stream));
-  TG_HIP(hipMemcpyAsync(w + nn, kappaLR, sizeof(cplx) * nn, hipMemcpyHostToDevice, c->stream));
-  TG_HIP(hipMemcpyAsync(w + 2 * nn, kappaRL, sizeof(cplx) * nn, hipMemcpyHostToDevice, c->stream));
+  std::vector<cplx> h(4 * nn);
+  permute_in(c, reinterpret_cast<const cplx*>(rhoLR), h.data());
+  permute_in(c, reinterpret_cast<const cplx*>(kappaLR), h.data() + nn);
+  permute_in(c, reinterpret_cast<const cplx*>(kappaRL), h.data() + 2 * nn);
+  TG_HIP(hipMemcpyAsync(w, h.data(), sizeof(cplx) * 3 * nn, hipMemcpyHostToDevice, c->stream));
   TG_TRY(contraction_run(c, 1, w, w + nn, w + 2 * nn, w + 3 * nn, w + 5 * nn, w + 6 * nn, 0));
   hipLaunchKernelGGL(add_h1_kernel, dim3(nblk(nn)), dim3(NT), 0, c->stream, (long long)nn, 1,
                      c->d_H1.as<cplx>(), w + 3 * nn, w + 4 * nn);
   TG_HIP(hipGetLastError());
   TG_HIP(hipStreamSynchronize(c->stream));
-  TG_HIP(hipMemcpy(gammaLR, w + 3 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
-  TG_HIP(hipMemcpy(hspLR, w + 4 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
-  TG_HIP(hipMemcpy(deltaLR, w + 5 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
-  TG_HIP(hipMemcpy(deltaRL, w + 6 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(h.data(), w + 3 * nn, sizeof(cplx) * 4 * nn, hipMemcpyDeviceToHost));
+  permute_out(c, h.data(), reinterpret_cast<cplx*>(gammaLR));
+  permute_out(c, h.data() + nn, reinterpret_cast<cplx*>(hspLR));
+  permute_out(c, h.data() + 2 * nn, reinterpret_cast<cplx*>(deltaLR));
+  permute_out(c, h.data() + 3 * nn, reinterpret_cast<cplx*>(deltaRL));
   return TG_OK;
 }
 
@@ -1225,17 +1285,20 @@ int tg_calculate_fields_diag(tg_ctx* c, const double* rhoLR, const double* kappa
   const size_t nn = (size_t)n * n;
   TG_TRY(c->d_work.ensure(sizeof(cplx) * 8 * nn));
   cplx* w = c->d_work.as<cplx>();
-  TG_HIP(hipMemcpyAsync(w, rhoLR, sizeof(cplx) * nn, hipMemcpyHostToDevice, c->stream));
-  TG_HIP(hipMemcpyAsync(w + nn, kappaLR, sizeof(cplx) * nn, hipMemcpyHostToDevice, c->stream));
+  std::vector<cplx> h(4 * nn);
+  permute_in(c, reinterpret_cast<const cplx*>(rhoLR), h.data());
+  permute_in(c, reinterpret_cast<const cplx*>(kappaLR), h.data() + nn);
+  TG_HIP(hipMemcpyAsync(w, h.data(), sizeof(cplx) * 2 * nn, hipMemcpyHostToDevice, c->stream));
   TG_TRY(contraction_run(c, 1, w, w + nn, nullptr, w + 3 * nn, w + 5 * nn, w + 6 * nn, 1));
   hipLaunchKernelGGL(add_h1_kernel, dim3(nblk(nn)), dim3(NT), 0, c->stream, (long long)nn, 1,
                      c->d_H1.as<cplx>(), w + 3 * nn, w + 4 * nn);
   TG_HIP(hipGetLastError());
   TG_HIP(hipStreamSynchronize(c->stream));
-  TG_HIP(hipMemcpy(gammaLR, w + 3 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
-  TG_HIP(hipMemcpy(hspLR, w + 4 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
-  TG_HIP(hipMemcpy(deltaLR, w + 5 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
-  if (deltaRL) TG_HIP(hipMemcpy(deltaRL, w + 6 * nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(h.data(), w + 3 * nn, sizeof(cplx) * 4 * nn, hipMemcpyDeviceToHost));
+  permute_out(c, h.data(), reinterpret_cast<cplx*>(gammaLR));
+  permute_out(c, h.data() + nn, reinterpret_cast<cplx*>(hspLR));
+  permute_out(c, h.data() + 2 * nn, reinterpret_cast<cplx*>(deltaLR));
+  if (deltaRL) permute_out(c, h.data() + 3 * nn, reinterpret_cast<cplx*>(deltaRL));
   return TG_OK;
 }
 
