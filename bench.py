@@ -58,6 +58,21 @@ def algorithmic_flops(ctx_info, n, A, nb):
     }
 
 
+def measured_traffic(workload, angles_per_launch, kernel):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    runs recorded under profiles/ (scaled by the copy-kernel calibration stated
+    there), when they were taken on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel or int(t.get("angles_per_launch", -1)) != angles_per_launch:
+        return None
+    return float(t["hbm_bytes_per_launch"])
+
+
 def contraction_sizes(h):
     """sum of s_g^2 of the ph and pp block channels (as built by tg_init)."""
     from collections import Counter
@@ -258,7 +273,8 @@ def main():
         achieved = fl[dom] / avg_s / 1e12
         roofline = {"bound": "mfma", "kernel": dom.split(":", 1)[1], "achieved": achieved,
                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                    "traffic": None, "flop_per_launch": fl[dom], "avg_launch_ms": avg_s * 1e3}
+                    "traffic": measured_traffic(args.workload, A, dom.split(":", 1)[1]),
+                    "flop_per_launch": fl[dom], "avg_launch_ms": avg_s * 1e3}
 
     # plain FP64 per-angle drop-in (denominator of the metric) and |dE| vs FP64
     plain = None
