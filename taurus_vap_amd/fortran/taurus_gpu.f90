@@ -19,7 +19,8 @@ implicit none
 public
 
 integer(c_int), parameter :: TG_OK = 0, TG_E_IMAG_GRADIENT = -5, &
-                             TG_FLAG_PLAIN = 1, TG_FLAG_TBME_FP32 = 2
+                             TG_FLAG_PLAIN = 1, TG_FLAG_TBME_FP32 = 2, &
+                             TG_FLAG_GRAPH = 4, TG_FLAG_CONJ_PAIRS = 8
 type(c_ptr) :: taurus_gpu_ctx = c_null_ptr
 
 interface
@@ -152,6 +153,14 @@ subroutine taurus_gpu_setup(ndim)
   call get_environment_variable('TAURUS_GPU_PLAIN', env, lenv, st)
   if ( st == 0 .and. lenv > 0 ) then
     if ( env(1:1) == '1' ) flags = ior(flags, TG_FLAG_PLAIN)
+  endif
+  call get_environment_variable('TAURUS_GPU_GRAPH', env, lenv, st)
+  if ( st == 0 .and. lenv > 0 ) then
+    if ( env(1:1) == '1' ) flags = ior(flags, TG_FLAG_GRAPH)
+  endif
+  call get_environment_variable('TAURUS_GPU_CONJ_PAIRS', env, lenv, st)
+  if ( st == 0 .and. lenv > 0 ) then
+    if ( env(1:1) == '1' ) flags = ior(flags, TG_FLAG_CONJ_PAIRS)
   endif
   h2b = int(storage_size(hamil_H2) / 8, c_int)
   if ( h2b == 4 ) flags = ior(flags, TG_FLAG_TBME_FP32)

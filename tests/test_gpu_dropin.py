@@ -24,7 +24,7 @@ EXE = os.path.join(ROOT, "taurus_vap_amd", "fortran", "_build", "taurus_vap_gpu3
 MG24 = os.path.join(GOLDEN, "mg24")
 
 
-def _run(M: int, xo: int = 1, iters: int = 3000, plain: bool = False):
+def _run(M: int, xo: int = 1, iters: int = 3000, plain: bool = False, extra_env=None):
     if not os.path.exists(EXE):
         pytest.skip("integration build not present (taurus_vap_amd/fortran/build_dropin.sh)")
     with tempfile.TemporaryDirectory() as td:
@@ -41,6 +41,7 @@ def _run(M: int, xo: int = 1, iters: int = 3000, plain: bool = False):
         env = dict(os.environ, OMP_NUM_THREADS="4")
         if plain:
             env["TAURUS_GPU_PLAIN"] = "1"
+        env.update(extra_env or {})
         with open(os.path.join(td, "input.txt")) as fin:
             r = subprocess.run([EXE], stdin=fin, capture_output=True, text=True, cwd=td, env=env,
                                timeout=600)
@@ -103,6 +104,19 @@ def test_dropin_mg24_vap_published(plain):
     e = _final_energy(out)
     rows = _iterations(out)
     print("drop-in VAP: %d iterations, E = %.6f (published %d, %.6f)" % (rows[-1][0], e, it_pub, e_pub))
+    assert abs(e - e_pub) <= 1.5e-6
+    assert abs(rows[-1][0] - it_pub) <= 10
+
+
+def test_dropin_mg24_vap_graph_conj_pairs():
+    """The same VAP run with the projection replayed as a captured hipGraph
+    (every iteration re-uploads its state) and one angle of each conjugate
+    pair computed: published 276 iterations, E = -82.773573 MeV."""
+    it_pub, e_pub = _published("results_vap_benchmark")
+    out = _run(5, extra_env={"TAURUS_GPU_GRAPH": "1", "TAURUS_GPU_CONJ_PAIRS": "1"})
+    e = _final_energy(out)
+    rows = _iterations(out)
+    print("drop-in VAP graph+pairs: %d iterations, E = %.6f" % (rows[-1][0], e))
     assert abs(e - e_pub) <= 1.5e-6
     assert abs(rows[-1][0] - it_pub) <= 10
 
