@@ -433,7 +433,7 @@ pfaffian_kernel(int n, const cplx* __restrict__ Aall, int lda, long long sA,
 // ---------------------------------------------------------------------------
 constexpr int PF_THREADS = 512;
 constexpr int PF_PS = 8;
-constexpr int PF_CH = 16;
+constexpr int PF_WIN = 256;   // columns whose pending vectors are staged in LDS at once
 
 __global__ void __launch_bounds__(PF_THREADS)
 pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long sA,
@@ -442,7 +442,7 @@ pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long
   __shared__ cplx xk[PF_PS], yk[PF_PS];
   __shared__ cplx swx[2][PF_PS], swy[2][PF_PS];
   __shared__ cplx swc[2];
-  __shared__ cplx cx[PF_CH][PF_PS], cy[PF_CH][PF_PS];
+  __shared__ cplx cx[PF_WIN][PF_PS], cy[PF_WIN][PF_PS];
   __shared__ double wv[PF_THREADS / 64];
   __shared__ int wi[PF_THREADS / 64];
   __shared__ cplx spiv;
@@ -565,24 +565,25 @@ pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long
       __syncthreads();
     }
     if (z) break;
-    // rank-2*ns update of the leading block [0, m)
-    for (int j0 = 0; j0 < m; j0 += PF_CH) {
-      if (t >= j0 && t < j0 + PF_CH && t < m)
+    // rank-2*ns update of the leading block [0, m): the pending vectors of
+    // PF_WIN columns are staged in LDS, then every thread streams its own row
+    for (int j0 = 0; j0 < m; j0 += PF_WIN) {
+      const int cw = min(PF_WIN, m - j0);
+      if (t >= j0 && t < j0 + cw)
 #pragma unroll
         for (int s2 = 0; s2 < PF_PS; ++s2) { cx[t - j0][s2] = X[s2]; cy[t - j0][s2] = Y[s2]; }
       __syncthreads();
       if (own && t < m) {
-        for (int c = 0; c < PF_CH; ++c) {
-          const int j = j0 + c;
-          if (j >= m) break;
-          cplx acc = W[t + (long long)j * n];
+        for (int c = 0; c < cw; ++c) {
+          const long long j = j0 + c;
+          cplx acc = W[t + j * n];
 #pragma unroll
           for (int s2 = 0; s2 < PF_PS; ++s2)
             if (s2 < ns) {
               cfma(acc, X[s2], cy[c][s2]);
               cfms(acc, Y[s2], cx[c][s2]);
             }
-          W[t + (long long)j * n] = acc;
+          W[t + j * n] = acc;
         }
       }
       __syncthreads();
