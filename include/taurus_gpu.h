@@ -174,6 +174,49 @@ int tg_set_gradient(tg_ctx* ctx, double gradient_eta, double gradient_mu, int co
 int tg_get_gradient(tg_ctx* ctx, double* gradient_Zi, double* gradient_norm);
 
 /*
+ * tg_set_constraints -- the constraint operators of set_constraints
+ * (src/module_constraints.f90:77-258) for the device iteration below:
+ * constraint_HO real[constraint_dim*ndim*ndim] (column-major HO matrices in the
+ * order of the sp arrays given to tg_init, one-body operators first and the
+ * constraint_pair pair operators last, as src/module_constraints.f90:104-250
+ * stores them), constraint_val real[constraint_dim], constraint_eps the
+ * tolerance of adjust_constraints.  constraint_dim <= 64.
+ */
+int tg_set_constraints(tg_ctx* ctx, int constraint_dim, int constraint_pair,
+                       const double* constraint_HO, const double* constraint_val,
+                       double constraint_eps);
+
+/*
+ * tg_iterate -- one pass of the gradient loop of src/taurus_vap.f90:77-113 for
+ * gradient_type 0 with the Bogoliubov matrices kept in HBM:
+ *   update_constraints_qpbasis      src/module_constraints.f90:381-397
+ *   project_wavefunction(0, ...)    src/module_projection.f90 (as tg_project_resident)
+ *   calculate_lagrange_multipliers  src/module_constraints.f90:413-484
+ *   calculate_gradient              src/module_gradient.f90:79-143
+ *   adjust_constraints              src/module_constraints.f90:270-373
+ *   gradient_Zim1 = gradient_Zi     src/taurus_vap.f90:113
+ * Needs tg_set_state (state and canonical basis), tg_set_gradient (eta, mu;
+ * its constraint arguments are replaced by the device's constraint_20) and
+ * tg_set_constraints.  Outputs (each may be NULL): pnp_scalars complex[7] of
+ * the projected state (as tg_get_results), gradient_norm, lagrange_lambda1
+ * real[constraint_dim], adjust_iterations, converged (gradient_norm <=
+ * gradient_eps: the state is then left unchanged).  Otherwise the device state
+ * becomes U1, V1 of adjust_constraints and a new tg_set_state (with the
+ * canonical basis of that state, construct_canonical_basis,
+ * src/module_wavefunctions.f90) must precede the next tg_iterate.  A failed
+ * Cholesky factorisation returns TG_E_SINGULAR where the reference stops; a
+ * singular multiplier system keeps the previous multipliers (:471-477).
+ */
+int tg_iterate(tg_ctx* ctx, double gradient_eps, double* pnp_scalars, double* gradient_norm,
+               double* lagrange_lambda1, int* adjust_iterations, int* converged);
+
+/*
+ * tg_get_state -- bogo_U0, bogo_V0 of the device state (column-major
+ * real[ndim*ndim] each, rows in the sp order given to tg_init).
+ */
+int tg_get_state(tg_ctx* ctx, double* U0, double* V0);
+
+/*
  * Device timing: with timing enabled, every stage and the dominant kernels of
  * tg_project_resident record HIP event pairs on the context stream (no
  * synchronisation is added); tg_get_timing returns the summed milliseconds and

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtaurus_gpu.so")
-SOURCES = ["tg_gemm.hip", "tg_mfma.hip", "tg_linalg.hip", "tg_fields.hip", "tg_project.hip"]
+SOURCES = ["tg_gemm.hip", "tg_mfma.hip", "tg_linalg.hip", "tg_fields.hip", "tg_project.hip", "tg_iterate.hip"]
 ARCH = os.environ.get("TAURUS_GPU_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -69,6 +69,8 @@ __host__ __device__ inline void cfms(cplx& a, cplx b, cplx c) {
 // Count of device (re)allocations made through DevBuf::ensure.
 unsigned long long devbuf_epoch();
 
+constexpr int TG_MAX_CONS = 64;   // Lagrange multiplier slots of the gradient buffer
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -220,7 +222,14 @@ struct tg_ctx {
   double grad_eta = 0, grad_mu = 0;
   int grad_ncons = 0;
   std::vector<double> grad_lambda;
-  DevBuf d_c20, d_Z;                              // constraint_20; Zi, Zim1, norm
+  DevBuf d_c20, d_Z;                              // constraint_20; Zi, Zim1, lambda, norm
+  // constraints of the device iteration (tg_set_constraints / tg_iterate)
+  int cons_dim = 0, cons_pair = 0;
+  double cons_eps = 0;
+  std::vector<double> cons_val;
+  DevBuf d_cHO;                                   // constraint operators, internal order
+  DevBuf d_it;                                    // iteration work (n*n real matrices)
+  std::vector<double> lambda_prev;                // multipliers of the previous iteration
   // timing: event pairs recorded without synchronisation, summed at collection
   struct Pending { std::string key; hipEvent_t e0, e1; };
   int timing = 0;
@@ -240,6 +249,14 @@ struct ScopedTiming {
   ~ScopedTiming();
 };
 int timing_collect(tg_ctx* c);
+
+// Projection of the resident state (tg_project.hip) and download of its sums.
+int tg_project_device(tg_ctx* c);
+// gradient_kernel on the given buffers (calculate_gradient, gradient_type 0)
+int tg_gradient_step(tg_ctx* c, const double* H20v, double* Z, const double* Zm1,
+                     const double* lam, const double* c20, double* part);
+int tg_fetch_results(tg_ctx* c, double* scal, double* pA, double* pH, double* pAH, double* H20v,
+                     double* imag);
 
 int contraction_build(tg_ctx* c, int64_t ntb, const int16_t* abcd, const void* H2,
                       int h2_bytes, const int8_t* trperm);

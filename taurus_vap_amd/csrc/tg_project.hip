@@ -913,7 +913,7 @@ int project_run(tg_ctx* c) {
     double* Zcur = Zb;
     double* Zprev = Zb + nn;
     double* lam = Zb + 2 * nn;
-    double* part = lam + std::max(c->grad_ncons, 1);
+    double* part = lam + TG_MAX_CONS;
     hipLaunchKernelGGL(gradient_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, c->grad_eta, c->grad_mu,
                        c->grad_ncons, lam, c->d_c20.as<double>(), H20v, Zprev, Zcur, part);
     TG_HIP(hipGetLastError());
@@ -1125,6 +1125,21 @@ int get_device_ready() {
 
 }  // namespace
 
+// entry points for the other translation units of the library
+int tg_project_device(tg_ctx* c) { return project_device(c); }
+int tg_gradient_step(tg_ctx* c, const double* H20v, double* Z, const double* Zm1,
+                     const double* lam, const double* c20, double* part) {
+  const long long nn = (long long)c->n * c->n;
+  hipLaunchKernelGGL(gradient_kernel, dim3(nblk(nn)), dim3(NT), 0, c->stream, nn, c->grad_eta,
+                     c->grad_mu, c->grad_ncons, lam, c20, H20v, Zm1, Z, part);
+  TG_HIP(hipGetLastError());
+  return TG_OK;
+}
+int tg_fetch_results(tg_ctx* c, double* scal, double* pA, double* pH, double* pAH, double* H20v,
+                     double* imag) {
+  return fetch_results(c, scal, pA, pH, pAH, H20v, imag);
+}
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -1226,7 +1241,7 @@ int tg_finalize(tg_ctx* c) {
   graph_release(c);
   for (DevBuf* b : {&c->d_H1, &c->d_U, &c->d_V, &c->d_D, &c->d_ang, &c->d_real6, &c->d_work,
                     &c->d_blk, &c->d_scal, &c->d_out, &c->d_ovl, &c->d_int, &c->d_pfw, &c->d_c20,
-                    &c->d_Z, &c->d_psign})
+                    &c->d_Z, &c->d_psign, &c->d_cHO, &c->d_it})
     b->release();
   for (Channel* ch : {&c->ph, &c->pp})
     for (DevBuf* b : {&ch->d_perm, &ch->d_groups, &ch->d_W, &ch->d_tile_group, &ch->d_tile_n0})
@@ -1596,7 +1611,8 @@ int tg_get_timing(tg_ctx* c, const char* key, double* ms_sum, int* count) {
 
 int tg_set_gradient(tg_ctx* c, double gradient_eta, double gradient_mu, int constraint_dim,
                     const double* constraint_20, const double* lagrange_lambda1) {
-  if (!c || constraint_dim < 0 || (constraint_dim > 0 && (!constraint_20 || !lagrange_lambda1))) {
+  if (!c || constraint_dim < 0 || constraint_dim > TG_MAX_CONS ||
+      (constraint_dim > 0 && (!constraint_20 || !lagrange_lambda1))) {
     tg_set_error("tg_set_gradient: invalid argument");
     return TG_E_ARG;
   }
@@ -1605,8 +1621,7 @@ int tg_set_gradient(tg_ctx* c, double gradient_eta, double gradient_mu, int cons
   c->grad_eta = gradient_eta;
   c->grad_mu = gradient_mu;
   c->grad_ncons = constraint_dim;
-  const size_t nl = std::max(constraint_dim, 1);
-  TG_TRY(c->d_Z.ensure(sizeof(double) * (2 * nn + nl + nn / NT + 2)));
+  TG_TRY(c->d_Z.ensure(sizeof(double) * (2 * nn + TG_MAX_CONS + nn / NT + 2)));
   TG_TRY(c->d_c20.ensure(sizeof(double) * std::max<size_t>(1, (size_t)constraint_dim * nn)));
   double* Zb = c->d_Z.as<double>();
   TG_HIP(hipMemset(Zb, 0, sizeof(double) * 2 * nn));
@@ -1630,7 +1645,7 @@ int tg_get_gradient(tg_ctx* c, double* gradient_Zi, double* gradient_norm) {
   if (gradient_Zi) TG_HIP(hipMemcpy(gradient_Zi, Zb, sizeof(double) * nn, hipMemcpyDeviceToHost));
   const size_t nb = (nn + NT - 1) / NT;
   std::vector<double> part(nb);
-  TG_HIP(hipMemcpy(part.data(), Zb + 2 * nn + std::max(c->grad_ncons, 1), sizeof(double) * nb,
+  TG_HIP(hipMemcpy(part.data(), Zb + 2 * nn + TG_MAX_CONS, sizeof(double) * nb,
                    hipMemcpyDeviceToHost));
   double sum = 0;
   for (double v : part) sum += v;

@@ -35,7 +35,8 @@ EXPORTS = [
     "tg_calculate_fields_diag", "tg_zskpfa", "tg_calculate_overlap",
     "tg_project_wavefunction", "tg_set_state", "tg_project_resident", "tg_sync",
     "tg_get_results", "tg_calculate_gradient", "tg_set_gradient", "tg_get_gradient",
-    "tg_enable_timing", "tg_reset_timing", "tg_get_timing",
+    "tg_enable_timing", "tg_reset_timing", "tg_get_timing", "tg_set_constraints",
+    "tg_iterate", "tg_get_state",
 ]
 
 _lib = None
@@ -75,6 +76,9 @@ def load_library():
     lib.tg_enable_timing.argtypes = [P, I]
     lib.tg_reset_timing.argtypes = [P]
     lib.tg_get_timing.argtypes = [P, ctypes.c_char_p, P, P]
+    lib.tg_set_constraints.argtypes = [P, I, I, P, P, D]
+    lib.tg_iterate.argtypes = [P, D, P, P, P, P, P]
+    lib.tg_get_state.argtypes = [P, P, P]
     _lib = lib
     return lib
 
@@ -221,6 +225,47 @@ class Context:
         norm = np.zeros(1)
         _check(load_library().tg_get_gradient(self._h, _p(Zi), _p(norm)), "tg_get_gradient")
         return Zi, float(norm[0])
+
+    # device-resident gradient iteration (gradient_type 0)
+    def set_constraints(self, constraint_HO, constraint_val, constraint_pair=0,
+                        constraint_eps=1e-6):
+        val = np.ascontiguousarray(np.asarray(constraint_val, dtype=np.float64).reshape(-1))
+        dim = val.size
+        HO = (np.ascontiguousarray(constraint_HO, dtype=np.float64).reshape(-1) if dim
+              else np.zeros(1))
+        if dim and HO.size != dim * self.n * self.n:
+            raise ValueError("constraint_HO must hold constraint_dim * ndim**2 values")
+        self._cons_keep = (HO, val)
+        _check(load_library().tg_set_constraints(self._h, dim, int(constraint_pair), _p(HO),
+                                                 _p(val if dim else np.zeros(1)),
+                                                 float(constraint_eps)), "tg_set_constraints")
+        self.constraint_dim = dim
+
+    def iterate(self, gradient_eps=0.0):
+        """One iteration of the gradient loop on the device (projection of the
+        current state, Lagrange multipliers, gradient, constraint adjustment).
+        Returns the projected scalars of the current state, the gradient norm,
+        the multipliers and the number of adjustment steps; the state on the
+        device becomes the evolved one (read it with get_state)."""
+        dim = max(1, getattr(self, "constraint_dim", 0))
+        scal = np.zeros(7, np.complex128)
+        gn = np.zeros(1)
+        lam = np.zeros(dim)
+        nadj = ctypes.c_int()
+        conv = ctypes.c_int()
+        _check(load_library().tg_iterate(self._h, float(gradient_eps), _p(scal), _p(gn), _p(lam),
+                                         ctypes.byref(nadj), ctypes.byref(conv)), "tg_iterate")
+        return dict(over=scal[0], ener=scal[1], prot=scal[2], neut=scal[3], prot2=scal[4],
+                    neut2=scal[5], pari=scal[6], gradient_norm=float(gn[0]),
+                    lagrange_lambda1=lam[:getattr(self, "constraint_dim", 0)].copy(),
+                    adjust_iterations=nadj.value, converged=bool(conv.value))
+
+    def get_state(self):
+        n = self.n
+        U = np.zeros((n, n), order="F")
+        V = np.zeros((n, n), order="F")
+        _check(load_library().tg_get_state(self._h, _p(U), _p(V)), "tg_get_state")
+        return U, V
 
     def enable_timing(self, on=True):
         _check(load_library().tg_enable_timing(self._h, 1 if on else 0), "tg_enable_timing")
