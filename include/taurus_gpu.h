@@ -45,6 +45,7 @@ extern "C" {
                                     complex conjugates) and add it as 2 Re(.) */
 
 typedef struct tg_ctx tg_ctx;
+typedef struct tg_group tg_group;
 
 /* Message of the last failing call on this thread. */
 const char* tg_last_error(void);
@@ -149,6 +150,26 @@ int tg_get_results(tg_ctx* ctx, double* pnp_scalars, double* pnp_A, double* pnp_
                    double* pnp_AH20, double* field_H20v, double* imag_grad);
 
 /*
+ * tg_set_states -- nstates independent states of this basis and Hamiltonian,
+ * projected together (every kernel of the angle loop batches the nstates x A
+ * items; the gauge sums, H20 and the gradient are per state).  The reference
+ * runs one state per process; this is the many-solves case of the same
+ * project_wavefunction(0, ...) (src/module_projection.f90:201-373).  State k:
+ * U0 + k*ndim*ndim, V0 + ..., D0 + ..., nocc0[k], nemp0[k], ovac0[k], and
+ * voveru0 + k*ndim holding its ndim - nocc0[k] values.  The angle grid,
+ * valence numbers and factor_delta are common.  tg_set_state is nstates = 1.
+ * Not with TG_FLAG_PLAIN.
+ * tg_get_results_state / tg_get_gradient_state -- the results of state k.
+ */
+int tg_set_states(tg_ctx* ctx, int nstates, const double* U0, const double* V0,
+                  const double* D0, const int* nocc0, const int* nemp0, const double* ovac0,
+                  const double* voveru0, int Mphip, int Mphin, double facpi, double valence_Z,
+                  double valence_N, double factor_delta, int iprint);
+int tg_get_results_state(tg_ctx* ctx, int state, double* pnp_scalars, double* pnp_A,
+                         double* pnp_H20, double* pnp_AH20, double* field_H20v,
+                         double* imag_grad);
+
+/*
  * tg_calculate_gradient -- calculate_gradient(ndim), src/module_gradient.f90:79-143.
  * opt_grad selection: gradient_type with the info_H11/info_hsp fall-back (:86-91);
  * eigen_H11/eigen_hsp real[ndim] (used by gradient_type 1/2, may be NULL for 0);
@@ -172,6 +193,7 @@ int tg_calculate_gradient(int ndim, int gradient_type, double gradient_eta, doub
 int tg_set_gradient(tg_ctx* ctx, double gradient_eta, double gradient_mu, int constraint_dim,
                     const double* constraint_20, const double* lagrange_lambda1);
 int tg_get_gradient(tg_ctx* ctx, double* gradient_Zi, double* gradient_norm);
+int tg_get_gradient_state(tg_ctx* ctx, int state, double* gradient_Zi, double* gradient_norm);
 
 /*
  * tg_set_constraints -- the constraint operators of set_constraints
@@ -179,11 +201,13 @@ int tg_get_gradient(tg_ctx* ctx, double* gradient_Zi, double* gradient_norm);
  * constraint_HO real[constraint_dim*ndim*ndim] (column-major HO matrices in the
  * order of the sp arrays given to tg_init, one-body operators first and the
  * constraint_pair pair operators last, as src/module_constraints.f90:104-250
- * stores them), constraint_val real[constraint_dim], constraint_eps the
- * tolerance of adjust_constraints.  constraint_dim <= 64.
+ * stores them), constraint_val real[nsets*constraint_dim] (constraint_val of
+ * :77-258; nsets = 1: the same values for every state, or one set per state of
+ * tg_set_states), constraint_eps the tolerance of adjust_constraints.
+ * constraint_dim <= 64.
  */
 int tg_set_constraints(tg_ctx* ctx, int constraint_dim, int constraint_pair,
-                       const double* constraint_HO, const double* constraint_val,
+                       const double* constraint_HO, int nsets, const double* constraint_val,
                        double constraint_eps);
 
 /*
@@ -195,26 +219,49 @@ int tg_set_constraints(tg_ctx* ctx, int constraint_dim, int constraint_pair,
  *   calculate_gradient              src/module_gradient.f90:79-143
  *   adjust_constraints              src/module_constraints.f90:270-373
  *   gradient_Zim1 = gradient_Zi     src/taurus_vap.f90:113
- * Needs tg_set_state (state and canonical basis), tg_set_gradient (eta, mu;
- * its constraint arguments are replaced by the device's constraint_20) and
- * tg_set_constraints.  Outputs (each may be NULL): pnp_scalars complex[7] of
- * the projected state (as tg_get_results), gradient_norm, lagrange_lambda1
- * real[constraint_dim], adjust_iterations, converged (gradient_norm <=
- * gradient_eps: the state is then left unchanged).  Otherwise the device state
- * becomes U1, V1 of adjust_constraints and a new tg_set_state (with the
- * canonical basis of that state, construct_canonical_basis,
- * src/module_wavefunctions.f90) must precede the next tg_iterate.  A failed
- * Cholesky factorisation returns TG_E_SINGULAR where the reference stops; a
- * singular multiplier system keeps the previous multipliers (:471-477).
+ * Every resident state takes the step at once (the adjustment loops run in
+ * lockstep; a state whose constraints are met adds nothing more to its Z).
+ * Needs tg_set_state(s) (states and canonical bases), tg_set_gradient (eta,
+ * mu; its constraint arguments are replaced by the device's constraint_20)
+ * and tg_set_constraints.  Outputs per state (each pointer may be NULL):
+ * pnp_scalars complex[7*nstates] of the projected states (as tg_get_results),
+ * gradient_norm[nstates], lagrange_lambda1 real[nstates*constraint_dim],
+ * adjust_iterations[nstates], converged[nstates] (gradient_norm <=
+ * gradient_eps: that state is left unchanged).  The other states become U1, V1
+ * of adjust_constraints, and a new tg_set_state(s) (with the canonical bases,
+ * construct_canonical_basis, src/module_wavefunctions.f90) must precede the
+ * next tg_iterate.  A failed Cholesky factorisation returns TG_E_SINGULAR where
+ * the reference stops; a singular multiplier system keeps the previous
+ * multipliers (:471-477).
  */
 int tg_iterate(tg_ctx* ctx, double gradient_eps, double* pnp_scalars, double* gradient_norm,
                double* lagrange_lambda1, int* adjust_iterations, int* converged);
 
 /*
- * tg_get_state -- bogo_U0, bogo_V0 of the device state (column-major
- * real[ndim*ndim] each, rows in the sp order given to tg_init).
+ * tg_get_state -- bogo_U0, bogo_V0 of the device states (column-major
+ * real[ndim*ndim] each, state k at + k*ndim*ndim, rows in the sp order given to
+ * tg_init).
  */
 int tg_get_state(tg_ctx* ctx, double* U0, double* V0);
+
+/*
+ * Groups: the projections of several contexts (each with its own basis size,
+ * Hamiltonian and states) issued together as one hipGraph launch per call
+ * (use_graph = 1: captured the second time a configuration is seen, replayed
+ * afterwards; 0: the same launches, eagerly, each context on its stream).
+ * tg_group_project is tg_project_resident for every context; tg_group_iterate
+ * is tg_iterate for every context around one group projection, with the
+ * outputs of context `index` read by tg_group_iterate_results (arrays as in
+ * tg_iterate).  The contexts stay owned by the caller and must outlive the
+ * group; a context is in at most one group.
+ */
+int tg_group_create(tg_group** group, int nctx, tg_ctx** ctxs, int use_graph);
+int tg_group_destroy(tg_group* group);
+int tg_group_project(tg_group* group);
+int tg_group_iterate(tg_group* group, double gradient_eps);
+int tg_group_iterate_results(tg_group* group, int index, double* pnp_scalars,
+                             double* gradient_norm, double* lagrange_lambda1,
+                             int* adjust_iterations, int* converged);
 
 /*
  * Device timing: with timing enabled, every stage and the dominant kernels of

@@ -153,11 +153,12 @@ gemm_batched_kernel(int M, int N, int K, typename Scal<TC>::type alpha,
                     const TA* __restrict__ A, int lda, long long sA,
                     const TB* __restrict__ B, int ldb, long long sB,
                     typename Scal<TC>::type beta, int use_beta,
-                    TC* __restrict__ C, int ldc, long long sC) {
+                    TC* __restrict__ C, int ldc, long long sC, int divA, int divB) {
   __shared__ typename Stage<TA>::type As[BK][BM + 1];
   __shared__ typename Stage<TB>::type Bs[BK][BN + 1];
   const long long b = blockIdx.z;
-  gemm_tile<TA, TB, TC, OPA, OPB>(M, N, K, alpha, A + b * sA, lda, B + b * sB, ldb, beta,
+  gemm_tile<TA, TB, TC, OPA, OPB>(M, N, K, alpha, A + (b / divA) * sA, lda, B + (b / divB) * sB,
+                                  ldb, beta,
                                   use_beta != 0, C + b * sC, ldc, blockIdx.x * BM,
                                   blockIdx.y * BN, As, Bs);
 }
@@ -182,11 +183,16 @@ gemm_grouped_kernel(int M, const cplx* __restrict__ X, cplx* __restrict__ C,
 template <typename TA, typename TB, typename TC, int OPA, int OPB>
 int launch(hipStream_t s, int M, int N, int K, typename Scal<TC>::type alpha, const TA* A,
            int lda, long long sA, const TB* B, int ldb, long long sB,
-           typename Scal<TC>::type beta, int use_beta, TC* C, int ldc, long long sC, int batch) {
+           typename Scal<TC>::type beta, int use_beta, TC* C, int ldc, long long sC, int batch,
+           int divA, int divB) {
   if (M <= 0 || N <= 0 || batch <= 0) return TG_OK;
+  if (batch > 65535) {
+    tg_set_error("gemm: batch too large");
+    return TG_E_ARG;
+  }
   dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
   hipLaunchKernelGGL((gemm_batched_kernel<TA, TB, TC, OPA, OPB>), grid, dim3(NT), 0, s, M, N,
-                     K, alpha, A, lda, sA, B, ldb, sB, beta, use_beta, C, ldc, sC);
+                     K, alpha, A, lda, sA, B, ldb, sB, beta, use_beta, C, ldc, sC, divA, divB);
   TG_HIP(hipGetLastError());
   return TG_OK;
 }
@@ -195,11 +201,11 @@ template <typename TA, typename TB, typename TC>
 int dispatch(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K,
              typename Scal<TC>::type alpha, const TA* A, int lda, long long sA, const TB* B,
              int ldb, long long sB, typename Scal<TC>::type beta, int use_beta, TC* C, int ldc,
-             long long sC, int batch) {
+             long long sC, int batch, int divA, int divB) {
 #define TG_CASE(OA, OB)                                                                   \
   if (opa == OA && opb == OB)                                                             \
     return launch<TA, TB, TC, OA, OB>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,    \
-                                      use_beta, C, ldc, sC, batch);
+                                      use_beta, C, ldc, sC, batch, divA, divB);
   TG_CASE(OP_N, OP_N) TG_CASE(OP_N, OP_T) TG_CASE(OP_N, OP_C)
   TG_CASE(OP_T, OP_N) TG_CASE(OP_T, OP_T) TG_CASE(OP_T, OP_C)
   TG_CASE(OP_C, OP_N) TG_CASE(OP_C, OP_T) TG_CASE(OP_C, OP_C)
@@ -339,49 +345,49 @@ int gemm_tune_grouped(hipStream_t s, int M, const cplx* X, cplx* C, const void* 
 
 int gemm_zz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
             const cplx* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
-            cplx beta, cplx* C, int ldc, long long sC, int batch) {
+            cplx beta, cplx* C, int ldc, long long sC, int batch, int divA, int divB) {
   if (M <= 0 || N <= 0 || batch <= 0) return TG_OK;
   if (sel().mfma)
     return mfma_gemm_zz(variant_for(GemmKey{K_ZZ, (int)opa, (int)opb, M, N, K, batch}), s, opa,
                         opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
-                        !is_zero(beta), C, ldc, sC, batch);
+                        !is_zero(beta), C, ldc, sC, batch, divA, divB);
   return dispatch<cplx, cplx, cplx>(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
-                                    !is_zero(beta), C, ldc, sC, batch);
+                                    !is_zero(beta), C, ldc, sC, batch, divA, divB);
 }
 
 int gemm_dz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
             const double* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
-            cplx beta, cplx* C, int ldc, long long sC, int batch) {
+            cplx beta, cplx* C, int ldc, long long sC, int batch, int divA, int divB) {
   if (M <= 0 || N <= 0 || batch <= 0) return TG_OK;
   if (opa == OP_C) opa = OP_T;
   if (sel().mfma)
     return mfma_gemm_dz(variant_for(GemmKey{K_DZ, (int)opa, (int)opb, M, N, K, batch}), s, opa,
                         opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
-                        !is_zero(beta), C, ldc, sC, batch);
+                        !is_zero(beta), C, ldc, sC, batch, divA, divB);
   return dispatch<double, cplx, cplx>(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
-                                      !is_zero(beta), C, ldc, sC, batch);
+                                      !is_zero(beta), C, ldc, sC, batch, divA, divB);
 }
 
 int gemm_zd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
             const cplx* A, int lda, long long sA, const double* B, int ldb, long long sB,
-            cplx beta, cplx* C, int ldc, long long sC, int batch) {
+            cplx beta, cplx* C, int ldc, long long sC, int batch, int divA, int divB) {
   if (M <= 0 || N <= 0 || batch <= 0) return TG_OK;
   if (opb == OP_C) opb = OP_T;
   if (sel().mfma)
     return mfma_gemm_zd(variant_for(GemmKey{K_ZD, (int)opa, (int)opb, M, N, K, batch}), s, opa,
                         opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
-                        !is_zero(beta), C, ldc, sC, batch);
+                        !is_zero(beta), C, ldc, sC, batch, divA, divB);
   return dispatch<cplx, double, cplx>(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
-                                      !is_zero(beta), C, ldc, sC, batch);
+                                      !is_zero(beta), C, ldc, sC, batch, divA, divB);
 }
 
 int gemm_dd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, double alpha,
             const double* A, int lda, long long sA, const double* B, int ldb, long long sB,
-            double beta, double* C, int ldc, long long sC, int batch) {
+            double beta, double* C, int ldc, long long sC, int batch, int divA, int divB) {
   if (opa == OP_C) opa = OP_T;
   if (opb == OP_C) opb = OP_T;
   return dispatch<double, double, double>(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB,
-                                          beta, beta != 0.0, C, ldc, sC, batch);
+                                          beta, beta != 0.0, C, ldc, sC, batch, divA, divB);
 }
 
 int gemm_grouped(hipStream_t s, int M, const cplx* X, cplx* C, const void* W, int w_is_fp32,

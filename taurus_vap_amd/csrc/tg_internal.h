@@ -82,25 +82,32 @@ struct DevBuf {
 // ---------------------------------------------------------------------------
 // GEMM (tg_gemm.hip):  C[b] = alpha * op(A[b]) op(B[b]) + beta * C[b]
 // column-major; op: 'N', 'T', 'C'; element types selected by the entry point.
+// A[b] = A + (b / divA) * sA, B[b] = B + (b / divB) * sB, C[b] = C + b * sC:
+// divA > 1 shares one A among divA consecutive batch elements (the per-state
+// operands of a batch of states x gauge angles).
 // ---------------------------------------------------------------------------
 enum TgOp { OP_N = 0, OP_T = 1, OP_C = 2 };
 
 // complex = complex x complex
 int gemm_zz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
             const cplx* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
-            cplx beta, cplx* C, int ldc, long long sC, int batch);
+            cplx beta, cplx* C, int ldc, long long sC, int batch, int divA = 1,
+            int divB = 1);
 // complex = real x complex
 int gemm_dz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
             const double* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
-            cplx beta, cplx* C, int ldc, long long sC, int batch);
+            cplx beta, cplx* C, int ldc, long long sC, int batch, int divA = 1,
+            int divB = 1);
 // complex = complex x real
 int gemm_zd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
             const cplx* A, int lda, long long sA, const double* B, int ldb, long long sB,
-            cplx beta, cplx* C, int ldc, long long sC, int batch);
+            cplx beta, cplx* C, int ldc, long long sC, int batch, int divA = 1,
+            int divB = 1);
 // real = real x real
 int gemm_dd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, double alpha,
             const double* A, int lda, long long sA, const double* B, int ldb, long long sB,
-            double beta, double* C, int ldc, long long sC, int batch);
+            double beta, double* C, int ldc, long long sC, int batch, int divA = 1,
+            int divB = 1);
 
 // Grouped GEMM for the contraction blocks:
 //   C_g (M x s_g, ld M) = X_g (M x s_g, ld M) * W_g (s_g x s_g, ld s_g)
@@ -163,13 +170,30 @@ struct Channel {
 
 // Configuration a captured projection graph is valid for.
 struct GraphKey {
-  long long v[16];
+  long long v[20];
   bool operator==(const GraphKey& o) const {
-    for (int i = 0; i < 16; ++i)
+    for (int i = 0; i < 20; ++i)
       if (v[i] != o.v[i]) return false;
     return true;
   }
 };
+
+// Canonical-basis data of one resident state (construct_canonical_basis).
+struct StateInfo {
+  int nocc = 0, nemp = 0;
+  double ovac = 0;
+  std::vector<double> voveru;                     // n - nocc values
+};
+
+// Per-state strides of the device layouts (S states of one context):
+//   d_U, d_V, d_D: state s at s * n^2;  d_real6: product k of state s at (k * S + s) * n^2
+//   d_out: state s at s * tg_out_stride(n) complex: pnp_A | pnp_H20 | pnp_AH20 | field_H20v
+//   d_Z: Zi [S n^2] | Zim1 [S n^2] | lambda [S * TG_MAX_CONS] | partial sums [S * tg_part_len(n)]
+inline long long tg_out_stride(int n) {
+  const long long nn = (long long)n * n;
+  return 3 * nn + (nn + 1) / 2;
+}
+inline long long tg_part_len(int n) { return (long long)n * n / 256 + 2; }
 
 struct tg_ctx {
   int n = 0;
@@ -186,13 +210,12 @@ struct tg_ctx {
   // contraction channels
   Channel ph, pp;
   int w_fp32 = 0;
-  // resident state
-  int A = 0;                                      // number of angles of the state
+  // resident states: ns independent states of this basis, projected together
+  int ns = 1;
+  int A = 0;                                      // number of angles of each state
   int Mphip = 0, Mphin = 0;
   double facpi = 0, valZ = 0, valN = 0, factor_delta = 1.0;
-  int nocc = 0, nemp = 0;
-  double ovac = 0;
-  std::vector<double> voveru;
+  std::vector<StateInfo> st;                      // canonical-basis data per state
   std::vector<double> phip, phin;
   std::vector<cplx> weight;                       // weip * wein
   std::vector<double> mult;                       // 2 for a conjugate pair, else 1
@@ -223,13 +246,16 @@ struct tg_ctx {
   int grad_ncons = 0;
   std::vector<double> grad_lambda;
   DevBuf d_c20, d_Z;                              // constraint_20; Zi, Zim1, lambda, norm
+  int c20_shared = 1;                             // constraint_20 / lambda common to all states
+  int z_states = 0;                               // states the Z buffers hold
   // constraints of the device iteration (tg_set_constraints / tg_iterate)
   int cons_dim = 0, cons_pair = 0;
   double cons_eps = 0;
-  std::vector<double> cons_val;
+  std::vector<double> cons_val;                   // cons_nsets x cons_dim
+  int cons_nsets = 1;
   DevBuf d_cHO;                                   // constraint operators, internal order
   DevBuf d_it;                                    // iteration work (n*n real matrices)
-  std::vector<double> lambda_prev;                // multipliers of the previous iteration
+  std::vector<double> lambda_prev;                // multipliers of the previous iteration (ns x dim)
   // timing: event pairs recorded without synchronisation, summed at collection
   struct Pending { std::string key; hipEvent_t e0, e1; };
   int timing = 0;
@@ -250,13 +276,20 @@ struct ScopedTiming {
 };
 int timing_collect(tg_ctx* c);
 
-// Projection of the resident state (tg_project.hip) and download of its sums.
+// Projection of the resident states (tg_project.hip) and download of the sums
+// of state s.
 int tg_project_device(tg_ctx* c);
-// gradient_kernel on the given buffers (calculate_gradient, gradient_type 0)
-int tg_gradient_step(tg_ctx* c, const double* H20v, double* Z, const double* Zm1,
-                     const double* lam, const double* c20, double* part);
-int tg_fetch_results(tg_ctx* c, double* scal, double* pA, double* pH, double* pAH, double* H20v,
-                     double* imag);
+// Allocations and set-up uploads of the resident states before a (captured)
+// projection; project_run launches only.  Both for the group capture.
+int tg_project_upload(tg_ctx* c);
+int tg_project_run(tg_ctx* c);
+// gradient_kernel for every state (calculate_gradient, gradient_type 0) with
+// per-state constraint_20 (c20 + s * ncons * n^2) and lambda (lam + s * TG_MAX_CONS)
+int tg_gradient_step(tg_ctx* c, const double* c20, const double* lam);
+int tg_fetch_results(tg_ctx* c, int state, double* scal, double* pA, double* pH, double* pAH,
+                     double* H20v, double* imag);
+// ensures the Z buffers for c->ns states (zeroed when they are (re)sized)
+int tg_ensure_z(tg_ctx* c);
 
 int contraction_build(tg_ctx* c, int64_t ntb, const int16_t* abcd, const void* H2,
                       int h2_bytes, const int8_t* trperm);

@@ -1,5 +1,6 @@
 // One gradient iteration of the reference driver on the device
-// (taurus_vap.f90:79-117 with gradient_type 0):
+// (taurus_vap.f90:77-113 with gradient_type 0), for every resident state of a
+// context at once:
 //   update_constraints_qpbasis          module_constraints.f90:381-397
 //   project_wavefunction(0, ...)        (tg_project.hip)
 //   calculate_lagrange_multipliers      module_constraints.f90:413-484
@@ -9,9 +10,16 @@
 //     evolve_wavefunction               module_constraints.f90:559-590
 // The constraint operators and the Bogoliubov matrices stay in HBM; the host
 // sees only the handful of constraint numbers that decide the adjustment loop
-// and its dim x dim linear systems.  The canonical basis of the new state is
-// left to the caller (tg_get_state, then tg_set_state).
+// and its dim x dim linear systems.  The states of a context run the
+// adjustment in lockstep: a state whose constraints are met keeps its Z (the
+// next evolutions reproduce the same U1, V1), so each state ends where its own
+// loop would have.  The canonical basis of the new states is left to the
+// caller (tg_get_state, then tg_set_state / tg_set_states).
+//
+// A group of contexts (tg_group_*) projects all its contexts, of any basis
+// sizes, as one hipGraph launch per iteration.
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -21,14 +29,20 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int NT1 = 1024;   // single-workgroup factorisations
+constexpr int MAXPAIRS = TG_MAX_CONS * (TG_MAX_CONS + 1) / 2;
 
 inline unsigned nblk(long long tot) { return (unsigned)((tot + NT - 1) / NT); }
 
-// C = (A - A^T) / 2 with A = T1 - T2  (set_operator_qpbasis 'f20'/'g20', factor -1)
+// C = (A - A^T) / 2 with A = T1 - T2  (set_operator_qpbasis 'f20'/'g20', factor -1);
+// state blockIdx.z: T1, T2 at + s * nn, C at + s * cs
 __global__ void __launch_bounds__(256)
-skew_half_kernel(int n, const double* __restrict__ T1, const double* __restrict__ T2,
-                 double* __restrict__ C) {
+skew_half_kernel(int n, const double* __restrict__ T10, const double* __restrict__ T20,
+                 double* __restrict__ C0, long long cs) {
   __shared__ double ta[32][33], tb[32][33];
+  const long long nn = (long long)n * n;
+  const double* T1 = T10 + blockIdx.z * nn;
+  const double* T2 = T20 + blockIdx.z * nn;
+  double* C = C0 + blockIdx.z * cs;
   const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   for (int r = ty; r < 32; r += 8) {
@@ -62,8 +76,10 @@ dots_kernel(long long len, const double* const* __restrict__ xs, const double* c
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
-__global__ void add_identity_kernel(int n, double* __restrict__ S) {
+// S += 1 for the matrix of state blockIdx.y (stride nn)
+__global__ void add_identity_kernel(int n, double* __restrict__ S0) {
   const int i = blockIdx.x * NT + threadIdx.x;
+  double* S = S0 + blockIdx.y * (long long)n * n;
   if (i < n) S[i + (long long)i * n] += 1.0;
 }
 
@@ -72,22 +88,37 @@ __global__ void copy_kernel(long long len, const double* __restrict__ src, doubl
   if (i < len) dst[i] = src[i];
 }
 
-// Z += sum_m x_m C20_m
-__global__ void axpy_cons_kernel(long long nn, int ncons, const double* __restrict__ x,
-                                 const double* __restrict__ c20, double* __restrict__ Z) {
+// dst = src for the states s (blockIdx.y, stride nn) with mask[s] != 0
+__global__ void masked_copy_kernel(long long nn, const int* __restrict__ mask,
+                                   const double* __restrict__ src, double* __restrict__ dst) {
+  const long long i = (long long)blockIdx.x * NT + threadIdx.x;
+  const long long o = blockIdx.y * nn;
+  if (i < nn && mask[blockIdx.y]) dst[o + i] = src[o + i];
+}
+
+// Z_s += sum_m x_s[m] C20_s,m   (state blockIdx.y: Z at + s nn, x at + s TG_MAX_CONS,
+// C20 at + s ncons nn)
+__global__ void axpy_cons_kernel(long long nn, int ncons, const double* __restrict__ x0,
+                                 const double* __restrict__ c200, double* __restrict__ Z0) {
   const long long i = (long long)blockIdx.x * NT + threadIdx.x;
   if (i >= nn) return;
+  const long long st = blockIdx.y;
+  const double* x = x0 + st * TG_MAX_CONS;
+  const double* c20 = c200 + st * ncons * nn;
+  double* Z = Z0 + st * nn;
   double s = Z[i];
   for (int m = 0; m < ncons; ++m) s = fma(x[m], c20[(long long)m * nn + i], s);
   Z[i] = s;
 }
 
 // In-place lower Cholesky factor of a symmetric positive matrix (column-major,
-// right-looking, one workgroup); the strict upper triangle is zeroed.
-// info = 0, or the first column with a non-positive pivot (1-based).
+// right-looking, one workgroup per matrix: blockIdx.x, stride n^2); the strict
+// upper triangle is zeroed.  info = 0, or the first column with a non-positive
+// pivot (1-based).
 __global__ void __launch_bounds__(NT1)
-cholesky_kernel(int n, double* __restrict__ A, int* __restrict__ info) {
+cholesky_kernel(int n, double* __restrict__ A0, int* __restrict__ info) {
   __shared__ int bad;
+  double* A = A0 + blockIdx.x * (long long)n * n;
   const int t = threadIdx.x;
   if (t == 0) bad = 0;
   __syncthreads();
@@ -116,12 +147,15 @@ cholesky_kernel(int n, double* __restrict__ A, int* __restrict__ info) {
     const int i = (int)(e % n), j = (int)(e / n);
     if (i < j) A[e] = 0.0;
   }
-  if (t == 0) *info = bad;
+  if (t == 0) info[blockIdx.x] = bad;
 }
 
-// X = L^-1 for lower-triangular L (dtrtri 'L','N'): thread j solves column j
+// X = L^-1 for lower-triangular L (dtrtri 'L','N'): thread j solves column j;
+// one workgroup per matrix (blockIdx.x, stride n^2)
 __global__ void __launch_bounds__(NT1)
-trtri_lower_kernel(int n, const double* __restrict__ L, double* __restrict__ X) {
+trtri_lower_kernel(int n, const double* __restrict__ L0, double* __restrict__ X0) {
+  const double* L = L0 + blockIdx.x * (long long)n * n;
+  double* X = X0 + blockIdx.x * (long long)n * n;
   for (int j = threadIdx.x; j < n; j += NT1) {
     for (int i = 0; i < j; ++i) X[i + (long long)j * n] = 0.0;
     X[j + (long long)j * n] = 1.0 / L[j + (long long)j * n];
@@ -159,43 +193,54 @@ bool small_solve(int m, std::vector<double> A, std::vector<double>& b) {
   return true;
 }
 
-// work layout: IT_COUNT matrices of n*n doubles, then a tail of
-//   [0, TAIL_X)      dot-product pointer lists and results (<= 3 * 2080 words)
-//   [TAIL_X, +64)    constraint corrections x
-//   [TAIL_INFO]      Cholesky info
+// Work layout: IT_COUNT slots of S n^2 doubles (matrix of state s at + s n^2),
+// then a tail of
+//   [0, 3 * S * MAXPAIRS)          dot-product pointer lists and results
+//   [TX, TX + S * TG_MAX_CONS)     constraint corrections x per state
+//   [TI, ...)                      Cholesky info and copy masks (ints, S each)
 enum { IT_S, IT_LINV, IT_A1, IT_A2, IT_U1, IT_V1, IT_R, IT_T1, IT_T2, IT_X, IT_COUNT };
-constexpr size_t TAIL_X = 2 * TG_MAX_CONS * TG_MAX_CONS;
-constexpr size_t TAIL_INFO = 4 * TG_MAX_CONS * TG_MAX_CONS;
-constexpr size_t TAIL_SIZE = TAIL_INFO + 64;
 
 struct Iter {
   tg_ctx* c;
-  int n;
+  int n, S;
   long long nn;
   hipStream_t s;
   double* w;
-  double* M(int k) const { return w + (size_t)k * nn; }
+  double* M(int k) const { return w + (size_t)k * S * nn; }
+  double* tail() const { return w + (size_t)IT_COUNT * S * nn; }
+  size_t tx() const { return 3 * (size_t)S * MAXPAIRS; }
+  size_t ti() const { return tx() + (size_t)S * TG_MAX_CONS; }
+  static size_t bytes(int n, int S) {
+    return sizeof(double) * ((size_t)IT_COUNT * S * n * n + 3 * (size_t)S * MAXPAIRS +
+                             (size_t)S * TG_MAX_CONS + (size_t)S + 64);
+  }
+  double* x() const { return tail() + tx(); }
+  int* info() const { return reinterpret_cast<int*>(tail() + ti()); }
+  int* mask() const { return info() + S; }
 };
 
-// constraint_20 of every constraint at (U, V) into c20 (dim x n*n)
+// constraint_20 of every constraint at (U, V) of every state into c20
+// (state s, constraint m at c20 + (s * dim + m) n^2)
 int constraints_qp(const Iter& it, const double* U, const double* V, double* c20) {
   tg_ctx* c = it.c;
-  const int n = it.n;
+  const int n = it.n, S = it.S, dim = c->cons_dim;
   const long long nn = it.nn;
-  for (int m = 0; m < c->cons_dim; ++m) {
+  double* X = it.M(IT_X);
+  double* T1 = it.M(IT_T1);
+  double* T2 = it.M(IT_T2);
+  for (int m = 0; m < dim; ++m) {
     const double* O = c->d_cHO.as<double>() + (size_t)m * nn;
-    const bool pair = m >= c->cons_dim - c->cons_pair;
-    double* X = it.M(IT_X);
+    const bool pair = m >= dim - c->cons_pair;
     // f20: T1 = U^T O V, T2 = V^T O^T U;  g20: T1 = U^T O U, T2 = V^T O V
-    TG_TRY(gemm_dd(it.s, OP_N, OP_N, n, n, n, 1.0, O, n, 0, pair ? U : V, n, 0, 0.0, X, n, 0, 1));
-    TG_TRY(gemm_dd(it.s, OP_T, OP_N, n, n, n, 1.0, U, n, 0, X, n, 0, 0.0, it.M(IT_T1), n, 0, 1));
+    TG_TRY(gemm_dd(it.s, OP_N, OP_N, n, n, n, 1.0, O, n, 0, pair ? U : V, n, nn, 0.0, X, n, nn, S));
+    TG_TRY(gemm_dd(it.s, OP_T, OP_N, n, n, n, 1.0, U, n, nn, X, n, nn, 0.0, T1, n, nn, S));
     if (pair)
-      TG_TRY(gemm_dd(it.s, OP_N, OP_N, n, n, n, 1.0, O, n, 0, V, n, 0, 0.0, X, n, 0, 1));
+      TG_TRY(gemm_dd(it.s, OP_N, OP_N, n, n, n, 1.0, O, n, 0, V, n, nn, 0.0, X, n, nn, S));
     else
-      TG_TRY(gemm_dd(it.s, OP_T, OP_N, n, n, n, 1.0, O, n, 0, U, n, 0, 0.0, X, n, 0, 1));
-    TG_TRY(gemm_dd(it.s, OP_T, OP_N, n, n, n, 1.0, V, n, 0, X, n, 0, 0.0, it.M(IT_T2), n, 0, 1));
-    hipLaunchKernelGGL(skew_half_kernel, dim3((n + 31) / 32, (n + 31) / 32), dim3(256), 0, it.s, n,
-                       it.M(IT_T1), it.M(IT_T2), c20 + (size_t)m * nn);
+      TG_TRY(gemm_dd(it.s, OP_T, OP_N, n, n, n, 1.0, O, n, 0, U, n, nn, 0.0, X, n, nn, S));
+    TG_TRY(gemm_dd(it.s, OP_T, OP_N, n, n, n, 1.0, V, n, nn, X, n, nn, 0.0, T2, n, nn, S));
+    hipLaunchKernelGGL(skew_half_kernel, dim3((n + 31) / 32, (n + 31) / 32, S), dim3(256), 0, it.s,
+                       n, T1, T2, c20 + (size_t)m * nn, (long long)dim * nn);
     TG_HIP(hipGetLastError());
   }
   return TG_OK;
@@ -207,13 +252,12 @@ int dots(const Iter& it, const std::vector<const double*>& xs, const std::vector
   const int np = (int)xs.size();
   out.assign(np, 0.0);
   if (np == 0) return TG_OK;
-  // pointer lists and results in the tail of the work area (TAIL_DOTS)
-  const double** pl = reinterpret_cast<const double**>(it.w + (size_t)IT_COUNT * it.nn);
-  double* res = reinterpret_cast<double*>(pl + 2 * np);
-  if (np > TG_MAX_CONS * (TG_MAX_CONS + 1) / 2) {
+  if (np > it.S * MAXPAIRS) {
     tg_set_error("constraint dot products: too many pairs");
     return TG_E_ARG;
   }
+  const double** pl = reinterpret_cast<const double**>(it.tail());
+  double* res = reinterpret_cast<double*>(pl + 2 * np);
   std::vector<const double*> h(2 * np);
   for (int p = 0; p < np; ++p) {
     h[p] = xs[p];
@@ -227,21 +271,380 @@ int dots(const Iter& it, const std::vector<const double*>& xs, const std::vector
   return TG_OK;
 }
 
-// Gram matrix A_ij = Tr(C20_i^T C20_j) (symmetric, dim x dim, host)
-int gram(const Iter& it, const double* c20, std::vector<double>& A) {
+// Gram matrices A_s,ij = Tr(C20_s,i^T C20_s,j) of the listed states (host, dim x dim each)
+int gram(const Iter& it, const double* c20, const std::vector<int>& states,
+         std::vector<std::vector<double>>& A) {
   const int m = it.c->cons_dim;
   std::vector<const double*> xs, ys;
-  for (int i = 0; i < m; ++i)
-    for (int j = i; j < m; ++j) {
-      xs.push_back(c20 + (size_t)i * it.nn);
-      ys.push_back(c20 + (size_t)j * it.nn);
-    }
+  for (int s : states)
+    for (int i = 0; i < m; ++i)
+      for (int j = i; j < m; ++j) {
+        xs.push_back(c20 + ((size_t)s * m + i) * it.nn);
+        ys.push_back(c20 + ((size_t)s * m + j) * it.nn);
+      }
   std::vector<double> d;
   TG_TRY(dots(it, xs, ys, d));
-  A.assign((size_t)m * m, 0.0);
+  A.assign(states.size(), std::vector<double>((size_t)m * m, 0.0));
   int k = 0;
-  for (int i = 0; i < m; ++i)
-    for (int j = i; j < m; ++j, ++k) A[i + j * m] = A[j + i * m] = d[k];
+  for (size_t q = 0; q < states.size(); ++q)
+    for (int i = 0; i < m; ++i)
+      for (int j = i; j < m; ++j, ++k) A[q][i + j * m] = A[q][j + i * m] = d[k];
+  return TG_OK;
+}
+
+// Outputs of the last iteration of a context (per state)
+struct IterOut {
+  std::vector<double> scal, gnorm, lambda;
+  std::vector<int> nadj, conv;
+};
+
+// ---- the iteration in two halves around the projection
+int iterate_pre(tg_ctx* c) {
+  if (!c->have_state || !c->grad_on) {
+    tg_set_error("tg_iterate: tg_set_state and tg_set_gradient must be called first");
+    return TG_E_ARG;
+  }
+  if (c->cons_nsets != 1 && c->cons_nsets != c->ns) {
+    tg_set_error("tg_iterate: constraint values given for " + std::to_string(c->cons_nsets) +
+                 " states, the context holds " + std::to_string(c->ns));
+    return TG_E_ARG;
+  }
+  const int n = c->n, S = c->ns, dim = c->cons_dim;
+  const long long nn = (long long)n * n;
+  TG_TRY(tg_ensure_z(c));
+  TG_TRY(c->d_it.ensure(Iter::bytes(n, S)));
+  TG_TRY(c->d_c20.ensure(sizeof(double) * std::max<size_t>(1, (size_t)S * dim * nn)));
+  Iter it{c, n, S, nn, c->stream, c->d_it.as<double>()};
+  // update_constraints_qpbasis
+  TG_TRY(constraints_qp(it, c->d_U.as<double>(), c->d_V.as<double>(), c->d_c20.as<double>()));
+  c->grad_ncons = dim;
+  c->c20_shared = 0;
+  return TG_OK;
+}
+
+int iterate_post(tg_ctx* c, double gradient_eps, IterOut& out) {
+  const int n = c->n, S = c->ns, dim = c->cons_dim;
+  const long long nn = (long long)n * n;
+  hipStream_t s = c->stream;
+  Iter it{c, n, S, nn, s, c->d_it.as<double>()};
+  double* U0 = c->d_U.as<double>();
+  double* V0 = c->d_V.as<double>();
+  double* c20 = c->d_c20.as<double>();
+  double* Zb = c->d_Z.as<double>();
+  double* Z = Zb;                                  // gradient_Zi, S states
+  double* Zm1 = Zb + S * nn;                       // gradient_Zim1
+  double* lam = Zb + 2 * S * nn;                   // S x TG_MAX_CONS
+  double* part = lam + (size_t)S * TG_MAX_CONS;    // S x tg_part_len(n)
+  const long long ps = tg_part_len(n);
+  const long long outs = tg_out_stride(n);
+  const cplx* out0 = c->d_out.as<cplx>();
+  auto H20v = [&](int st) {
+    return reinterpret_cast<const double*>(out0 + (size_t)st * outs + 3 * nn);
+  };
+  auto cval = [&](int st, int m) { return c->cons_val[(c->cons_nsets == 1 ? 0 : st) * dim + m]; };
+
+  // projected scalars, and the reference's stop on a complex gradient
+  // (calculate_projected_H20, module_projection.f90:751-755)
+  out.scal.assign((size_t)S * 14, 0.0);
+  for (int st = 0; st < S; ++st)
+    TG_TRY(tg_fetch_results(c, st, out.scal.data() + 14 * st, nullptr, nullptr, nullptr, nullptr,
+                            nullptr));
+  // calculate_lagrange_multipliers: A_ij = Tr(C_i C_j^T), b_i = Tr(H20 C_i^T)
+  out.lambda.assign((size_t)S * dim, 0.0);
+  if (c->lambda_prev.size() != (size_t)S * dim) c->lambda_prev.assign((size_t)S * dim, 0.0);
+  std::vector<double> lamh((size_t)S * TG_MAX_CONS, 0.0);
+  if (dim > 0) {
+    std::vector<int> all(S);
+    for (int st = 0; st < S; ++st) all[st] = st;
+    std::vector<std::vector<double>> A;
+    TG_TRY(gram(it, c20, all, A));
+    std::vector<const double*> xs, ys;
+    for (int st = 0; st < S; ++st)
+      for (int i = 0; i < dim; ++i) {
+        xs.push_back(H20v(st));
+        ys.push_back(c20 + ((size_t)st * dim + i) * nn);
+      }
+    std::vector<double> b;
+    TG_TRY(dots(it, xs, ys, b));
+    for (int st = 0; st < S; ++st) {
+      std::vector<double> bs(b.begin() + st * dim, b.begin() + (st + 1) * dim);
+      double* prev = c->lambda_prev.data() + (size_t)st * dim;
+      // the reference keeps the multipliers of the previous iteration when the
+      // system is singular (module_constraints.f90:471-477)
+      if (small_solve(dim, A[st], bs)) std::copy(bs.begin(), bs.end(), prev);
+      for (int i = 0; i < dim; ++i) {
+        lamh[(size_t)st * TG_MAX_CONS + i] = prev[i];
+        out.lambda[(size_t)st * dim + i] = prev[i];
+      }
+    }
+    TG_HIP(hipMemcpyAsync(lam, lamh.data(), sizeof(double) * lamh.size(), hipMemcpyHostToDevice, s));
+  }
+  // calculate_gradient (gradient_type 0): Z = -eta (H20v - sum_j lambda_j C20_j) + mu Zim1
+  TG_TRY(tg_gradient_step(c, c20, lam));
+  std::vector<double> ph((size_t)S * ps);
+  TG_HIP(hipMemcpyAsync(ph.data(), part, sizeof(double) * ph.size(), hipMemcpyDeviceToHost, s));
+  TG_HIP(hipStreamSynchronize(s));
+  const long long nb = (nn + NT - 1) / NT;
+  out.gnorm.assign(S, 0.0);
+  out.conv.assign(S, 0);
+  out.nadj.assign(S, 0);
+  std::vector<int> evolve;
+  for (int st = 0; st < S; ++st) {
+    double sum = 0;
+    for (long long k = 0; k < nb; ++k) sum += ph[(size_t)st * ps + k];
+    out.gnorm[st] = sqrt(sum) / c->grad_eta;
+    out.conv[st] = out.gnorm[st] <= gradient_eps;
+    if (!out.conv[st]) evolve.push_back(st);
+  }
+  if (evolve.empty()) return TG_OK;
+
+  // adjust_constraints: evolve with Z, then fix the constraints with the C20,
+  // every state in lockstep (a state whose constraints are met adds nothing to Z)
+  const int maxit = 100;
+  std::vector<int> open_state(S, 0);
+  for (int st : evolve) open_state[st] = 1;
+  double* xd = it.x();
+  int* dinfo = it.info();
+  for (int k = 1; k <= maxit; ++k) {
+    for (int st = 0; st < S; ++st)
+      if (open_state[st]) out.nadj[st] = k;
+    // cholesky_factorization: L L^T = 1 + Z^T Z
+    TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, n, 1.0, Z, n, nn, Z, n, nn, 0.0, it.M(IT_S), n, nn, S));
+    hipLaunchKernelGGL(add_identity_kernel, dim3(nblk(n), S), dim3(NT), 0, s, n, it.M(IT_S));
+    hipLaunchKernelGGL(cholesky_kernel, dim3(S), dim3(NT1), 0, s, n, it.M(IT_S), dinfo);
+    hipLaunchKernelGGL(trtri_lower_kernel, dim3(S), dim3(NT1), 0, s, n, it.M(IT_S),
+                       it.M(IT_LINV));
+    TG_HIP(hipGetLastError());
+    std::vector<int> info(S, 0);
+    TG_HIP(hipMemcpyAsync(info.data(), dinfo, sizeof(int) * S, hipMemcpyDeviceToHost, s));
+    // evolve_wavefunction: U1 = (U0 + V0 Z) L^-T, V1 = (V0 + U0 Z) L^-T
+    hipLaunchKernelGGL(copy_kernel, dim3(nblk(S * nn)), dim3(NT), 0, s, S * nn, U0, it.M(IT_A1));
+    hipLaunchKernelGGL(copy_kernel, dim3(nblk(S * nn)), dim3(NT), 0, s, S * nn, V0, it.M(IT_A2));
+    TG_HIP(hipGetLastError());
+    TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, V0, n, nn, Z, n, nn, 1.0, it.M(IT_A1), n, nn, S));
+    TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, U0, n, nn, Z, n, nn, 1.0, it.M(IT_A2), n, nn, S));
+    TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, it.M(IT_A1), n, nn, it.M(IT_LINV), n, nn, 0.0,
+                   it.M(IT_U1), n, nn, S));
+    TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, it.M(IT_A2), n, nn, it.M(IT_LINV), n, nn, 0.0,
+                   it.M(IT_V1), n, nn, S));
+    TG_HIP(hipStreamSynchronize(s));
+    for (int st : evolve)
+      if (info[st] != 0) {
+        tg_set_error("Cholesky factorization has failed, info = " + std::to_string(info[st]) +
+                     " (state " + std::to_string(st) + ")");
+        return TG_E_SINGULAR;
+      }
+    if (dim == 0) break;
+    // expectation values <O_m>: Tr(O rho) (one-body), Tr(O kappa^T) (pair)
+    const bool any_pair = c->cons_pair > 0, any_obo = dim > c->cons_pair;
+    double* rho = it.M(IT_R);
+    double* kap = it.M(IT_T1);
+    if (any_obo)
+      TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, it.M(IT_V1), n, nn, it.M(IT_V1), n, nn, 0.0,
+                     rho, n, nn, S));
+    if (any_pair)
+      TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, it.M(IT_V1), n, nn, it.M(IT_U1), n, nn, 0.0,
+                     kap, n, nn, S));
+    std::vector<int> act;
+    for (int st = 0; st < S; ++st)
+      if (open_state[st]) act.push_back(st);
+    std::vector<const double*> xs, ys;
+    for (int st : act)
+      for (int m = 0; m < dim; ++m) {
+        xs.push_back(c->d_cHO.as<double>() + (size_t)m * nn);
+        ys.push_back((m >= dim - c->cons_pair ? kap : rho) + (size_t)st * nn);
+      }
+    std::vector<double> ev;
+    TG_TRY(dots(it, xs, ys, ev));
+    std::vector<std::vector<double>> Bv(act.size(), std::vector<double>(dim));
+    std::vector<int> still;
+    for (size_t q = 0; q < act.size(); ++q) {
+      int open = 0;
+      for (int m = 0; m < dim; ++m) {
+        Bv[q][m] = cval(act[q], m) - ev[q * dim + m];
+        if (fabs(Bv[q][m]) > c->cons_eps) ++open;
+      }
+      if (open == 0) open_state[act[q]] = 0;
+      else still.push_back((int)q);
+    }
+    // constraint_20 at the evolved states
+    TG_TRY(constraints_qp(it, it.M(IT_U1), it.M(IT_V1), c20));
+    if (still.empty()) break;
+    std::vector<int> sts;
+    for (int q : still) sts.push_back(act[q]);
+    std::vector<std::vector<double>> A;
+    TG_TRY(gram(it, c20, sts, A));
+    std::vector<double> xh((size_t)S * TG_MAX_CONS, 0.0);
+    for (size_t r = 0; r < still.size(); ++r) {
+      std::vector<double> b = Bv[still[r]];
+      if (!small_solve(dim, A[r], b)) {
+        tg_set_error("adjust_constraints: singular constraint system (state " +
+                     std::to_string(sts[r]) + ")");
+        return TG_E_SINGULAR;
+      }
+      std::copy(b.begin(), b.end(), xh.begin() + (size_t)sts[r] * TG_MAX_CONS);
+    }
+    TG_HIP(hipMemcpyAsync(xd, xh.data(), sizeof(double) * xh.size(), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(axpy_cons_kernel, dim3(nblk(nn), S), dim3(NT), 0, s, nn, dim, xd, c20, Z);
+    TG_HIP(hipGetLastError());
+    TG_HIP(hipStreamSynchronize(s));
+  }
+  // the evolved states and the momentum term of their next step
+  std::vector<int> mask(S, 0);
+  for (int st : evolve) mask[st] = 1;
+  int* dmask = it.mask();
+  TG_HIP(hipMemcpyAsync(dmask, mask.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(masked_copy_kernel, dim3(nblk(nn), S), dim3(NT), 0, s, nn, dmask,
+                     it.M(IT_U1), U0);
+  hipLaunchKernelGGL(masked_copy_kernel, dim3(nblk(nn), S), dim3(NT), 0, s, nn, dmask,
+                     it.M(IT_V1), V0);
+  hipLaunchKernelGGL(masked_copy_kernel, dim3(nblk(nn), S), dim3(NT), 0, s, nn, dmask, Z, Zm1);
+  TG_HIP(hipGetLastError());
+  TG_HIP(hipStreamSynchronize(s));
+  c->have_state = 0;   // the canonical basis of the new states is still to be set
+  return TG_OK;
+}
+
+void copy_out(const tg_ctx* c, const IterOut& o, double* pnp_scalars, double* gradient_norm,
+              double* lagrange_lambda1, int* adjust_iterations, int* converged) {
+  const int S = c->ns;
+  if (pnp_scalars) std::copy(o.scal.begin(), o.scal.end(), pnp_scalars);
+  if (gradient_norm) std::copy(o.gnorm.begin(), o.gnorm.end(), gradient_norm);
+  if (lagrange_lambda1) std::copy(o.lambda.begin(), o.lambda.end(), lagrange_lambda1);
+  for (int st = 0; st < S; ++st) {
+    if (adjust_iterations) adjust_iterations[st] = o.nadj.empty() ? 0 : o.nadj[st];
+    if (converged) converged[st] = o.conv.empty() ? 0 : o.conv[st];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// group of contexts projected as one graph launch
+// ---------------------------------------------------------------------------
+struct tg_group {
+  std::vector<tg_ctx*> ctx;
+  hipEvent_t ev_start = nullptr;
+  std::vector<hipEvent_t> ev_done;
+  int use_graph = 1, graph_valid = 0, graph_seen = 0;
+  std::vector<GraphKey> key_v, seen_key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<IterOut> last;
+};
+
+namespace {
+
+GraphKey ctx_key(tg_ctx* c) {
+  GraphKey k{};
+  k.v[0] = (long long)(intptr_t)c;
+  k.v[1] = c->n;
+  k.v[2] = c->A;
+  k.v[3] = c->ns;
+  k.v[4] = c->iprint;
+  k.v[5] = c->grad_on;
+  k.v[6] = c->grad_ncons;
+  k.v[7] = c->c20_shared;
+  k.v[8] = (long long)devbuf_epoch();
+  k.v[9] = c->Mphip;
+  k.v[10] = c->Mphin;
+  unsigned long long h = 1469598103934665603ULL;
+  for (const StateInfo& si : c->st)
+    for (int x : {si.nocc, si.nemp}) {
+      h ^= (unsigned long long)(unsigned)x;
+      h *= 1099511628211ULL;
+    }
+  k.v[11] = (long long)h;
+  double d[3] = {c->factor_delta, c->grad_eta, c->grad_mu};
+  std::memcpy(&k.v[12], d, sizeof(d));
+  k.v[15] = c->plain;
+  k.v[16] = c->w_fp32;
+  return k;
+}
+
+void group_release_graph(tg_group* g) {
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->exec = nullptr;
+  g->graph = nullptr;
+  g->graph_valid = 0;
+}
+
+// Launches of every context: context 0 on its own stream, the others forked
+// from it and joined back, so one capture on context 0's stream holds them all.
+int group_launch(tg_group* g) {
+  tg_ctx* c0 = g->ctx[0];
+  const size_t k = g->ctx.size();
+  TG_HIP(hipEventRecord(g->ev_start, c0->stream));
+  for (size_t i = 1; i < k; ++i) TG_HIP(hipStreamWaitEvent(g->ctx[i]->stream, g->ev_start, 0));
+  for (size_t i = 1; i < k; ++i) TG_TRY(tg_project_run(g->ctx[i]));
+  TG_TRY(tg_project_run(c0));
+  for (size_t i = 1; i < k; ++i) {
+    TG_HIP(hipEventRecord(g->ev_done[i], g->ctx[i]->stream));
+    TG_HIP(hipStreamWaitEvent(c0->stream, g->ev_done[i], 0));
+  }
+  return TG_OK;
+}
+
+// the projections of all contexts; graph capture on the second call of a
+// configuration, replay afterwards.  Every context's stream is ordered after
+// the work when this returns.
+int group_project(tg_group* g) {
+  for (tg_ctx* c : g->ctx) {
+    if (!c->have_state) {
+      tg_set_error("tg_group_project: a context has no state");
+      return TG_E_ARG;
+    }
+    TG_TRY(tg_project_upload(c));
+  }
+  tg_ctx* c0 = g->ctx[0];
+  std::vector<GraphKey> keys;
+  for (tg_ctx* c : g->ctx) keys.push_back(ctx_key(c));
+  bool timing = false;
+  for (tg_ctx* c : g->ctx) timing = timing || c->timing;
+  int rc = TG_OK;
+  if (!g->use_graph || timing) {   // event pairs of the timing are not captured
+    rc = group_launch(g);
+  } else if (g->graph_valid && keys == g->key_v) {
+    TG_HIP(hipGraphLaunch(g->exec, c0->stream));
+  } else if (!(g->graph_seen && keys == g->seen_key)) {
+    g->graph_seen = 1;
+    g->seen_key = keys;
+    rc = group_launch(g);
+  } else {
+    group_release_graph(g);
+    TG_HIP(hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
+    const int r = group_launch(g);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(c0->stream, &gr);
+    if (r != TG_OK || e != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      if (r != TG_OK) return r;
+      tg_set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      return TG_E_HIP;
+    }
+    g->graph = gr;
+    if (hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0) != hipSuccess) {
+      g->exec = nullptr;
+      group_release_graph(g);
+      tg_set_error("hipGraphInstantiate failed");
+      return TG_E_HIP;
+    }
+    g->graph_valid = 1;
+    g->key_v = keys;
+    if (std::getenv("TG_GRAPH_VERBOSE")) {
+      size_t nodes = 0;
+      (void)hipGraphGetNodes(gr, nullptr, &nodes);
+      fprintf(stderr, "[taurus_gpu] group of %zu contexts captured: %zu graph nodes\n",
+              g->ctx.size(), nodes);
+    }
+    TG_HIP(hipGraphLaunch(g->exec, c0->stream));
+  }
+  TG_TRY(rc);
+  // later work and synchronisations on any context's stream follow the launch
+  TG_HIP(hipEventRecord(g->ev_start, c0->stream));
+  for (size_t i = 1; i < g->ctx.size(); ++i)
+    TG_HIP(hipStreamWaitEvent(g->ctx[i]->stream, g->ev_start, 0));
   return TG_OK;
 }
 
@@ -250,10 +653,10 @@ int gram(const Iter& it, const double* c20, std::vector<double>& A) {
 extern "C" {
 
 int tg_set_constraints(tg_ctx* c, int constraint_dim, int constraint_pair,
-                       const double* constraint_HO, const double* constraint_val,
+                       const double* constraint_HO, int nsets, const double* constraint_val,
                        double constraint_eps) {
   if (!c || constraint_dim < 0 || constraint_dim > TG_MAX_CONS || constraint_pair < 0 ||
-      constraint_pair > constraint_dim ||
+      constraint_pair > constraint_dim || nsets < 1 ||
       (constraint_dim > 0 && (!constraint_HO || !constraint_val))) {
     tg_set_error("tg_set_constraints: invalid argument");
     return TG_E_ARG;
@@ -263,7 +666,12 @@ int tg_set_constraints(tg_ctx* c, int constraint_dim, int constraint_pair,
   c->cons_dim = constraint_dim;
   c->cons_pair = constraint_pair;
   c->cons_eps = constraint_eps;
-  c->cons_val.assign(constraint_val, constraint_val + constraint_dim);
+  c->cons_nsets = nsets;
+  if (constraint_dim > 0)
+    c->cons_val.assign(constraint_val, constraint_val + (size_t)nsets * constraint_dim);
+  else
+    c->cons_val.clear();
+  c->lambda_prev.clear();
   if (constraint_dim == 0) return TG_OK;
   std::vector<double> h((size_t)constraint_dim * nn);
   for (int m = 0; m < constraint_dim; ++m)
@@ -278,156 +686,20 @@ int tg_set_constraints(tg_ctx* c, int constraint_dim, int constraint_pair,
 
 int tg_iterate(tg_ctx* c, double gradient_eps, double* pnp_scalars, double* gradient_norm,
                double* lagrange_lambda1, int* adjust_iterations, int* converged) {
-  if (!c || !c->have_state || !c->grad_on) {
-    tg_set_error("tg_iterate: tg_set_state and tg_set_gradient must be called first");
+  if (!c) {
+    tg_set_error("tg_iterate: invalid argument");
     return TG_E_ARG;
   }
-  const int n = c->n, dim = c->cons_dim;
-  const long long nn = (long long)n * n;
-  hipStream_t s = c->stream;
-  TG_TRY(c->d_it.ensure(sizeof(double) * ((size_t)IT_COUNT * nn + TAIL_SIZE)));
-  TG_TRY(c->d_c20.ensure(sizeof(double) * std::max<size_t>(1, (size_t)dim * nn)));
-  Iter it{c, n, nn, s, c->d_it.as<double>()};
-  double* U0 = c->d_U.as<double>();
-  double* V0 = c->d_V.as<double>();
-  double* c20 = c->d_c20.as<double>();
-  double* Zb = c->d_Z.as<double>();
-  double* Z = Zb;                 // gradient_Zi
-  double* Zm1 = Zb + nn;          // gradient_Zim1
-  double* lam = Zb + 2 * nn;
-  double* part = lam + TG_MAX_CONS;
-
-  // update_constraints_qpbasis
-  TG_TRY(constraints_qp(it, U0, V0, c20));
-  // project_wavefunction (the gradient step is taken below, with new multipliers)
+  TG_TRY(iterate_pre(c));
+  // project_wavefunction (the gradient step is taken after, with new multipliers)
   const int grad_was = c->grad_on;
   c->grad_on = 0;
-  int r = tg_project_device(c);
+  const int r = tg_project_device(c);
   c->grad_on = grad_was;
   TG_TRY(r);
-  // projected scalars, and the reference's stop on a complex gradient
-  // (calculate_projected_H20, module_projection.f90:751-755)
-  TG_TRY(tg_fetch_results(c, pnp_scalars, nullptr, nullptr, nullptr, nullptr, nullptr));
-  double* H20v = reinterpret_cast<double*>(c->d_out.as<double>() + 6 * nn);
-  // calculate_lagrange_multipliers: A_ij = Tr(C_i C_j^T), b_i = Tr(H20 C_i^T)
-  std::vector<double> lamh(dim, 0.0);
-  if (dim > 0) {
-    std::vector<double> A, b;
-    TG_TRY(gram(it, c20, A));
-    std::vector<const double*> xs, ys;
-    for (int i = 0; i < dim; ++i) {
-      xs.push_back(H20v);
-      ys.push_back(c20 + (size_t)i * nn);
-    }
-    TG_TRY(dots(it, xs, ys, b));
-    if (small_solve(dim, A, b)) {
-      lamh = b;
-      c->lambda_prev = lamh;
-    } else {
-      // the reference keeps the multipliers of the previous iteration
-      lamh = c->lambda_prev;
-      lamh.resize(dim, 0.0);
-    }
-    TG_HIP(hipMemcpyAsync(lam, lamh.data(), sizeof(double) * dim, hipMemcpyHostToDevice, s));
-  }
-  // calculate_gradient (gradient_type 0)
-  c->grad_ncons = dim;
-  {
-    const long long nb = (nn + NT - 1) / NT;
-    // Z = -eta (H20v - sum_j lambda_j C20_j) + mu Zim1; partial sums of Z^2
-    TG_TRY(tg_gradient_step(c, H20v, Z, Zm1, lam, c20, part));
-    std::vector<double> ph(nb);
-    TG_HIP(hipMemcpyAsync(ph.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
-    TG_HIP(hipStreamSynchronize(s));
-    double sum = 0;
-    for (double v : ph) sum += v;
-    const double gn = sqrt(sum) / c->grad_eta;
-    if (gradient_norm) *gradient_norm = gn;
-    if (converged) *converged = gn <= gradient_eps;
-    if (lagrange_lambda1)
-      for (int i = 0; i < dim; ++i) lagrange_lambda1[i] = lamh[i];
-    if (gn <= gradient_eps) {
-      if (adjust_iterations) *adjust_iterations = 0;
-      return TG_OK;
-    }
-  }
-  // adjust_constraints: evolve with Z, then fix the constraints with the C20
-  int iters = 0;
-  const int maxit = 100;
-  int* dinfo = reinterpret_cast<int*>(it.w + (size_t)IT_COUNT * nn + TAIL_INFO);
-  for (int k = 1; k <= maxit; ++k) {
-    iters = k;
-    // cholesky_factorization: L L^T = 1 + Z^T Z
-    TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, n, 1.0, Z, n, 0, Z, n, 0, 0.0, it.M(IT_S), n, 0, 1));
-    hipLaunchKernelGGL(add_identity_kernel, dim3(nblk(n)), dim3(NT), 0, s, n, it.M(IT_S));
-    hipLaunchKernelGGL(cholesky_kernel, dim3(1), dim3(NT1), 0, s, n, it.M(IT_S), dinfo);
-    hipLaunchKernelGGL(trtri_lower_kernel, dim3(1), dim3(NT1), 0, s, n, it.M(IT_S),
-                       it.M(IT_LINV));
-    TG_HIP(hipGetLastError());
-    int info = 0;
-    TG_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
-    // evolve_wavefunction: U1 = (U0 + V0 Z) L^-T, V1 = (V0 + U0 Z) L^-T
-    hipLaunchKernelGGL(copy_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, U0, it.M(IT_A1));
-    hipLaunchKernelGGL(copy_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, V0, it.M(IT_A2));
-    TG_HIP(hipGetLastError());
-    TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, V0, n, 0, Z, n, 0, 1.0, it.M(IT_A1), n, 0, 1));
-    TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, U0, n, 0, Z, n, 0, 1.0, it.M(IT_A2), n, 0, 1));
-    TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, it.M(IT_A1), n, 0, it.M(IT_LINV), n, 0, 0.0,
-                   it.M(IT_U1), n, 0, 1));
-    TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, it.M(IT_A2), n, 0, it.M(IT_LINV), n, 0, 0.0,
-                   it.M(IT_V1), n, 0, 1));
-    TG_HIP(hipStreamSynchronize(s));
-    if (info != 0) {
-      tg_set_error("Cholesky factorization has failed, info = " + std::to_string(info));
-      return TG_E_SINGULAR;
-    }
-    if (dim == 0) break;
-    // expectation values <O_m>: Tr(O rho) (one-body), Tr(O kappa^T) (pair)
-    std::vector<const double*> xs, ys;
-    std::vector<double> ev;
-    const bool any_pair = c->cons_pair > 0, any_obo = dim > c->cons_pair;
-    double* rho = it.M(IT_R);
-    double* kap = it.M(IT_T1);
-    if (any_obo)
-      TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, it.M(IT_V1), n, 0, it.M(IT_V1), n, 0, 0.0, rho,
-                     n, 0, 1));
-    if (any_pair)
-      TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, it.M(IT_V1), n, 0, it.M(IT_U1), n, 0, 0.0, kap,
-                     n, 0, 1));
-    for (int m = 0; m < dim; ++m) {
-      xs.push_back(c->d_cHO.as<double>() + (size_t)m * nn);
-      ys.push_back(m >= dim - c->cons_pair ? kap : rho);
-    }
-    TG_TRY(dots(it, xs, ys, ev));
-    std::vector<double> B(dim);
-    int open = 0;
-    for (int m = 0; m < dim; ++m) {
-      B[m] = c->cons_val[m] - ev[m];
-      if (fabs(B[m]) > c->cons_eps) ++open;
-    }
-    // constraint_20 at the evolved state
-    TG_TRY(constraints_qp(it, it.M(IT_U1), it.M(IT_V1), c20));
-    if (open == 0) break;
-    std::vector<double> A;
-    TG_TRY(gram(it, c20, A));
-    if (!small_solve(dim, A, B)) {
-      tg_set_error("adjust_constraints: singular constraint system");
-      return TG_E_SINGULAR;
-    }
-    double* xd = it.w + (size_t)IT_COUNT * nn + TAIL_X;
-    TG_HIP(hipMemcpyAsync(xd, B.data(), sizeof(double) * dim, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(axpy_cons_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, dim, xd, c20, Z);
-    TG_HIP(hipGetLastError());
-    TG_HIP(hipStreamSynchronize(s));
-  }
-  // the evolved state and the momentum term of the next step
-  hipLaunchKernelGGL(copy_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, it.M(IT_U1), U0);
-  hipLaunchKernelGGL(copy_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, it.M(IT_V1), V0);
-  hipLaunchKernelGGL(copy_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, Z, Zm1);
-  TG_HIP(hipGetLastError());
-  TG_HIP(hipStreamSynchronize(s));
-  if (adjust_iterations) *adjust_iterations = iters;
-  c->have_state = 0;   // the canonical basis of the new state is still to be set
+  IterOut o;
+  TG_TRY(iterate_post(c, gradient_eps, o));
+  copy_out(c, o, pnp_scalars, gradient_norm, lagrange_lambda1, adjust_iterations, converged);
   return TG_OK;
 }
 
@@ -436,18 +708,93 @@ int tg_get_state(tg_ctx* c, double* U, double* V) {
     tg_set_error("tg_get_state: invalid argument");
     return TG_E_ARG;
   }
-  const int n = c->n;
+  const int n = c->n, S = c->ns;
   const size_t nn = (size_t)n * n;
-  std::vector<double> h(2 * nn);
+  std::vector<double> h(2 * nn * S);
   TG_HIP(hipStreamSynchronize(c->stream));
-  TG_HIP(hipMemcpy(h.data(), c->d_U.p, sizeof(double) * nn, hipMemcpyDeviceToHost));
-  TG_HIP(hipMemcpy(h.data() + nn, c->d_V.p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(h.data(), c->d_U.p, sizeof(double) * nn * S, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(h.data() + nn * S, c->d_V.p, sizeof(double) * nn * S, hipMemcpyDeviceToHost));
   // rows back to the reference single-particle order
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) {
-      U[c->sp_perm[i] + (size_t)j * n] = h[i + (size_t)j * n];
-      V[c->sp_perm[i] + (size_t)j * n] = h[nn + i + (size_t)j * n];
+  for (int k = 0; k < S; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const size_t dst = k * nn + c->sp_perm[i] + (size_t)j * n, src = k * nn + i + (size_t)j * n;
+        U[dst] = h[src];
+        V[dst] = h[nn * S + src];
+      }
+  return TG_OK;
+}
+
+int tg_group_create(tg_group** out, int nctx, tg_ctx** ctxs, int use_graph) {
+  if (!out || nctx < 1 || !ctxs) {
+    tg_set_error("tg_group_create: invalid argument");
+    return TG_E_ARG;
+  }
+  for (int i = 0; i < nctx; ++i) {
+    bool bad = !ctxs[i];
+    for (int j = 0; j < i && !bad; ++j) bad = ctxs[i] == ctxs[j];
+    if (bad) {
+      tg_set_error("tg_group_create: contexts must be distinct and non-null");
+      return TG_E_ARG;
     }
+  }
+  tg_group* g = new tg_group();
+  g->ctx.assign(ctxs, ctxs + nctx);
+  g->use_graph = use_graph ? 1 : 0;
+  g->ev_done.assign(nctx, nullptr);
+  bool ok = hipEventCreateWithFlags(&g->ev_start, hipEventDisableTiming) == hipSuccess;
+  for (int i = 1; i < nctx && ok; ++i)
+    ok = hipEventCreateWithFlags(&g->ev_done[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    tg_group_destroy(g);
+    tg_set_error("tg_group_create: event creation failed");
+    return TG_E_HIP;
+  }
+  *out = g;
+  return TG_OK;
+}
+
+int tg_group_destroy(tg_group* g) {
+  if (!g) return TG_OK;
+  for (tg_ctx* c : g->ctx)
+    if (c && c->stream) (void)hipStreamSynchronize(c->stream);
+  group_release_graph(g);
+  if (g->ev_start) (void)hipEventDestroy(g->ev_start);
+  for (hipEvent_t e : g->ev_done)
+    if (e) (void)hipEventDestroy(e);
+  delete g;
+  return TG_OK;
+}
+
+int tg_group_project(tg_group* g) {
+  if (!g) return TG_E_ARG;
+  return group_project(g);
+}
+
+int tg_group_iterate(tg_group* g, double gradient_eps) {
+  if (!g) return TG_E_ARG;
+  for (tg_ctx* c : g->ctx) TG_TRY(iterate_pre(c));
+  std::vector<int> was;
+  for (tg_ctx* c : g->ctx) {
+    was.push_back(c->grad_on);
+    c->grad_on = 0;
+  }
+  const int r = group_project(g);
+  for (size_t i = 0; i < g->ctx.size(); ++i) g->ctx[i]->grad_on = was[i];
+  TG_TRY(r);
+  g->last.assign(g->ctx.size(), IterOut());
+  for (size_t i = 0; i < g->ctx.size(); ++i) TG_TRY(iterate_post(g->ctx[i], gradient_eps, g->last[i]));
+  return TG_OK;
+}
+
+int tg_group_iterate_results(tg_group* g, int index, double* pnp_scalars, double* gradient_norm,
+                             double* lagrange_lambda1, int* adjust_iterations, int* converged) {
+  if (!g || index < 0 || index >= (int)g->last.size()) {
+    tg_set_error("tg_group_iterate_results: no such context or no iteration yet");
+    return TG_E_ARG;
+  }
+  copy_out(g->ctx[index], g->last[index], pnp_scalars, gradient_norm, lagrange_lambda1,
+           adjust_iterations, converged);
   return TG_OK;
 }
 

@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(64 * WM * WN)
 mfma_batched_kernel(int M, int N, int K, cplx alpha, const TA* __restrict__ A, int lda,
                     long long sA, const TB* __restrict__ B, int ldb, long long sB, cplx beta,
                     int use_beta, cplx* __restrict__ C, int ldc, long long sC, int batch,
-                    int tiles_m, int tiles) {
+                    int tiles_m, int tiles, int divA, int divB) {
   typedef Tile<TA, TB, OPA, OPB, BM, BN, WM, WN> T;
   __shared__ typename T::Smem sm;
   const int L = blockIdx.x;
@@ -216,8 +216,8 @@ mfma_batched_kernel(int M, int N, int K, cplx alpha, const TA* __restrict__ A, i
   const long long b = xcd + (long long)XCDS * slot;
   if (b >= batch) return;
   const int tm = tile % tiles_m, tn = tile / tiles_m;
-  T::run(M, N, K, alpha, A + b * sA, lda, B + b * sB, ldb, beta, use_beta != 0, C + b * sC, ldc,
-         tm * BM, tn * BN, sm);
+  T::run(M, N, K, alpha, A + (b / divA) * sA, lda, B + (b / divB) * sB, ldb, beta, use_beta != 0,
+         C + b * sC, ldc, tm * BM, tn * BN, sm);
 }
 
 // Grouped contraction: C_g (M x s_g) = X_g (M x s_g) W_g (s_g x s_g), W real (float/double).
@@ -238,7 +238,7 @@ mfma_grouped_kernel(int M, const cplx* __restrict__ X, cplx* __restrict__ C,
 template <typename TA, typename TB, int OPA, int OPB, int BM, int BN, int WM, int WN>
 int launch_batched(hipStream_t s, int M, int N, int K, cplx alpha, const TA* A, int lda,
                    long long sA, const TB* B, int ldb, long long sB, cplx beta, int use_beta,
-                   cplx* C, int ldc, long long sC, int batch) {
+                   cplx* C, int ldc, long long sC, int batch, int divA, int divB) {
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   const int tiles = tiles_m * tiles_n;
   const long long slots = (batch + XCDS - 1) / XCDS;
@@ -249,7 +249,7 @@ int launch_batched(hipStream_t s, int M, int N, int K, cplx alpha, const TA* A, 
   }
   hipLaunchKernelGGL((mfma_batched_kernel<TA, TB, OPA, OPB, BM, BN, WM, WN>), dim3((unsigned)grid),
                      dim3(64 * WM * WN), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
-                     use_beta, C, ldc, sC, batch, tiles_m, tiles);
+                     use_beta, C, ldc, sC, batch, tiles_m, tiles, divA, divB);
   TG_HIP(hipGetLastError());
   return TG_OK;
 }
@@ -259,21 +259,22 @@ int launch_batched(hipStream_t s, int M, int N, int K, cplx alpha, const TA* A, 
 template <typename TA, typename TB, int OPA, int OPB>
 int launch_variant(int variant, hipStream_t s, int M, int N, int K, cplx alpha, const TA* A,
                    int lda, long long sA, const TB* B, int ldb, long long sB, cplx beta,
-                   int use_beta, cplx* C, int ldc, long long sC, int batch) {
+                   int use_beta, cplx* C, int ldc, long long sC, int batch, int divA, int divB) {
   switch (variant) {
     case 0:
       return launch_batched<TA, TB, OPA, OPB, 64, 64, 2, 2>(s, M, N, K, alpha, A, lda, sA, B, ldb,
-                                                            sB, beta, use_beta, C, ldc, sC, batch);
+                                                            sB, beta, use_beta, C, ldc, sC, batch, divA, divB);
     case 1:
       return launch_batched<TA, TB, OPA, OPB, 48, 48, 1, 3>(s, M, N, K, alpha, A, lda, sA, B, ldb,
-                                                            sB, beta, use_beta, C, ldc, sC, batch);
+                                                            sB, beta, use_beta, C, ldc, sC, batch, divA, divB);
     case 2:
       return launch_batched<TA, TB, OPA, OPB, 112, 112, 1, 7>(s, M, N, K, alpha, A, lda, sA, B,
                                                               ldb, sB, beta, use_beta, C, ldc, sC,
-                                                              batch);
+                                                              batch, divA, divB);
     case 3:
       return launch_batched<TA, TB, OPA, OPB, 96, 64, 2, 2>(s, M, N, K, alpha, A, lda, sA, B, ldb,
-                                                            sB, beta, use_beta, C, ldc, sC, batch);
+                                                            sB, beta, use_beta, C, ldc, sC, batch,
+                                                            divA, divB);
   }
   tg_set_error("mfma gemm: bad tile variant");
   return TG_E_ARG;
@@ -282,11 +283,11 @@ int launch_variant(int variant, hipStream_t s, int M, int N, int K, cplx alpha, 
 template <typename TA, typename TB>
 int dispatch(int variant, hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
              const TA* A, int lda, long long sA, const TB* B, int ldb, long long sB, cplx beta,
-             int use_beta, cplx* C, int ldc, long long sC, int batch) {
+             int use_beta, cplx* C, int ldc, long long sC, int batch, int divA, int divB) {
 #define TG_CASE(OA, OB)                                                                        \
   if (opa == OA && opb == OB)                                                                  \
     return launch_variant<TA, TB, OA, OB>(variant, s, M, N, K, alpha, A, lda, sA, B, ldb, sB,  \
-                                          beta, use_beta, C, ldc, sC, batch);
+                                          beta, use_beta, C, ldc, sC, batch, divA, divB);
   TG_CASE(OP_N, OP_N) TG_CASE(OP_N, OP_T) TG_CASE(OP_T, OP_N) TG_CASE(OP_T, OP_T)
   if (IsC<TA>::v) { TG_CASE(OP_C, OP_N) TG_CASE(OP_C, OP_T) }
   if (IsC<TB>::v) { TG_CASE(OP_N, OP_C) TG_CASE(OP_T, OP_C) }
@@ -300,25 +301,28 @@ int dispatch(int variant, hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K
 
 int mfma_gemm_zz(int variant, hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
                  const cplx* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
-                 cplx beta, int use_beta, cplx* C, int ldc, long long sC, int batch) {
+                 cplx beta, int use_beta, cplx* C, int ldc, long long sC, int batch, int divA,
+                 int divB) {
   return dispatch<cplx, cplx>(variant, s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
-                              use_beta, C, ldc, sC, batch);
+                              use_beta, C, ldc, sC, batch, divA, divB);
 }
 
 int mfma_gemm_dz(int variant, hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
                  const double* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
-                 cplx beta, int use_beta, cplx* C, int ldc, long long sC, int batch) {
+                 cplx beta, int use_beta, cplx* C, int ldc, long long sC, int batch, int divA,
+                 int divB) {
   if (opa == OP_C) opa = OP_T;
   return dispatch<double, cplx>(variant, s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
-                                use_beta, C, ldc, sC, batch);
+                                use_beta, C, ldc, sC, batch, divA, divB);
 }
 
 int mfma_gemm_zd(int variant, hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
                  const cplx* A, int lda, long long sA, const double* B, int ldb, long long sB,
-                 cplx beta, int use_beta, cplx* C, int ldc, long long sC, int batch) {
+                 cplx beta, int use_beta, cplx* C, int ldc, long long sC, int batch, int divA,
+                 int divB) {
   if (opb == OP_C) opb = OP_T;
   return dispatch<cplx, double>(variant, s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
-                                use_beta, C, ldc, sC, batch);
+                                use_beta, C, ldc, sC, batch, divA, divB);
 }
 
 int mfma_gemm_grouped(int variant, hipStream_t s, int M, const cplx* X, cplx* C, const void* W,

@@ -77,21 +77,28 @@ inline unsigned nblk(long long tot) { return (unsigned)((tot + NT - 1) / NT); }
 // ---------------------------------------------------------------------------
 // elementwise kernels
 // ---------------------------------------------------------------------------
-// U_phi, V_phi of every angle from the six real matrices
-// (R6 = [Kp Kn Lp Ln Mp Mn], Kx = U^T Pi_x U, Lx = V^T Pi_x V, Mx = V^T Pi_x U)
-__global__ void build_uv_phi_kernel(int n, int na, const double* __restrict__ R6,
-                                    const cplx* __restrict__ ph, cplx* __restrict__ Uphi,
-                                    cplx* __restrict__ Vphi, long long stride) {
+// U_phi, V_phi of every item from the six real matrices of its state
+// (R6 = [Kp Kn Lp Ln Mp Mn], Kx = U^T Pi_x U, Lx = V^T Pi_x V, Mx = V^T Pi_x U;
+// product k of state s at (k * S + s) * n^2).  Item b0 + a is angle (b0 + a) % A
+// of state (b0 + a) / A.
+__global__ void build_uv_phi_kernel(int n, int na, long long b0, int A, int S,
+                                    const double* __restrict__ R6, const cplx* __restrict__ ph,
+                                    cplx* __restrict__ Uphi, cplx* __restrict__ Vphi,
+                                    long long stride) {
   const long long nn = (long long)n * n;
   long long idx = (long long)blockIdx.x * NT + threadIdx.x;
   if (idx >= nn * na) return;
   const long long a = idx / nn;
+  const long long b = b0 + a;
+  const long long st = b / A, ang = b % A;
   const int r = (int)(idx % nn);
   const int i = r % n, j = r / n;
   const long long rt = j + (long long)i * n;
-  const cplx ep = ph[2 * a], en = ph[2 * a + 1];
-  const double Kp = R6[r], Kn = R6[nn + r], Lp = R6[2 * nn + r], Ln = R6[3 * nn + r];
-  const double Mp = R6[4 * nn + r], Mn = R6[5 * nn + r], Mpt = R6[4 * nn + rt], Mnt = R6[5 * nn + rt];
+  const cplx ep = ph[2 * ang], en = ph[2 * ang + 1];
+  const double* R = R6 + st * nn;
+  const long long ks = (long long)S * nn;
+  const double Kp = R[r], Kn = R[ks + r], Lp = R[2 * ks + r], Ln = R[3 * ks + r];
+  const double Mp = R[4 * ks + r], Mn = R[5 * ks + r], Mpt = R[4 * ks + rt], Mnt = R[5 * ks + rt];
   cplx u, v;
   // e_p Kp + e_n Kn + conj(e_p) Lp + conj(e_n) Ln
   u.x = ep.x * (Kp + Lp) + en.x * (Kn + Ln);
@@ -104,10 +111,10 @@ __global__ void build_uv_phi_kernel(int n, int na, const double* __restrict__ R6
 }
 
 // Ds(i,j) = sgn(i) D(i,j)
-__global__ void row_sign_kernel(int n, const double* __restrict__ sgn, const double* __restrict__ D,
-                                double* __restrict__ Ds) {
+__global__ void row_sign_kernel(long long tot, int n, const double* __restrict__ sgn,
+                                const double* __restrict__ D, double* __restrict__ Ds) {
   long long idx = (long long)blockIdx.x * NT + threadIdx.x;
-  if (idx >= (long long)n * n) return;
+  if (idx >= tot) return;
   Ds[idx] = sgn[idx % n] * D[idx];
 }
 
@@ -116,12 +123,14 @@ __global__ void conj_kernel(long long tot, cplx* __restrict__ X) {
   if (idx < tot) X[idx].y = -X[idx].y;
 }
 
-// dst_a = complex(src) for every angle
-__global__ void bcast_real_kernel(long long nn, int na, const double* __restrict__ src,
-                                  cplx* __restrict__ dst) {
+// dst_a = complex(src of the item's state) for every item (state (b0 + a) / A,
+// state s at src + s * nn)
+__global__ void bcast_real_kernel(long long nn, int na, long long b0, int A,
+                                  const double* __restrict__ src, cplx* __restrict__ dst) {
   long long idx = (long long)blockIdx.x * NT + threadIdx.x;
   if (idx >= nn * na) return;
-  dst[idx] = cmk(src[idx % nn], 0.0);
+  const long long st = (b0 + idx / nn) / A;
+  dst[idx] = cmk(src[st * nn + idx % nn], 0.0);
 }
 
 // h_a = Gamma_a + H1
@@ -207,18 +216,26 @@ traces_kernel(int n, const cplx* __restrict__ H1, const cplx* __restrict__ rho,
 }
 
 // Q = T (e_p Pp + e_n Pn)[nemp:, nemp:] T and JQ = J Q (reduced overlap).
-// JQ row 2j = s_j Q row 2j+1, row 2j+1 = -s_j Q row 2j.
-__global__ void build_q_kernel(int n, int nemp, int nb, int na, const double* __restrict__ Pp,
-                               const double* __restrict__ Pn, const double* __restrict__ Tsc,
-                               const double* __restrict__ sgn, const cplx* __restrict__ ph,
+// JQ row 2j = s_j Q row 2j+1, row 2j+1 = -s_j Q row 2j.  Item b0 + a is angle
+// (b0 + a) % A of state (b0 + a) / A: Pp, Pn of state s at + s * n^2, its
+// scaling Tsc and signs sgn at + s * os.
+__global__ void build_q_kernel(int n, int nemp, int nb, int na, long long b0, int A,
+                               const double* __restrict__ Pp0, const double* __restrict__ Pn0,
+                               const double* __restrict__ Tsc0, const double* __restrict__ sgn0,
+                               long long os, const cplx* __restrict__ ph,
                                cplx* __restrict__ Q, cplx* __restrict__ JQ) {
   const long long mm = (long long)nb * nb;
   long long idx = (long long)blockIdx.x * NT + threadIdx.x;
   if (idx >= mm * na) return;
   const long long a = idx / mm;
+  const long long st = (b0 + a) / A, ang = (b0 + a) % A;
+  const double* Pp = Pp0 + st * n * n;
+  const double* Pn = Pn0 + st * n * n;
+  const double* Tsc = Tsc0 + st * os;
+  const double* sgn = sgn0 + st * os;
   const int r = (int)(idx % mm);
   const int i = r % nb, j = r / nb;
-  const cplx ep = ph[2 * a], en = ph[2 * a + 1];
+  const cplx ep = ph[2 * ang], en = ph[2 * ang + 1];
   auto q = [&](int ii) {
     const long long o = (ii + nemp) + (long long)(j + nemp) * n;
     const double pp = Pp[o], pn = Pn[o];
@@ -237,12 +254,14 @@ __global__ void build_q_kernel(int n, int nemp, int nb, int na, const double* __
   }
 }
 
-// K += J (the sgn-weighted symplectic unit)
-__global__ void add_j_kernel(int nb, int na, const double* __restrict__ sgn, cplx* __restrict__ K) {
+// K += J (the sgn-weighted symplectic unit; signs of state s at sgn0 + s * os)
+__global__ void add_j_kernel(int nb, int na, long long b0, int A, const double* __restrict__ sgn0,
+                             long long os, cplx* __restrict__ K) {
   long long idx = (long long)blockIdx.x * NT + threadIdx.x;
   const long long pairs = (long long)(nb / 2) * na;
   if (idx >= pairs) return;
   const long long a = idx / (nb / 2);
+  const double* sgn = sgn0 + ((b0 + a) / A) * os;
   const int p = (int)(idx % (nb / 2));
   const int i = 2 * p;
   const long long mm = (long long)nb * nb;
@@ -252,28 +271,34 @@ __global__ void add_j_kernel(int nb, int na, const double* __restrict__ sgn, cpl
 }
 
 // Rn2 = (e_p Pp + e_n Pn)[nemp:, nemp:] for the direct overlap
-__global__ void build_rn2_kernel(int n, int nemp, int n2, int na, const double* __restrict__ Pp,
-                                 const double* __restrict__ Pn, const cplx* __restrict__ ph,
-                                 cplx* __restrict__ R2) {
+__global__ void build_rn2_kernel(int n, int nemp, int n2, int na, long long b0, int A,
+                                 const double* __restrict__ Pp, const double* __restrict__ Pn,
+                                 const cplx* __restrict__ ph, cplx* __restrict__ R2) {
   const long long mm = (long long)n2 * n2;
   long long idx = (long long)blockIdx.x * NT + threadIdx.x;
   if (idx >= mm * na) return;
   const long long a = idx / mm;
+  const long long st = (b0 + a) / A, ang = (b0 + a) % A;
   const int r = (int)(idx % mm);
   const int i = r % n2, j = r / n2;
-  const cplx ep = ph[2 * a], en = ph[2 * a + 1];
-  const long long o = (i + nemp) + (long long)(j + nemp) * n;
+  const cplx ep = ph[2 * ang], en = ph[2 * ang + 1];
+  const long long o = (i + nemp) + (long long)(j + nemp) * n + st * n * n;
   R2[idx] = cmk(ep.x * Pp[o] + en.x * Pn[o], ep.y * Pp[o] + en.y * Pn[o]);
 }
 
 // M_reg of calculate_overlap (:891-918) from v/u and Rinv (n2 x n2)
-__global__ void build_mreg_kernel(int n2, int nb0, int nb1, int nreg, int na,
-                                  const double* __restrict__ vou0, const double* __restrict__ vou1,
-                                  const cplx* __restrict__ Rinv, cplx* __restrict__ M) {
+// (v/u of the item's state: vou + ((b0 + a) / A) * vs)
+__global__ void build_mreg_kernel(int n2, int nb0, int nb1, int nreg, int na, long long b0, int A,
+                                  const double* __restrict__ vou0_, const double* __restrict__ vou1_,
+                                  long long vs, const cplx* __restrict__ Rinv,
+                                  cplx* __restrict__ M) {
   const long long mm = (long long)nreg * nreg;
   long long idx = (long long)blockIdx.x * NT + threadIdx.x;
   if (idx >= mm * na) return;
   const long long a = idx / mm;
+  const long long so = ((b0 + a) / A) * vs;
+  const double* vou0 = vou0_ + so;
+  const double* vou1 = vou1_ + so;
   const int r = (int)(idx % mm);
   const int i = r % nreg, j = r / nreg;   // 0-based row i, column j
   const cplx* Ri = Rinv + a * (long long)n2 * n2;
@@ -296,12 +321,20 @@ __global__ void build_mreg_kernel(int n2, int nb0, int nb1, int nreg, int na,
 }
 
 // sum_gauge scalars (:425-458): f1 = w*o, pnp += f1*{1, E, Z, N, Z2, N2}
-// per-angle in: over[a], scal[a*9+k]; writes f1/f2 per angle and the 6 sums
+// per-angle in: over[a], scal[a*9+k]; writes f1/f2 per angle and the 6 sums.
+// One workgroup per state (blockIdx.x): items s*na .. s*na + na - 1, sums at pnp + 8 s.
 __global__ void __launch_bounds__(NT)
-gauge_scalars_kernel(int na, int n, cplx H0, const cplx* __restrict__ w, const cplx* __restrict__ over,
-                     const cplx* __restrict__ sc, const cplx* __restrict__ pari,
-                     const double* __restrict__ mult, cplx* __restrict__ f12,
-                     cplx* __restrict__ ener, cplx* __restrict__ pnp) {
+gauge_scalars_kernel(int na, int n, cplx H0, const cplx* __restrict__ w, const cplx* __restrict__ over0,
+                     const cplx* __restrict__ sc0, const cplx* __restrict__ pari0,
+                     const double* __restrict__ mult, cplx* __restrict__ f120,
+                     cplx* __restrict__ ener0, cplx* __restrict__ pnp0) {
+  const long long st = blockIdx.x;
+  const cplx* over = over0 + st * na;
+  const cplx* sc = sc0 + st * na * 9;
+  const cplx* pari = pari0 ? pari0 + st * na : nullptr;
+  cplx* f12 = f120 + st * 2 * na;
+  cplx* ener = ener0 + st * na;
+  cplx* pnp = pnp0 + st * 8;
   __shared__ cplx red[7][NT];
   cplx acc[7];
   for (int k = 0; k < 7; ++k) acc[k] = cmk(0.0, 0.0);
@@ -338,13 +371,21 @@ gauge_scalars_kernel(int na, int n, cplx H0, const cplx* __restrict__ w, const c
 }
 
 // pnp_A = sum f1 A, pnp_H20 = sum f1 H20, pnp_AH20 = sum f2 A   (angles in order)
-__global__ void gauge_matrices_kernel(long long nn, int na, const cplx* __restrict__ f12,
+// state blockIdx.y: items s*na .., sums at pA/pH/pAH + s * os
+__global__ void gauge_matrices_kernel(long long nn, int na, const cplx* __restrict__ f120,
                                       const double* __restrict__ mult,
-                                      const cplx* __restrict__ Aphi, const cplx* __restrict__ H20,
-                                      long long stride, cplx* __restrict__ pA,
-                                      cplx* __restrict__ pH, cplx* __restrict__ pAH) {
+                                      const cplx* __restrict__ Aphi0, const cplx* __restrict__ H200,
+                                      long long stride, cplx* __restrict__ pA0,
+                                      cplx* __restrict__ pH0, cplx* __restrict__ pAH0, long long os) {
   long long idx = (long long)blockIdx.x * NT + threadIdx.x;
   if (idx >= nn) return;
+  const long long st = blockIdx.y;
+  const cplx* f12 = f120 + st * 2 * na;
+  const cplx* Aphi = Aphi0 + st * na * stride;
+  const cplx* H20 = H200 + st * na * stride;
+  cplx* pA = pA0 + st * os;
+  cplx* pH = pH0 + st * os;
+  cplx* pAH = pAH0 + st * os;
   cplx sa = cmk(0, 0), sh = cmk(0, 0), sah = cmk(0, 0);
   for (int a = 0; a < na; ++a) {
     const cplx f1 = f12[2 * a], f2 = f12[2 * a + 1];
@@ -367,9 +408,18 @@ __global__ void gauge_matrices_kernel(long long nn, int na, const cplx* __restri
 
 // calculate_projected_H20 (:729-759); imag[block] = max Im over the block
 __global__ void __launch_bounds__(NT)
-projected_h20_kernel(int n, const cplx* __restrict__ pnp, const cplx* __restrict__ pA,
-                     const cplx* __restrict__ pH, const cplx* __restrict__ pAH,
-                     double* __restrict__ H20v, double* __restrict__ imag) {
+projected_h20_kernel(int n, const cplx* __restrict__ pnp0, const cplx* __restrict__ pA0,
+                     const cplx* __restrict__ pH0, const cplx* __restrict__ pAH0,
+                     double* __restrict__ H20v0, double* __restrict__ imag0, long long os,
+                     long long is) {
+  // state blockIdx.y: sums at pnp + 8 s, matrices at + s * os (complex), imag at + s * is
+  const long long st = blockIdx.y;
+  const cplx* pnp = pnp0 + 8 * st;
+  const cplx* pA = pA0 + st * os;
+  const cplx* pH = pH0 + st * os;
+  const cplx* pAH = pAH0 + st * os;
+  double* H20v = reinterpret_cast<double*>(reinterpret_cast<cplx*>(H20v0) + st * os);
+  double* imag = imag0 + st * is;
   const long long nn = (long long)n * n;
   long long idx = (long long)blockIdx.x * NT + threadIdx.x;
   const cplx over = pnp[0];
@@ -397,15 +447,16 @@ projected_h20_kernel(int n, const cplx* __restrict__ pnp, const cplx* __restrict
   if (threadIdx.x == 0) imag[blockIdx.x] = red[0];
 }
 
-// overlap assembly: o = pre * phase * exp(logabs + lnorm)
-__global__ void overlap_finish_kernel(int na, const double* __restrict__ logabs,
+// overlap assembly: o = pre * phase * exp(logabs + lnorm) (lnorm of the item's
+// state at lnorm0 + ((b0 + a) / A) * ls)
+__global__ void overlap_finish_kernel(int na, long long b0, int A, const double* __restrict__ logabs,
                                       const cplx* __restrict__ phase, const int* __restrict__ zero,
-                                      const cplx* __restrict__ pre, const double* __restrict__ lnorm,
-                                      cplx* __restrict__ over) {
+                                      const cplx* __restrict__ pre, const double* __restrict__ lnorm0,
+                                      long long ls, cplx* __restrict__ over) {
   int a = blockIdx.x * NT + threadIdx.x;
   if (a >= na) return;
   if (zero[a]) { over[a] = cmk(0.0, 0.0); return; }
-  const double m = exp(logabs[a] + lnorm[0]);
+  const double m = exp(logabs[a] + lnorm0[((b0 + a) / A) * ls]);
   const cplx z = cmul(pre[a], phase[a]);
   over[a] = cmk(m * z.x, m * z.y);
 }
@@ -415,9 +466,11 @@ __global__ void fill_cplx_kernel(long long na, cplx v, cplx* __restrict__ x) {
   if (a < na) x[a] = v;
 }
 
-__global__ void fill_from_kernel(int na, const cplx* __restrict__ v, cplx* __restrict__ x) {
+// x[a] = the value of the item's state (v0 + ((b0 + a) / A) * vs)
+__global__ void fill_from_kernel(int na, long long b0, int A, const cplx* __restrict__ v0,
+                                 long long vs, cplx* __restrict__ x) {
   int a = blockIdx.x * NT + threadIdx.x;
-  if (a < na) x[a] = v[0];
+  if (a < na) x[a] = v0[((b0 + a) / A) * vs];
 }
 
 // Pfaffian of an empty block: log|pf| = 0, phase 1
@@ -483,9 +536,19 @@ namespace {
 // calculate_gradient with fixed eta, mu (gradient_type 0, module_gradient.f90:
 // 122-141): Z = -eta (H20v - sum_j lambda_j C20_j) + mu Zm1; part[b] = sum Z^2.
 __global__ void __launch_bounds__(NT)
-gradient_kernel(long long nn, double eta, double mu, int ncons, const double* __restrict__ lam,
-                const double* __restrict__ c20, const double* __restrict__ H20v,
-                const double* __restrict__ Zm1, double* __restrict__ Z, double* __restrict__ part) {
+gradient_kernel(long long nn, double eta, double mu, int ncons, const double* __restrict__ lam0,
+                long long ls, const double* __restrict__ c200, long long cs,
+                const double* __restrict__ H20v0, long long hs, const double* __restrict__ Zm10,
+                double* __restrict__ Z0, double* __restrict__ part0, long long ps) {
+  // state blockIdx.y: lambda at + s * ls, constraint_20 at + s * cs, field_H20v at
+  // + s * hs, Z / Zm1 at + s * nn, partial sums at + s * ps
+  const long long st = blockIdx.y;
+  const double* lam = lam0 + st * ls;
+  const double* c20 = c200 + st * cs;
+  const double* H20v = H20v0 + st * hs;
+  const double* Zm1 = Zm10 + st * nn;
+  double* Z = Z0 + st * nn;
+  double* part = part0 + st * ps;
   long long idx = (long long)blockIdx.x * NT + threadIdx.x;
   double z2 = 0.0;
   if (idx < nn) {
@@ -507,24 +570,53 @@ gradient_kernel(long long nn, double eta, double mu, int ncons, const double* __
 
 int sign_pow(long long e) { return (e % 2 == 0) ? 1 : -1; }
 
-// per-angle complex buffers of the loop body
+// per-item complex buffers of the loop body
 enum { W_UPHI, W_VPHI, W_UT, W_VT, W_RHO, W_KLR, W_KRL, W_G, W_H, W_DLR, W_DRL, W_T, W_X, W_Y,
        W_Z, W_COUNT };
 
-// Device pointers and host scalars of one projection of the resident state
-// (pure host arithmetic on the context; no HIP calls).
+// Overlap shape of one state (calculate_overlap with state 0 = state 1,
+// :803-818): sizes, signfac and the prefactor of the reduced Pfaffian form.
+struct OvlShape {
+  int nocc = 0, nemp = 0, nbloc = 0, n0 = 0, n2 = 0, signfac = 1;
+  bool reduced = false;
+  double pre = 1.0;
+};
+
+OvlShape ovl_shape(int n, const StateInfo& si) {
+  OvlShape o;
+  o.nocc = si.nocc;
+  o.nemp = si.nemp;
+  o.nbloc = n - o.nocc - o.nemp;
+  o.reduced = o.nocc == 0 && o.nbloc > 0 && (o.nbloc % 2 == 0);
+  o.n0 = n - o.nocc;
+  o.n2 = n - o.nemp;
+  const long long e1 = (long long)o.n2 * (o.n2 + 1) / 2;
+  const long long e2 = (long long)o.nocc * (o.nocc - 1) / 2;
+  const long long e3 = (long long)o.nocc * (o.nocc + 1) + (long long)o.nocc * (o.n0 + o.n2) +
+                       (long long)o.nocc * o.n0;
+  o.signfac = sign_pow(e1) * sign_pow(e2) * sign_pow(e3);
+  double prod_sgn = 1.0;
+  if (o.reduced)
+    for (int q = 0; q < o.nbloc / 2; ++q)
+      if (si.voveru[o.nemp + 2 * q] < 0) prod_sgn = -prod_sgn;
+  o.pre = o.signfac * prod_sgn * sign_pow(o.nbloc / 2);
+  return o;
+}
+
+// Device pointers and host scalars of one projection of the resident states
+// (pure host arithmetic on the context; no HIP calls).  The batch holds
+// B = S * A items, item b being angle b % A of state b / A.
 struct RunPlan {
-  int n, h, A, chunk;
-  long long nn;
-  bool proj, need_ovl, reduced;
-  int nocc, nemp, nbloc, n0, n2, signfac;
-  double pre;                               // reduced-overlap prefactor
+  int n, h, A, S, B, chunk;
+  long long nn, os, outs, is;               // strides: overlap set-up, outputs, imag
+  bool proj, need_ovl, uniform;
+  std::vector<OvlShape> sh;
   cplx *work, *Akeep, *Hkeep;
   cplx *d_ph, *d_tr, *d_over, *d_ener, *d_f12, *d_pnp, *d_pre, *d_det, *d_phase, *d_w, *d_pari;
   double *d_logabs, *d_imag, *d_mult;
   int *d_info, *d_zero, *d_perm;
-  double *d_T, *d_sg, *d_ovac, *d_vou;      // overlap set-up uploaded with the state
-  cplx* d_pre0;
+  double *d_T, *d_sg, *d_ovac, *d_vou;      // overlap set-up of state 0 (state s at + s * os)
+  cplx* d_pre0;                             // state s at + s * os / 2
   size_t work_bytes, scal_bytes, int_bytes, ovl_bytes, pfw_bytes;
 };
 
@@ -533,71 +625,69 @@ RunPlan make_plan(tg_ctx* c) {
   P.n = c->n;
   P.h = P.n / 2;
   P.A = c->A;
+  P.S = c->ns;
+  P.B = P.S * P.A;
   P.nn = (long long)P.n * P.n;
-  P.chunk = c->plain ? 1 : P.A;
+  P.chunk = c->plain ? 1 : P.B;
+  P.os = 3 * (long long)P.n + 8;
+  P.outs = tg_out_stride(P.n);
+  P.is = tg_part_len(P.n);
   P.proj = std::max(c->Mphip, c->Mphin) > 1;
   P.need_ovl = P.proj || c->iprint;
-  P.nocc = c->nocc;
-  P.nemp = c->nemp;
-  P.nbloc = P.n - P.nocc - P.nemp;
-  P.reduced = P.nocc == 0 && P.nbloc > 0 && (P.nbloc % 2 == 0);
-  // signfac of :811-814 with state 0 = state 1
-  P.n0 = P.n - P.nocc;
-  P.n2 = P.n - P.nemp;
-  const long long e1 = (long long)P.n2 * (P.n2 + 1) / 2;
-  const long long e2 = (long long)P.nocc * (P.nocc - 1) / 2;
-  const long long e3 = (long long)P.nocc * (P.nocc + 1) + (long long)P.nocc * (P.n0 + P.n2) +
-                       (long long)P.nocc * P.n0;
-  P.signfac = sign_pow(e1) * sign_pow(e2) * sign_pow(e3);
-  double prod_sgn = 1.0;
-  if (P.reduced)
-    for (int q = 0; q < P.nbloc / 2; ++q)
-      if (c->voveru[P.nemp + 2 * q] < 0) prod_sgn = -prod_sgn;
-  P.pre = P.signfac * prod_sgn * sign_pow(P.nbloc / 2);
-  const int A = P.A, n = P.n;
+  P.uniform = true;
+  for (int s = 0; s < P.S; ++s) {
+    P.sh.push_back(ovl_shape(P.n, c->st[s]));
+    if (P.sh[s].nocc != P.sh[0].nocc || P.sh[s].nemp != P.sh[0].nemp) P.uniform = false;
+  }
+  const int A = P.A, B = P.B, S = P.S, n = P.n;
   const long long nn = P.nn;
   const size_t wsz = (size_t)W_COUNT * P.chunk * nn;
-  P.work_bytes = sizeof(cplx) * (wsz + 2 * (size_t)A * nn);
-  // scalars: ph (2A) | traces (9A) | over (A) | ener (A) | f12 (2A) | pnp (8) | pre (A) | det (A)
-  //          | phase (A) | w (A) | pari (A) | logabs (A doubles) | imag (doubles)
-  P.scal_bytes = sizeof(cplx) * (20 * (size_t)A + 8) + sizeof(double) * A +
-                 sizeof(double) * (nn / NT + 2) + sizeof(double) * A;
-  P.int_bytes = sizeof(int) * ((size_t)A * (2 * n + 4) + 8);
-  // overlap set-up: T (n) | sg (n) | ovac (2) | pre0 (cplx) | v/u (n)
-  P.ovl_bytes = sizeof(double) * (3 * (size_t)n + 8);
+  P.work_bytes = sizeof(cplx) * (wsz + 2 * (size_t)B * nn);
+  // scalars: ph (2A) | traces (9B) | over (B) | ener (B) | f12 (2B) | pnp (8S) | pre (B)
+  //          | det (B) | phase (B) | w (A) | pari (B) | logabs (B doubles)
+  //          | imag (S * is doubles) | mult (A doubles)
+  P.scal_bytes = sizeof(cplx) * (3 * (size_t)A + 17 * (size_t)B + 8 * (size_t)S) +
+                 sizeof(double) * ((size_t)B + (size_t)S * P.is + A);
+  P.int_bytes = sizeof(int) * ((size_t)B * (n + 2) + 8);
+  // overlap set-up per state: T (n) | sg (n) | ovac (2) | pre0 (cplx) | v/u (n)
+  P.ovl_bytes = sizeof(double) * (size_t)S * P.os;
   // overlap work (its own buffers: the overlap runs on the second stream,
-  // concurrently with the Thouless / field / H20 chain)
-  //   reduced: Q | JQ | K | Pfaffian work, chunk * nb^2 complex each
-  //   direct:  R2 | M_reg | Pfaffian work | det (chunk) | info (chunk int) | perm (chunk * n int)
+  // concurrently with the Thouless / field / H20 chain), per item of a segment
+  //   reduced: Q | JQ | K | Pfaffian work, nb^2 complex each
+  //   direct:  R2 | M_reg | Pfaffian work | det | info (int) | perm (n int)
   P.pfw_bytes = 0;
-  if (P.need_ovl && P.reduced) {
-    P.pfw_bytes = sizeof(cplx) * 4 * (size_t)P.chunk * P.nbloc * P.nbloc;
-  } else if (P.need_ovl) {
-    const int nreg = 2 * P.nbloc;
-    const long long m2 = (long long)P.n2 * P.n2, mr = (long long)nreg * nreg;
-    P.pfw_bytes = sizeof(cplx) * ((size_t)P.chunk * (m2 + 2 * mr) + P.chunk) +
-                  sizeof(int) * ((size_t)P.chunk * (P.n + 1));
-  }
+  if (P.need_ovl)
+    for (const OvlShape& o : P.sh) {
+      size_t per;
+      if (o.reduced) {
+        per = sizeof(cplx) * 4 * (size_t)o.nbloc * o.nbloc;
+      } else {
+        const long long nreg = 2 * (long long)o.nbloc;
+        per = sizeof(cplx) * ((size_t)o.n2 * o.n2 + 2 * (size_t)(nreg * nreg) + 1) +
+              sizeof(int) * (size_t)(n + 1);
+      }
+      P.pfw_bytes = std::max(P.pfw_bytes, per * (size_t)P.chunk + 256);
+    }
   P.work = c->d_work.as<cplx>();
   P.Akeep = P.work + wsz;
-  P.Hkeep = P.Akeep + (size_t)A * nn;
+  P.Hkeep = P.Akeep + (size_t)B * nn;
   P.d_ph = c->d_scal.as<cplx>();
   P.d_tr = P.d_ph + 2 * A;
-  P.d_over = P.d_tr + 9 * A;
-  P.d_ener = P.d_over + A;
-  P.d_f12 = P.d_ener + A;
-  P.d_pnp = P.d_f12 + 2 * A;
-  P.d_pre = P.d_pnp + 8;
-  P.d_det = P.d_pre + A;
-  P.d_phase = P.d_det + A;
-  P.d_w = P.d_phase + A;
+  P.d_over = P.d_tr + 9 * (size_t)B;
+  P.d_ener = P.d_over + B;
+  P.d_f12 = P.d_ener + B;
+  P.d_pnp = P.d_f12 + 2 * (size_t)B;
+  P.d_pre = P.d_pnp + 8 * (size_t)S;
+  P.d_det = P.d_pre + B;
+  P.d_phase = P.d_det + B;
+  P.d_w = P.d_phase + B;
   P.d_pari = P.d_w + A;
-  P.d_logabs = reinterpret_cast<double*>(P.d_pari + A);
-  P.d_imag = P.d_logabs + A;
-  P.d_mult = P.d_imag + (nn / NT + 2);
+  P.d_logabs = reinterpret_cast<double*>(P.d_pari + B);
+  P.d_imag = P.d_logabs + B;
+  P.d_mult = P.d_imag + (size_t)S * P.is;
   P.d_info = c->d_int.as<int>();
-  P.d_zero = P.d_info + A;
-  P.d_perm = P.d_zero + A;
+  P.d_zero = P.d_info + B;
+  P.d_perm = P.d_zero + B;
   P.d_T = c->d_ovl.as<double>();
   P.d_sg = P.d_T + n;
   P.d_ovac = P.d_sg + n;
@@ -607,19 +697,20 @@ RunPlan make_plan(tg_ctx* c) {
 }
 
 // Allocations, uploads of the state-dependent set-up (angles, weights, overlap
-// scaling, ln-norm and prefactor) and, on the first call at a size, the GEMM
-// tile autotuning.  Runs eagerly, never under stream capture.
+// scaling, ln-norm and prefactor of every state) and, on the first call at a
+// size, the GEMM tile autotuning.  Runs eagerly, never under stream capture.
 int project_upload(tg_ctx* c) {
   RunPlan P = make_plan(c);
-  const int n = P.n, A = P.A;
+  const int n = P.n, A = P.A, S = P.S;
   const long long nn = P.nn;
   hipStream_t s = c->stream;
   TG_TRY(c->d_work.ensure(P.work_bytes));
   TG_TRY(c->d_scal.ensure(P.scal_bytes));
   TG_TRY(c->d_int.ensure(P.int_bytes));
   TG_TRY(c->d_ovl.ensure(P.ovl_bytes));
-  TG_TRY(c->d_out.ensure(sizeof(cplx) * 3 * nn + sizeof(double) * nn));
+  TG_TRY(c->d_out.ensure(sizeof(cplx) * (size_t)S * P.outs));
   if (P.pfw_bytes) TG_TRY(c->d_pfw.ensure(P.pfw_bytes));
+  if (c->grad_on) TG_TRY(tg_ensure_z(c));
   P = make_plan(c);   // pointers of the (re)allocated buffers
 
   std::vector<cplx> hph(2 * A), hw(A);
@@ -628,18 +719,22 @@ int project_upload(tg_ctx* c) {
     hph[2 * a + 1] = cmk(cos(c->phin[a]), sin(c->phin[a]));
     hw[a] = c->weight[a];
   }
-  std::vector<double> ovl(3 * (size_t)n + 8, 0.0);
-  double* Tsc = ovl.data();
-  double* sg = Tsc + n;
-  if (P.reduced)
-    for (int q = 0; q < P.nbloc / 2; ++q) {
-      const double x = c->voveru[P.nemp + 2 * q];
-      sg[q] = (x < 0) ? -1.0 : 1.0;
-      Tsc[2 * q] = Tsc[2 * q + 1] = sqrt(fabs(x));
-    }
-  sg[n] = (P.nbloc == 0) ? 0.0 : c->ovac;                 // d_ovac
-  sg[n + 2] = P.pre;                                      // d_pre0 (re, im = 0)
-  for (int i = 0; i < P.nbloc; ++i) sg[n + 4 + i] = c->voveru[P.nemp + i];   // d_vou
+  std::vector<double> ovl((size_t)S * P.os, 0.0);
+  for (int st = 0; st < S; ++st) {
+    const OvlShape& o = P.sh[st];
+    const StateInfo& si = c->st[st];
+    double* Tsc = ovl.data() + (size_t)st * P.os;
+    double* sg = Tsc + n;
+    if (o.reduced)
+      for (int q = 0; q < o.nbloc / 2; ++q) {
+        const double x = si.voveru[o.nemp + 2 * q];
+        sg[q] = (x < 0) ? -1.0 : 1.0;
+        Tsc[2 * q] = Tsc[2 * q + 1] = sqrt(fabs(x));
+      }
+    sg[n] = (o.nbloc == 0) ? 0.0 : si.ovac;                             // d_ovac
+    sg[n + 2] = o.pre;                                                  // d_pre0 (re, im = 0)
+    for (int i = 0; i < o.nbloc; ++i) sg[n + 4 + i] = si.voveru[o.nemp + i];   // d_vou
+  }
   TG_HIP(hipMemcpyAsync(P.d_ph, hph.data(), sizeof(cplx) * 2 * A, hipMemcpyHostToDevice, s));
   TG_HIP(hipMemcpyAsync(P.d_w, hw.data(), sizeof(cplx) * A, hipMemcpyHostToDevice, s));
   TG_HIP(hipMemcpyAsync(P.d_mult, c->mult.data(), sizeof(double) * A, hipMemcpyHostToDevice, s));
@@ -649,7 +744,8 @@ int project_upload(tg_ctx* c) {
   // first iteration at this size: autotune the tile shapes of the batched GEMMs
   // of the loop body on zeroed scratch operands (W_T, W_X -> W_Y)
   const int chunk = P.chunk;
-  if (c->tuned_n != n || c->tuned_chunk != chunk || c->tuned_nb != P.nbloc) {
+  const int nb0 = (P.uniform && P.sh[0].reduced && P.need_ovl) ? P.sh[0].nbloc : -1;
+  if (c->tuned_n != n || c->tuned_chunk != chunk || c->tuned_nb != nb0) {
     auto Wb = [&](int k) { return P.work + (size_t)k * chunk * nn; };
     const double* U = c->d_U.as<double>();
     cplx* Ta = Wb(W_T);
@@ -664,14 +760,14 @@ int project_upload(tg_ctx* c) {
     TG_TRY(gemm_tune_dz(s, OP_N, OP_N, n, n, n, U, n, 0, Xa, n, nn, Ya, n, nn, chunk));
     TG_TRY(gemm_tune_dz(s, OP_N, OP_T, n, n, n, U, n, 0, Xa, n, nn, Ya, n, nn, chunk));
     TG_TRY(gemm_tune_zd(s, OP_N, OP_T, n, n, n, Xa, n, nn, U, n, 0, Ya, n, nn, chunk));
-    if (P.reduced && P.need_ovl) {
-      const int nb = P.nbloc;
+    if (nb0 > 0) {
+      const int nb = nb0;
       const long long mm = (long long)nb * nb;
       TG_TRY(gemm_tune_zz(s, OP_T, OP_N, nb, nb, nb, Xa, nb, mm, Ta, nb, mm, Ya, nb, mm, chunk));
     }
     c->tuned_n = n;
     c->tuned_chunk = chunk;
-    c->tuned_nb = P.nbloc;
+    c->tuned_nb = nb0;
   }
   c->state_dirty = 0;
   return TG_OK;
@@ -681,7 +777,7 @@ int project_upload(tg_ctx* c) {
 // Buffers and the uploaded set-up come from project_upload.
 int project_run(tg_ctx* c) {
   const RunPlan P = make_plan(c);
-  const int n = P.n, h = P.h, A = P.A;
+  const int n = P.n, h = P.h, A = P.A, S = P.S, B = P.B;
   const long long nn = P.nn;
   hipStream_t s = c->stream;
   const int chunk = P.chunk;
@@ -689,68 +785,51 @@ int project_run(tg_ctx* c) {
   const double* U = c->d_U.as<double>();
   const double* V = c->d_V.as<double>();
   const double* D = c->d_D.as<double>();
-  double* R6 = c->d_real6.as<double>();      // Kp Kn Lp Ln Mp Mn Pp Pn
+  double* R6 = c->d_real6.as<double>();      // Kp Kn Lp Ln Mp Mn Pp Pn [parity Pp Pn] [Ds]
+  const long long ks = (long long)S * nn;    // stride of one product over the S states
   const cplx one = cmk(1, 0), zero = cmk(0, 0);
+  if (chunk != B && S != 1) {
+    tg_set_error("projection: angle chunks need a single state");
+    return TG_E_ARG;
+  }
 
-  // ---- angle-independent real products (once per iteration)
+  // ---- angle-independent real products (once per iteration, every state)
   {
     ScopedTiming tm(c, "gemm");
     for (int sp = 0; sp < 2; ++sp) {
       const int r0 = sp * h;
-      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, U + r0, n, 0, U + r0, n, 0, 0.0, R6 + (0 + sp) * nn, n, 0, 1));
-      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, V + r0, n, 0, V + r0, n, 0, 0.0, R6 + (2 + sp) * nn, n, 0, 1));
-      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, V + r0, n, 0, U + r0, n, 0, 0.0, R6 + (4 + sp) * nn, n, 0, 1));
-      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, D + r0, n, 0, D + r0, n, 0, 0.0, R6 + (6 + sp) * nn, n, 0, 1));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, U + r0, n, nn, U + r0, n, nn, 0.0, R6 + (0 + sp) * ks, n, nn, S));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, V + r0, n, nn, V + r0, n, nn, 0.0, R6 + (2 + sp) * ks, n, nn, S));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, V + r0, n, nn, U + r0, n, nn, 0.0, R6 + (4 + sp) * ks, n, nn, S));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, D + r0, n, nn, D + r0, n, nn, 0.0, R6 + (6 + sp) * ks, n, nn, S));
     }
     if (c->iprint) {
       // D0^T diag((-1)^l Pi_p) D0 and D0^T diag((-1)^l Pi_n) D0 (generate_rotation_parity :607-625)
-      double* Ds = R6 + 10 * nn;
-      hipLaunchKernelGGL(row_sign_kernel, dim3(nblk(nn)), dim3(NT), 0, s, n, c->d_psign.as<double>(),
-                         D, Ds);
+      double* Ds = R6 + 10 * ks;
+      hipLaunchKernelGGL(row_sign_kernel, dim3(nblk(ks)), dim3(NT), 0, s, ks, n,
+                         c->d_psign.as<double>(), D, Ds);
       TG_HIP(hipGetLastError());
       for (int sp = 0; sp < 2; ++sp) {
         const int r0 = sp * h;
-        TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, D + r0, n, 0, Ds + r0, n, 0, 0.0, R6 + (8 + sp) * nn, n, 0, 1));
+        TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, D + r0, n, nn, Ds + r0, n, nn, 0.0, R6 + (8 + sp) * ks, n, nn, S));
       }
     }
   }
-  const double* Pp = R6 + 6 * nn;
-  const double* Pn = R6 + 7 * nn;
+  const double* Pp = R6 + 6 * ks;
+  const double* Pn = R6 + 7 * ks;
   // the overlaps depend only on R6 and the set-up: fork them to the second stream
   hipStream_t s2 = c->stream2;
   TG_HIP(hipEventRecord(c->ev_fork, s));
   TG_HIP(hipStreamWaitEvent(s2, c->ev_fork, 0));
 
   auto Wb = [&](int k) { return P.work + (size_t)k * chunk * nn; };
-  cplx* Akeep = P.Akeep;                     // A_phi of every angle
-  cplx* Hkeep = P.Hkeep;                     // H20_phi of every angle
-  cplx* d_ph = P.d_ph;
-  cplx* d_tr = P.d_tr;
-  cplx* d_over = P.d_over;
-  cplx* d_ener = P.d_ener;
-  cplx* d_f12 = P.d_f12;
-  cplx* d_pnp = P.d_pnp;
-  cplx* d_pre = P.d_pre;
-  cplx* d_det = P.d_det;
-  cplx* d_phase = P.d_phase;
-  cplx* d_w = P.d_w;
-  cplx* d_pari = P.d_pari;
-  double* d_logabs = P.d_logabs;
-  double* d_imag = P.d_imag;
-  int* d_info = P.d_info;
-  int* d_zero = P.d_zero;
-  int* d_perm = P.d_perm;
-  const int nemp = P.nemp;
-  const int nbloc = P.nbloc;
-  const bool reduced = P.reduced;
-  const int n2 = P.n2;
-  const int signfac = P.signfac;
-  double* d_T = P.d_T;
-  double* d_sg = P.d_sg;
+  cplx* Akeep = P.Akeep;                     // A_phi of every item
+  cplx* Hkeep = P.Hkeep;                     // H20_phi of every item
+  const long long os = P.os;
 
-  // ---- loop over angle chunks
-  for (int a0 = 0; a0 < A; a0 += chunk) {
-    const int na = std::min(chunk, A - a0);
+  // ---- loop over item chunks (one chunk of all items unless plain)
+  for (int a0 = 0; a0 < B; a0 += chunk) {
+    const int na = std::min(chunk, B - a0);
     const long long tot = nn * na;
     cplx* Uphi = Wb(W_UPHI);
     cplx* Vphi = Wb(W_VPHI);
@@ -771,96 +850,116 @@ int project_run(tg_ctx* c) {
 
     // overlaps <Phi|R|Phi> for the gauge rotation (rot_over) and, when printing,
     // for parity x gauge (rot_pari, :279-284).  R_n = D0^T R D0 = ep*Pa + en*Pb.
-    auto overlap = [&](const double* Pa, const double* Pb, cplx* dst) -> int {
-      if (reduced) {
+    // Items are processed in segments of one overlap shape: the whole chunk when
+    // every state has the same nocc / nemp, else one segment per state.
+    auto overlap_segment = [&](const double* Pa, const double* Pb, long long b0, int cnt,
+                               const OvlShape& o, cplx* dst) -> int {
+      const int lo = (int)(b0 - a0);         // chunk-local offset of the segment
+      if (o.reduced) {
         ScopedTiming tm(c, "overlap", s2);
-        const int nb = nbloc;
+        const int nb = o.nbloc;
         const long long mm = (long long)nb * nb;
         cplx* Q = c->d_pfw.as<cplx>();
-        cplx* JQ = Q + (size_t)chunk * mm;
-        cplx* K = JQ + (size_t)chunk * mm;
-        cplx* PW = K + (size_t)chunk * mm;
-        hipLaunchKernelGGL(build_q_kernel, dim3(nblk(mm * na)), dim3(NT), 0, s2, n, nemp, nb, na, Pa,
-                           Pb, d_T, d_sg, d_ph + 2 * a0, Q, JQ);
+        cplx* JQ = Q + (size_t)cnt * mm;
+        cplx* K = JQ + (size_t)cnt * mm;
+        cplx* PW = K + (size_t)cnt * mm;
+        hipLaunchKernelGGL(build_q_kernel, dim3(nblk(mm * cnt)), dim3(NT), 0, s2, n, o.nemp, nb, cnt,
+                           b0, A, Pa, Pb, P.d_T, P.d_sg, os, P.d_ph, Q, JQ);
         TG_HIP(hipGetLastError());
-        TG_TRY(gemm_zz(s2, OP_T, OP_N, nb, nb, nb, one, Q, nb, mm, JQ, nb, mm, zero, K, nb, mm, na));
-        hipLaunchKernelGGL(add_j_kernel, dim3(nblk((long long)(nb / 2) * na)), dim3(NT), 0, s2, nb,
-                           na, d_sg, K);
+        TG_TRY(gemm_zz(s2, OP_T, OP_N, nb, nb, nb, one, Q, nb, mm, JQ, nb, mm, zero, K, nb, mm, cnt));
+        hipLaunchKernelGGL(add_j_kernel, dim3(nblk((long long)(nb / 2) * cnt)), dim3(NT), 0, s2, nb,
+                           cnt, b0, A, P.d_sg, os, K);
         TG_HIP(hipGetLastError());
         {
           ScopedTiming tk(c, "kernel:pfaffian", s2);
-          TG_TRY(batched_pfaffian(s2, nb, K, nb, mm, na, PW, d_logabs, d_phase, d_zero));
+          TG_TRY(batched_pfaffian(s2, nb, K, nb, mm, cnt, PW, P.d_logabs + lo, P.d_phase + lo,
+                                  P.d_zero + lo));
         }
-        hipLaunchKernelGGL(fill_from_kernel, dim3(nblk(na)), dim3(NT), 0, s2, na, P.d_pre0, d_pre);
-        hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s2, na, d_logabs,
-                           d_phase, d_zero, d_pre, P.d_ovac, dst);
+        hipLaunchKernelGGL(fill_from_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt, b0, A, P.d_pre0,
+                           os / 2, P.d_pre + lo);
+        hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt, b0, A,
+                           P.d_logabs + lo, P.d_phase + lo, P.d_zero + lo, P.d_pre + lo, P.d_ovac,
+                           os, dst + lo);
         TG_HIP(hipGetLastError());
         return TG_OK;
       }
       // reference sequence: R2 -> inverse + det -> M_reg -> Pfaffian(nreg)
       ScopedTiming tm(c, "overlap", s2);
-      const int nreg = 2 * nbloc;
+      const int nreg = 2 * o.nbloc, n2 = o.n2;
       const long long m2 = (long long)n2 * n2, mr = (long long)nreg * nreg;
       cplx* R2 = c->d_pfw.as<cplx>();
-      cplx* M = R2 + (size_t)chunk * m2;
-      cplx* PW = M + (size_t)chunk * mr;
-      cplx* det2 = PW + (size_t)chunk * mr;
-      int* info2 = reinterpret_cast<int*>(det2 + chunk);
-      int* perm2 = info2 + chunk;
-      const double* d_vou = P.d_vou;
-      hipLaunchKernelGGL(build_rn2_kernel, dim3(nblk(m2 * na)), dim3(NT), 0, s2, n, nemp, n2, na, Pa,
-                         Pb, d_ph + 2 * a0, R2);
+      cplx* M = R2 + (size_t)cnt * m2;
+      cplx* PW = M + (size_t)cnt * mr;
+      cplx* det2 = PW + (size_t)cnt * mr;
+      int* info2 = reinterpret_cast<int*>(det2 + cnt);
+      int* perm2 = info2 + cnt;
+      hipLaunchKernelGGL(build_rn2_kernel, dim3(nblk(m2 * cnt)), dim3(NT), 0, s2, n, o.nemp, n2, cnt,
+                         b0, A, Pa, Pb, P.d_ph, R2);
       TG_HIP(hipGetLastError());
-      TG_TRY(batched_inverse(s2, n2, R2, n2, m2, na, det2, info2, perm2));
-      if (nbloc > 0) {
-        hipLaunchKernelGGL(build_mreg_kernel, dim3(nblk(mr * na)), dim3(NT), 0, s2, n2, nbloc, nbloc,
-                           nreg, na, d_vou, d_vou, R2, M);
+      TG_TRY(batched_inverse(s2, n2, R2, n2, m2, cnt, det2, info2, perm2));
+      if (o.nbloc > 0) {
+        hipLaunchKernelGGL(build_mreg_kernel, dim3(nblk(mr * cnt)), dim3(NT), 0, s2, n2, o.nbloc,
+                           o.nbloc, nreg, cnt, b0, A, P.d_vou, P.d_vou, os, R2, M);
         TG_HIP(hipGetLastError());
-        TG_TRY(batched_pfaffian(s2, nreg, M, nreg, mr, na, PW, d_logabs, d_phase, d_zero));
+        TG_TRY(batched_pfaffian(s2, nreg, M, nreg, mr, cnt, PW, P.d_logabs + lo, P.d_phase + lo,
+                                P.d_zero + lo));
       } else {
-        hipLaunchKernelGGL(unit_pfaffian_kernel, dim3(nblk(na)), dim3(NT), 0, s2, na, d_logabs,
-                           d_phase, d_zero);
+        hipLaunchKernelGGL(unit_pfaffian_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt,
+                           P.d_logabs + lo, P.d_phase + lo, P.d_zero + lo);
         TG_HIP(hipGetLastError());
       }
-      hipLaunchKernelGGL(scale_det_kernel, dim3(nblk(na)), dim3(NT), 0, s2, na, (double)signfac,
-                         det2, d_pre);
-      hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(na)), dim3(NT), 0, s2, na, d_logabs,
-                         d_phase, d_zero, d_pre, P.d_ovac, dst);
+      hipLaunchKernelGGL(scale_det_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt,
+                         (double)o.signfac, det2, P.d_pre + lo);
+      hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt, b0, A,
+                         P.d_logabs + lo, P.d_phase + lo, P.d_zero + lo, P.d_pre + lo, P.d_ovac, os,
+                         dst + lo);
       TG_HIP(hipGetLastError());
+      return TG_OK;
+    };
+    auto overlap = [&](const double* Pa, const double* Pb, cplx* dst) -> int {
+      if (P.uniform) return overlap_segment(Pa, Pb, a0, na, P.sh[0], dst);
+      for (int st = a0 / A; st <= (a0 + na - 1) / A; ++st) {
+        const long long b0 = std::max<long long>((long long)st * A, a0);
+        const long long b1 = std::min<long long>((long long)(st + 1) * A, a0 + na);
+        TG_TRY(overlap_segment(Pa, Pb, b0, (int)(b1 - b0), P.sh[st], dst));
+      }
       return TG_OK;
     };
     if (!proj) {
       hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, cmk(1.0, 0.0),
-                         d_over + a0);
+                         P.d_over + a0);
       TG_HIP(hipGetLastError());
     } else {
-      TG_TRY(overlap(Pp, Pn, d_over + a0));
+      TG_TRY(overlap(Pp, Pn, P.d_over + a0));
     }
-    if (c->iprint) TG_TRY(overlap(R6 + 8 * nn, R6 + 9 * nn, d_pari + a0));
-
+    if (c->iprint) TG_TRY(overlap(R6 + 8 * ks, R6 + 9 * ks, P.d_pari + a0));
 
     // Thouless matrix A_phi (calculate_thouless :684-718)
     {
       ScopedTiming tm(c, "thouless_inverse");
-      hipLaunchKernelGGL(build_uv_phi_kernel, dim3(nblk(tot)), dim3(NT), 0, s, n, na, R6,
-                         d_ph + 2 * a0, Uphi, Vphi, nn);
+      hipLaunchKernelGGL(build_uv_phi_kernel, dim3(nblk(tot)), dim3(NT), 0, s, n, na, (long long)a0,
+                         A, S, R6, P.d_ph, Uphi, Vphi, nn);
       TG_HIP(hipGetLastError());
       ScopedTiming tk(c, "kernel:gj_inverse");
-      TG_TRY(batched_inverse(s, n, Uphi, n, nn, na, d_det, d_info, d_perm));
+      TG_TRY(batched_inverse(s, n, Uphi, n, nn, na, P.d_det, P.d_info, P.d_perm));
     }
+    // the real U, V of an item's state: batch element b of a launch over the
+    // chunk reads U + (b / A) * nn (one chunk holds whole states, or S = 1)
     {
       ScopedTiming tm(c, "gemm");
       TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, Vphi, n, nn, Uphi, n, nn, zero, Aphi, n, nn, na));
       hipLaunchKernelGGL(conj_kernel, dim3(nblk(tot)), dim3(NT), 0, s, tot, Aphi);
-      hipLaunchKernelGGL(bcast_real_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, U, Ut);
-      hipLaunchKernelGGL(bcast_real_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, V, Vt);
+      hipLaunchKernelGGL(bcast_real_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, (long long)a0, A,
+                         U, Ut);
+      hipLaunchKernelGGL(bcast_real_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, (long long)a0, A,
+                         V, Vt);
       TG_HIP(hipGetLastError());
-      TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, V, n, 0, Aphi, n, nn, one, Ut, n, nn, na));
-      TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, U, n, 0, Aphi, n, nn, one, Vt, n, nn, na));
+      TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, V, n, nn, Aphi, n, nn, one, Ut, n, nn, na, A, 1));
+      TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, U, n, nn, Aphi, n, nn, one, Vt, n, nn, na, A, 1));
       // densities (calculate_densities with conj(U~), conj(V~))
-      TG_TRY(gemm_zd(s, OP_N, OP_T, n, n, n, one, Vt, n, nn, V, n, 0, zero, rho, n, nn, na));
-      TG_TRY(gemm_zd(s, OP_N, OP_T, n, n, n, one, Vt, n, nn, U, n, 0, zero, kLR, n, nn, na));
-      TG_TRY(gemm_dz(s, OP_N, OP_T, n, n, n, one, V, n, 0, Ut, n, nn, zero, kRL, n, nn, na));
+      TG_TRY(gemm_zd(s, OP_N, OP_T, n, n, n, one, Vt, n, nn, V, n, nn, zero, rho, n, nn, na, 1, A));
+      TG_TRY(gemm_zd(s, OP_N, OP_T, n, n, n, one, Vt, n, nn, U, n, nn, zero, kLR, n, nn, na, 1, A));
+      TG_TRY(gemm_dz(s, OP_N, OP_T, n, n, n, one, V, n, nn, Ut, n, nn, zero, kRL, n, nn, na, A, 1));
     }
     // fields
     {
@@ -872,7 +971,7 @@ int project_run(tg_ctx* c) {
     }
     // particle numbers and energy
     hipLaunchKernelGGL(traces_kernel, dim3(na), dim3(NT), 0, s, n, c->d_H1.as<cplx>(), rho, kLR,
-                       kRL, G, dLR, nn, d_tr + 9 * (size_t)a0);
+                       kRL, G, dLR, nn, P.d_tr + 9 * (size_t)a0);
     TG_HIP(hipGetLastError());
     // H20 (calculate_H20 with U~, V~) as G - G^T,
     //   X1 = h V~ + fd/2 DeltaLR U~,  X2 = -fd/2 DeltaRL V~,  G = U~^T X1 + V~^T X2
@@ -894,30 +993,33 @@ int project_run(tg_ctx* c) {
   TG_HIP(hipEventRecord(c->ev_join, s2));
   TG_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
 
-  // ---- sum_gauge and calculate_projected_H20
+  // ---- sum_gauge and calculate_projected_H20, per state
   cplx* pA = c->d_out.as<cplx>();
   cplx* pH = pA + nn;
   cplx* pAH = pH + nn;
   double* H20v = reinterpret_cast<double*>(pAH + nn);
-  hipLaunchKernelGGL(gauge_scalars_kernel, dim3(1), dim3(NT), 0, s, A, n, c->H0, d_w, d_over, d_tr,
-                     c->iprint ? d_pari : nullptr, P.d_mult, d_f12, d_ener, d_pnp);
-  hipLaunchKernelGGL(gauge_matrices_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, A, d_f12, P.d_mult,
-                     Akeep, Hkeep, nn, pA, pH, pAH);
-  hipLaunchKernelGGL(projected_h20_kernel, dim3(nblk(nn)), dim3(NT), 0, s, n, d_pnp, pA, pH, pAH,
-                     H20v, d_imag);
+  hipLaunchKernelGGL(gauge_scalars_kernel, dim3(S), dim3(NT), 0, s, A, n, c->H0, P.d_w, P.d_over,
+                     P.d_tr, c->iprint ? P.d_pari : nullptr, P.d_mult, P.d_f12, P.d_ener, P.d_pnp);
+  hipLaunchKernelGGL(gauge_matrices_kernel, dim3(nblk(nn), S), dim3(NT), 0, s, nn, A, P.d_f12,
+                     P.d_mult, Akeep, Hkeep, nn, pA, pH, pAH, P.outs);
+  hipLaunchKernelGGL(projected_h20_kernel, dim3(nblk(nn), S), dim3(NT), 0, s, n, P.d_pnp, pA, pH,
+                     pAH, H20v, P.d_imag, P.outs, P.is);
   TG_HIP(hipGetLastError());
   if (c->grad_on) {
-    // Z buffers: [Zcur | Zprev | lambda | partial sums]; the new Z becomes the
-    // momentum term of the next step (gradient_Zim1 = gradient_Zi)
+    // Z buffers: [Zi | Zim1 | lambda | partial sums], S states each; the new Z
+    // becomes the momentum term of the next step (gradient_Zim1 = gradient_Zi)
     double* Zb = c->d_Z.as<double>();
     double* Zcur = Zb;
-    double* Zprev = Zb + nn;
-    double* lam = Zb + 2 * nn;
-    double* part = lam + TG_MAX_CONS;
-    hipLaunchKernelGGL(gradient_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, c->grad_eta, c->grad_mu,
-                       c->grad_ncons, lam, c->d_c20.as<double>(), H20v, Zprev, Zcur, part);
+    double* Zprev = Zb + ks;
+    double* lam = Zb + 2 * ks;
+    double* part = lam + (size_t)S * TG_MAX_CONS;
+    const long long ls = c->c20_shared ? 0 : TG_MAX_CONS;
+    const long long cs = c->c20_shared ? 0 : (long long)c->grad_ncons * nn;
+    hipLaunchKernelGGL(gradient_kernel, dim3(nblk(nn), S), dim3(NT), 0, s, nn, c->grad_eta,
+                       c->grad_mu, c->grad_ncons, lam, ls, c->d_c20.as<double>(), cs, H20v,
+                       2 * P.outs, Zprev, Zcur, part, (long long)tg_part_len(n));
     TG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(copy_real_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, Zcur, Zprev);
+    hipLaunchKernelGGL(copy_real_kernel, dim3(nblk(ks)), dim3(NT), 0, s, ks, Zcur, Zprev);
     TG_HIP(hipGetLastError());
   }
   return TG_OK;
@@ -932,10 +1034,16 @@ GraphKey graph_key(const tg_ctx* c) {
   std::memcpy(&b1, &fd, 8);
   std::memcpy(&b2, &eta, 8);
   std::memcpy(&b3, &mu, 8);
-  const long long vals[16] = {c->n, c->A, c->Mphip, c->Mphin, c->plain, c->nocc, c->nemp,
+  unsigned long long shapes = 1469598103934665603ULL;   // FNV-1a over (nocc, nemp) of the states
+  for (const StateInfo& si : c->st)
+    for (int x : {si.nocc, si.nemp}) {
+      shapes ^= (unsigned long long)(unsigned)x;
+      shapes *= 1099511628211ULL;
+    }
+  const long long vals[20] = {c->n, c->A, c->Mphip, c->Mphin, c->plain, c->ns, (long long)shapes,
                               c->iprint, c->grad_on, c->grad_ncons, b1, b2, b3,
-                              (long long)devbuf_epoch(), c->w_fp32, 1};
-  for (int i = 0; i < 16; ++i) k.v[i] = vals[i];
+                              (long long)devbuf_epoch(), c->w_fp32, c->c20_shared, 1, 0, 0, 0};
+  for (int i = 0; i < 20; ++i) k.v[i] = vals[i];
   return k;
 }
 
@@ -1008,39 +1116,57 @@ void permute_out(const tg_ctx* c, const cplx* src, cplx* dst) {
       dst[c->sp_perm[i] + (size_t)c->sp_perm[j] * n] = src[i + (size_t)j * n];
 }
 
-int upload_state(tg_ctx* c, const double* U0, const double* V0, const double* D0, int nocc0,
-                 int nemp0, double ovac0, const double* voveru0, int Mphip, int Mphin,
-                 double facpi, double valZ, double valN, double factor_delta, int iprint) {
+// S states of this basis (state k: U0 + k n^2, V0 + k n^2, D0 + k n^2,
+// voveru0 + k n holding n - nocc0[k] values) and the common projection set-up.
+int upload_states(tg_ctx* c, int S, const double* U0, const double* V0, const double* D0,
+                  const int* nocc0, const int* nemp0, const double* ovac0, const double* voveru0,
+                  int Mphip, int Mphin, double facpi, double valZ, double valN, double factor_delta,
+                  int iprint) {
   const int n = c->n;
-  if (!U0 || !V0 || !D0 || Mphip < 1 || Mphin < 1 || nocc0 < 0 || nemp0 < 0 ||
-      nocc0 + nemp0 > n || (n - nocc0 > 0 && !voveru0)) {
-    tg_set_error("tg_set_state: invalid argument");
+  bool ok = S >= 1 && U0 && V0 && D0 && nocc0 && nemp0 && ovac0 && Mphip >= 1 && Mphin >= 1;
+  for (int k = 0; ok && k < S; ++k)
+    ok = nocc0[k] >= 0 && nemp0[k] >= 0 && nocc0[k] + nemp0[k] <= n &&
+         (n - nocc0[k] == 0 || voveru0);
+  if (!ok) {
+    tg_set_error("tg_set_states: invalid argument");
+    return TG_E_ARG;
+  }
+  if (c->plain && S > 1) {
+    tg_set_error("tg_set_states: the plain per-angle mode holds one state");
     return TG_E_ARG;
   }
   const size_t nn = (size_t)n * n;
-  TG_TRY(c->d_U.ensure(sizeof(double) * nn));
-  TG_TRY(c->d_V.ensure(sizeof(double) * nn));
-  TG_TRY(c->d_D.ensure(sizeof(double) * nn));
-  TG_TRY(c->d_real6.ensure(sizeof(double) * 11 * nn));
+  TG_TRY(c->d_U.ensure(sizeof(double) * nn * S));
+  TG_TRY(c->d_V.ensure(sizeof(double) * nn * S));
+  TG_TRY(c->d_D.ensure(sizeof(double) * nn * S));
+  TG_TRY(c->d_real6.ensure(sizeof(double) * 11 * nn * S));
   // rows (single-particle index) into the internal order
-  std::vector<double> h((size_t)3 * nn);
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) {
-      const size_t src = (size_t)c->sp_perm[i] + (size_t)j * n, dst = (size_t)i + (size_t)j * n;
-      h[dst] = U0[src];
-      h[nn + dst] = V0[src];
-      h[2 * nn + dst] = D0[src];
-    }
-  TG_HIP(hipMemcpyAsync(c->d_U.p, h.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
-  TG_HIP(hipMemcpyAsync(c->d_V.p, h.data() + nn, sizeof(double) * nn, hipMemcpyHostToDevice,
-                        c->stream));
-  TG_HIP(hipMemcpyAsync(c->d_D.p, h.data() + 2 * nn, sizeof(double) * nn, hipMemcpyHostToDevice,
-                        c->stream));
+  std::vector<double> h((size_t)3 * nn * S);
+  double* hU = h.data();
+  double* hV = hU + nn * S;
+  double* hD = hV + nn * S;
+  for (int k = 0; k < S; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const size_t src = k * nn + (size_t)c->sp_perm[i] + (size_t)j * n;
+        const size_t dst = k * nn + (size_t)i + (size_t)j * n;
+        hU[dst] = U0[src];
+        hV[dst] = V0[src];
+        hD[dst] = D0[src];
+      }
+  TG_HIP(hipMemcpyAsync(c->d_U.p, hU, sizeof(double) * nn * S, hipMemcpyHostToDevice, c->stream));
+  TG_HIP(hipMemcpyAsync(c->d_V.p, hV, sizeof(double) * nn * S, hipMemcpyHostToDevice, c->stream));
+  TG_HIP(hipMemcpyAsync(c->d_D.p, hD, sizeof(double) * nn * S, hipMemcpyHostToDevice, c->stream));
   TG_HIP(hipStreamSynchronize(c->stream));
-  c->nocc = nocc0;
-  c->nemp = nemp0;
-  c->ovac = ovac0;
-  c->voveru.assign(voveru0, voveru0 + (n - nocc0));
+  c->st.assign(S, StateInfo());
+  for (int k = 0; k < S; ++k) {
+    StateInfo& si = c->st[k];
+    si.nocc = nocc0[k];
+    si.nemp = nemp0[k];
+    si.ovac = ovac0[k];
+    if (n - nocc0[k] > 0) si.voveru.assign(voveru0 + (size_t)k * n, voveru0 + (size_t)k * n + (n - nocc0[k]));
+  }
+  c->ns = S;
   c->Mphip = Mphip;
   c->Mphin = Mphin;
   c->facpi = facpi;
@@ -1048,6 +1174,7 @@ int upload_state(tg_ctx* c, const double* U0, const double* V0, const double* D0
   c->valN = valN;
   c->factor_delta = factor_delta;
   c->iprint = iprint ? 1 : 0;
+  if (c->grad_on) TG_TRY(tg_ensure_z(c));
   // assign_angles (:112-185).  With TG_FLAG_CONJ_PAIRS and the full [0, 2 pi)
   // grid, the grid is closed under phi -> -phi (mod 2 pi): for real U, V every
   // per-angle quantity at -phi is the complex conjugate of that at phi, and so
@@ -1084,15 +1211,19 @@ int upload_state(tg_ctx* c, const double* U0, const double* V0, const double* D0
   return TG_OK;
 }
 
-int fetch_results(tg_ctx* c, double* scal, double* pA, double* pH, double* pAH, double* H20v,
-                  double* imag) {
+int fetch_results(tg_ctx* c, int state, double* scal, double* pA, double* pH, double* pAH,
+                  double* H20v, double* imag) {
   const int n = c->n;
   const size_t nn = (size_t)n * n;
+  if (state < 0 || state >= c->ns) {
+    tg_set_error("results: state index out of range");
+    return TG_E_ARG;
+  }
   TG_HIP(hipStreamSynchronize(c->stream));
-  const cplx* d_out = c->d_out.as<cplx>();
   const RunPlan P = make_plan(c);
-  const cplx* d_pnp = P.d_pnp;
-  const double* d_imag = P.d_imag;
+  const cplx* d_out = c->d_out.as<cplx>() + (size_t)state * P.outs;
+  const cplx* d_pnp = P.d_pnp + 8 * (size_t)state;
+  const double* d_imag = P.d_imag + (size_t)state * P.is;
   if (scal) TG_HIP(hipMemcpy(scal, d_pnp, sizeof(cplx) * 7, hipMemcpyDeviceToHost));
   if (pA) TG_HIP(hipMemcpy(pA, d_out, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
   if (pH) TG_HIP(hipMemcpy(pH, d_out + nn, sizeof(cplx) * nn, hipMemcpyDeviceToHost));
@@ -1127,17 +1258,40 @@ int get_device_ready() {
 
 // entry points for the other translation units of the library
 int tg_project_device(tg_ctx* c) { return project_device(c); }
-int tg_gradient_step(tg_ctx* c, const double* H20v, double* Z, const double* Zm1,
-                     const double* lam, const double* c20, double* part) {
+int tg_project_upload(tg_ctx* c) { return c->state_dirty ? project_upload(c) : TG_OK; }
+int tg_project_run(tg_ctx* c) { return project_run(c); }
+int tg_ensure_z(tg_ctx* c) {
   const long long nn = (long long)c->n * c->n;
-  hipLaunchKernelGGL(gradient_kernel, dim3(nblk(nn)), dim3(NT), 0, c->stream, nn, c->grad_eta,
-                     c->grad_mu, c->grad_ncons, lam, c20, H20v, Zm1, Z, part);
+  const int S = c->ns;
+  const size_t need = sizeof(double) * ((size_t)2 * S * nn + (size_t)S * TG_MAX_CONS +
+                                        (size_t)S * tg_part_len(c->n));
+  if (c->z_states == S && c->d_Z.bytes >= need) return TG_OK;
+  TG_TRY(c->d_Z.ensure(need));
+  TG_HIP(hipMemsetAsync(c->d_Z.p, 0, need, c->stream));
+  // multipliers common to every state (tg_set_gradient) in the slot of state 0
+  if (c->c20_shared && !c->grad_lambda.empty())
+    TG_HIP(hipMemcpyAsync(c->d_Z.as<double>() + 2 * S * nn, c->grad_lambda.data(),
+                          sizeof(double) * c->grad_lambda.size(), hipMemcpyHostToDevice, c->stream));
+  TG_HIP(hipStreamSynchronize(c->stream));
+  c->z_states = S;
+  return TG_OK;
+}
+int tg_gradient_step(tg_ctx* c, const double* c20, const double* lam) {
+  const int n = c->n, S = c->ns;
+  const long long nn = (long long)n * n;
+  double* Zb = c->d_Z.as<double>();
+  double* part = Zb + 2 * S * nn + (size_t)S * TG_MAX_CONS;
+  const double* H20v = reinterpret_cast<const double*>(c->d_out.as<cplx>() + 3 * nn);
+  hipLaunchKernelGGL(gradient_kernel, dim3(nblk(nn), S), dim3(NT), 0, c->stream, nn, c->grad_eta,
+                     c->grad_mu, c->grad_ncons, lam, (long long)TG_MAX_CONS, c20,
+                     (long long)c->grad_ncons * nn, H20v, 2 * tg_out_stride(n), Zb + S * nn, Zb,
+                     part, tg_part_len(n));
   TG_HIP(hipGetLastError());
   return TG_OK;
 }
-int tg_fetch_results(tg_ctx* c, double* scal, double* pA, double* pH, double* pAH, double* H20v,
-                     double* imag) {
-  return fetch_results(c, scal, pA, pH, pAH, H20v, imag);
+int tg_fetch_results(tg_ctx* c, int state, double* scal, double* pA, double* pH, double* pAH,
+                     double* H20v, double* imag) {
+  return fetch_results(c, state, scal, pA, pH, pAH, H20v, imag);
 }
 
 // ===========================================================================
@@ -1455,7 +1609,7 @@ int tg_calculate_overlap(int ndim, int nocc0, int nemp0, double ovac0, const dou
   if (r == TG_OK && (isla0 + isla1) != 3) {
     if (nreg > 0 && nreg % 2 == 0) {
       hipLaunchKernelGGL(build_mreg_kernel, dim3(nblk((long long)mr)), dim3(NT), 0, s, n2, nbloc0,
-                         nbloc1, nreg, 1, dv0, dv1, dRn, dM);
+                         nbloc1, nreg, 1, 0LL, 1, dv0, dv1, 0LL, dRn, dM);
       if (hipGetLastError() != hipSuccess) r = TG_E_HIP;
       if (r == TG_OK) r = batched_pfaffian(s, nreg, dM, nreg, 0, 1, dPW, dL, dPh, dZ);
       double la = 0;
@@ -1501,8 +1655,20 @@ int tg_set_state(tg_ctx* c, const double* U0, const double* V0, const double* D0
                  double facpi, double valence_Z, double valence_N, double factor_delta,
                  int iprint) {
   if (!c) return TG_E_ARG;
-  return upload_state(c, U0, V0, D0, nocc0, nemp0, ovac0, voveru0, Mphip, Mphin, facpi, valence_Z,
-                      valence_N, factor_delta, iprint);
+  const int n = c->n;
+  std::vector<double> vou(n, 0.0);
+  if (voveru0 && nocc0 >= 0 && nocc0 <= n) std::copy(voveru0, voveru0 + (n - nocc0), vou.begin());
+  return upload_states(c, 1, U0, V0, D0, &nocc0, &nemp0, &ovac0, voveru0 ? vou.data() : nullptr,
+                       Mphip, Mphin, facpi, valence_Z, valence_N, factor_delta, iprint);
+}
+
+int tg_set_states(tg_ctx* c, int nstates, const double* U0, const double* V0, const double* D0,
+                  const int* nocc0, const int* nemp0, const double* ovac0,
+                  const double* voveru0, int Mphip, int Mphin, double facpi, double valence_Z,
+                  double valence_N, double factor_delta, int iprint) {
+  if (!c) return TG_E_ARG;
+  return upload_states(c, nstates, U0, V0, D0, nocc0, nemp0, ovac0, voveru0, Mphip, Mphin, facpi,
+                       valence_Z, valence_N, factor_delta, iprint);
 }
 
 int tg_project_resident(tg_ctx* c) {
@@ -1523,7 +1689,14 @@ int tg_sync(tg_ctx* c) {
 int tg_get_results(tg_ctx* c, double* pnp_scalars, double* pnp_A, double* pnp_H20,
                    double* pnp_AH20, double* field_H20v, double* imag_grad) {
   if (!c || !c->have_state) return TG_E_ARG;
-  return fetch_results(c, pnp_scalars, pnp_A, pnp_H20, pnp_AH20, field_H20v, imag_grad);
+  return fetch_results(c, 0, pnp_scalars, pnp_A, pnp_H20, pnp_AH20, field_H20v, imag_grad);
+}
+
+int tg_get_results_state(tg_ctx* c, int state, double* pnp_scalars, double* pnp_A,
+                         double* pnp_H20, double* pnp_AH20, double* field_H20v,
+                         double* imag_grad) {
+  if (!c || !c->have_state) return TG_E_ARG;
+  return fetch_results(c, state, pnp_scalars, pnp_A, pnp_H20, pnp_AH20, field_H20v, imag_grad);
 }
 
 int tg_project_wavefunction(tg_ctx* c, int iprint, const double* U0, const double* V0,
@@ -1533,10 +1706,10 @@ int tg_project_wavefunction(tg_ctx* c, int iprint, const double* U0, const doubl
                             double* pnp_scalars, double* pnp_A, double* pnp_H20,
                             double* pnp_AH20, double* field_H20v, double* imag_grad) {
   if (!c) return TG_E_ARG;
-  TG_TRY(upload_state(c, U0, V0, D0, nocc0, nemp0, ovac0, voveru0, Mphip, Mphin, facpi, valence_Z,
+  TG_TRY(tg_set_state(c, U0, V0, D0, nocc0, nemp0, ovac0, voveru0, Mphip, Mphin, facpi, valence_Z,
                       valence_N, factor_delta, iprint));
   TG_TRY(tg_project_resident(c));
-  return fetch_results(c, pnp_scalars, pnp_A, pnp_H20, pnp_AH20, field_H20v, imag_grad);
+  return fetch_results(c, 0, pnp_scalars, pnp_A, pnp_H20, pnp_AH20, field_H20v, imag_grad);
 }
 
 int tg_calculate_gradient(int ndim, int gradient_type, double gradient_eta, double gradient_mu,
@@ -1621,13 +1794,12 @@ int tg_set_gradient(tg_ctx* c, double gradient_eta, double gradient_mu, int cons
   c->grad_eta = gradient_eta;
   c->grad_mu = gradient_mu;
   c->grad_ncons = constraint_dim;
-  TG_TRY(c->d_Z.ensure(sizeof(double) * (2 * nn + TG_MAX_CONS + nn / NT + 2)));
+  c->c20_shared = 1;           // one constraint_20 / lambda set for every state
+  c->grad_lambda.assign(lagrange_lambda1, lagrange_lambda1 + constraint_dim);
+  c->z_states = 0;             // zero Zi, Zim1 of every state
+  TG_TRY(tg_ensure_z(c));      // (and uploads the multipliers)
   TG_TRY(c->d_c20.ensure(sizeof(double) * std::max<size_t>(1, (size_t)constraint_dim * nn)));
-  double* Zb = c->d_Z.as<double>();
-  TG_HIP(hipMemset(Zb, 0, sizeof(double) * 2 * nn));
   if (constraint_dim > 0) {
-    TG_HIP(hipMemcpy(Zb + 2 * nn, lagrange_lambda1, sizeof(double) * constraint_dim,
-                     hipMemcpyHostToDevice));
     TG_HIP(hipMemcpy(c->d_c20.p, constraint_20, sizeof(double) * constraint_dim * nn,
                      hipMemcpyHostToDevice));
   }
@@ -1635,18 +1807,28 @@ int tg_set_gradient(tg_ctx* c, double gradient_eta, double gradient_mu, int cons
 }
 
 int tg_get_gradient(tg_ctx* c, double* gradient_Zi, double* gradient_norm) {
-  if (!c || !c->grad_on) {
+  return tg_get_gradient_state(c, 0, gradient_Zi, gradient_norm);
+}
+
+int tg_get_gradient_state(tg_ctx* c, int state, double* gradient_Zi, double* gradient_norm) {
+  if (!c || !c->grad_on || c->z_states < 1) {
     tg_set_error("tg_get_gradient: tg_set_gradient was not called");
     return TG_E_ARG;
   }
+  if (state < 0 || state >= c->z_states) {
+    tg_set_error("tg_get_gradient: state index out of range");
+    return TG_E_ARG;
+  }
   const size_t nn = (size_t)c->n * c->n;
+  const int S = c->z_states;
   TG_HIP(hipStreamSynchronize(c->stream));
   const double* Zb = c->d_Z.as<double>();
-  if (gradient_Zi) TG_HIP(hipMemcpy(gradient_Zi, Zb, sizeof(double) * nn, hipMemcpyDeviceToHost));
+  if (gradient_Zi)
+    TG_HIP(hipMemcpy(gradient_Zi, Zb + state * nn, sizeof(double) * nn, hipMemcpyDeviceToHost));
   const size_t nb = (nn + NT - 1) / NT;
   std::vector<double> part(nb);
-  TG_HIP(hipMemcpy(part.data(), Zb + 2 * nn + TG_MAX_CONS, sizeof(double) * nb,
-                   hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(part.data(), Zb + 2 * nn * S + (size_t)S * TG_MAX_CONS + state * tg_part_len(c->n),
+                   sizeof(double) * nb, hipMemcpyDeviceToHost));
   double sum = 0;
   for (double v : part) sum += v;
   if (gradient_norm) *gradient_norm = sqrt(sum) / c->grad_eta;

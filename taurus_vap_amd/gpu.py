@@ -36,7 +36,9 @@ EXPORTS = [
     "tg_project_wavefunction", "tg_set_state", "tg_project_resident", "tg_sync",
     "tg_get_results", "tg_calculate_gradient", "tg_set_gradient", "tg_get_gradient",
     "tg_enable_timing", "tg_reset_timing", "tg_get_timing", "tg_set_constraints",
-    "tg_iterate", "tg_get_state",
+    "tg_iterate", "tg_get_state", "tg_set_states", "tg_get_results_state",
+    "tg_get_gradient_state", "tg_group_create", "tg_group_destroy", "tg_group_project",
+    "tg_group_iterate", "tg_group_iterate_results",
 ]
 
 _lib = None
@@ -76,9 +78,17 @@ def load_library():
     lib.tg_enable_timing.argtypes = [P, I]
     lib.tg_reset_timing.argtypes = [P]
     lib.tg_get_timing.argtypes = [P, ctypes.c_char_p, P, P]
-    lib.tg_set_constraints.argtypes = [P, I, I, P, P, D]
+    lib.tg_set_constraints.argtypes = [P, I, I, P, I, P, D]
     lib.tg_iterate.argtypes = [P, D, P, P, P, P, P]
     lib.tg_get_state.argtypes = [P, P, P]
+    lib.tg_set_states.argtypes = [P, I, P, P, P, P, P, P, P, I, I, D, D, D, D, I]
+    lib.tg_get_results_state.argtypes = [P, I, P, P, P, P, P, P]
+    lib.tg_get_gradient_state.argtypes = [P, I, P, P]
+    lib.tg_group_create.argtypes = [ctypes.POINTER(P), I, P, I]
+    lib.tg_group_destroy.argtypes = [P]
+    lib.tg_group_project.argtypes = [P]
+    lib.tg_group_iterate.argtypes = [P, D]
+    lib.tg_group_iterate_results.argtypes = [P, I, P, P, P, P, P]
     _lib = lib
     return lib
 
@@ -124,6 +134,8 @@ class Context:
         _check(lib.tg_init(ctypes.byref(h), self.n, *[_p(x) for x in self._keep], H0.real, H0.imag,
                            _p(H1), ntb, _p(abcd), _p(H2), H2.itemsize, _p(trperm), flags), "tg_init")
         self._h = h
+        self.nstates = 1
+        self.constraint_dim = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -187,10 +199,39 @@ class Context:
         if vou.size == 0:
             vou = np.zeros(1)
         self._state_keep = (U0, V0, D0, vou)
+        self.nstates = 1
         _check(load_library().tg_set_state(self._h, _p(U0), _p(V0), _p(D0), int(nocc0), int(nemp0),
                                            float(ovac0), _p(vou), int(Mphip), int(Mphin),
                                            float(facpi), float(valence_Z), float(valence_N),
                                            float(factor_delta), int(iprint)), "tg_set_state")
+
+    def set_states(self, states, Mphip, Mphin, facpi, valence_Z, valence_N, factor_delta=1.0,
+                   iprint=0):
+        """Several independent states of this basis, projected together.
+        states: list of (U0, V0, canon) with canon = construct_canonical_basis(U0, V0)."""
+        n = self.n
+        S = len(states)
+        U = np.zeros((S, n, n))
+        V = np.zeros((S, n, n))
+        D = np.zeros((S, n, n))
+        vou = np.zeros((S, n))
+        nocc = np.zeros(S, np.int32)
+        nemp = np.zeros(S, np.int32)
+        ovac = np.zeros(S)
+        for k, (Uk, Vk, cn) in enumerate(states):
+            # state k column-major at + k n^2: the transposed C-order slab
+            U[k] = np.asarray(Uk, dtype=np.float64).T
+            V[k] = np.asarray(Vk, dtype=np.float64).T
+            D[k] = np.asarray(cn["D"], dtype=np.float64).T
+            v = np.asarray(cn["voveru"], dtype=np.float64).reshape(-1)
+            vou[k, :v.size] = v
+            nocc[k], nemp[k], ovac[k] = int(cn["nocc"]), int(cn["nemp"]), float(cn["ovac"])
+        self._state_keep = (U, V, D, vou, nocc, nemp, ovac)
+        _check(load_library().tg_set_states(self._h, S, _p(U), _p(V), _p(D), _p(nocc), _p(nemp),
+                                            _p(ovac), _p(vou), int(Mphip), int(Mphin), float(facpi),
+                                            float(valence_Z), float(valence_N), float(factor_delta),
+                                            int(iprint)), "tg_set_states")
+        self.nstates = S
 
     def project_resident(self):
         _check(load_library().tg_project_resident(self._h), "tg_project_resident")
@@ -198,13 +239,13 @@ class Context:
     def sync(self):
         _check(load_library().tg_sync(self._h), "tg_sync")
 
-    def get_results(self):
+    def get_results(self, state=0):
         n = self.n
         scal = np.zeros(7, np.complex128)
         H20v = np.zeros(n * n)
         imag = np.zeros(1)
-        _check(load_library().tg_get_results(self._h, _p(scal), None, None, None, _p(H20v), _p(imag)),
-               "tg_get_results")
+        _check(load_library().tg_get_results_state(self._h, int(state), _p(scal), None, None, None,
+                                                   _p(H20v), _p(imag)), "tg_get_results")
         return dict(over=scal[0], ener=scal[1], prot=scal[2], neut=scal[3], prot2=scal[4],
                     neut2=scal[5], pari=scal[6], H20v=H20v, imag=float(imag[0]))
 
@@ -220,52 +261,69 @@ class Context:
         _check(load_library().tg_set_gradient(self._h, float(gradient_eta), float(gradient_mu), ncons,
                                               _p(c20), _p(lam)), "tg_set_gradient")
 
-    def get_gradient(self):
+    def get_gradient(self, state=0):
         Zi = np.zeros(self.n * self.n)
         norm = np.zeros(1)
-        _check(load_library().tg_get_gradient(self._h, _p(Zi), _p(norm)), "tg_get_gradient")
+        _check(load_library().tg_get_gradient_state(self._h, int(state), _p(Zi), _p(norm)),
+               "tg_get_gradient")
         return Zi, float(norm[0])
 
     # device-resident gradient iteration (gradient_type 0)
     def set_constraints(self, constraint_HO, constraint_val, constraint_pair=0,
                         constraint_eps=1e-6):
-        val = np.ascontiguousarray(np.asarray(constraint_val, dtype=np.float64).reshape(-1))
-        dim = val.size
+        """constraint_val: one set (constraint_dim values) for every state, or
+        one row per state (nstates x constraint_dim)."""
+        val2 = np.atleast_2d(np.asarray(constraint_val, dtype=np.float64))
+        nsets, dim = val2.shape
+        val = np.ascontiguousarray(val2.reshape(-1))
         HO = (np.ascontiguousarray(constraint_HO, dtype=np.float64).reshape(-1) if dim
               else np.zeros(1))
         if dim and HO.size != dim * self.n * self.n:
             raise ValueError("constraint_HO must hold constraint_dim * ndim**2 values")
         self._cons_keep = (HO, val)
         _check(load_library().tg_set_constraints(self._h, dim, int(constraint_pair), _p(HO),
-                                                 _p(val if dim else np.zeros(1)),
+                                                 max(1, nsets), _p(val if dim else np.zeros(1)),
                                                  float(constraint_eps)), "tg_set_constraints")
         self.constraint_dim = dim
 
+    def _iterate_buffers(self):
+        S, dim = self.nstates, self.constraint_dim
+        return (np.zeros(7 * S, np.complex128), np.zeros(S), np.zeros(max(1, S * dim)),
+                np.zeros(S, np.int32), np.zeros(S, np.int32))
+
+    def _iterate_dicts(self, scal, gn, lam, nadj, conv):
+        S, dim = self.nstates, self.constraint_dim
+        out = []
+        for k in range(S):
+            sc = scal[7 * k:7 * k + 7]
+            out.append(dict(over=sc[0], ener=sc[1], prot=sc[2], neut=sc[3], prot2=sc[4],
+                            neut2=sc[5], pari=sc[6], gradient_norm=float(gn[k]),
+                            lagrange_lambda1=lam[k * dim:(k + 1) * dim].copy(),
+                            adjust_iterations=int(nadj[k]), converged=bool(conv[k])))
+        return out
+
     def iterate(self, gradient_eps=0.0):
         """One iteration of the gradient loop on the device (projection of the
-        current state, Lagrange multipliers, gradient, constraint adjustment).
-        Returns the projected scalars of the current state, the gradient norm,
-        the multipliers and the number of adjustment steps; the state on the
-        device becomes the evolved one (read it with get_state)."""
-        dim = max(1, getattr(self, "constraint_dim", 0))
-        scal = np.zeros(7, np.complex128)
-        gn = np.zeros(1)
-        lam = np.zeros(dim)
-        nadj = ctypes.c_int()
-        conv = ctypes.c_int()
-        _check(load_library().tg_iterate(self._h, float(gradient_eps), _p(scal), _p(gn), _p(lam),
-                                         ctypes.byref(nadj), ctypes.byref(conv)), "tg_iterate")
-        return dict(over=scal[0], ener=scal[1], prot=scal[2], neut=scal[3], prot2=scal[4],
-                    neut2=scal[5], pari=scal[6], gradient_norm=float(gn[0]),
-                    lagrange_lambda1=lam[:getattr(self, "constraint_dim", 0)].copy(),
-                    adjust_iterations=nadj.value, converged=bool(conv.value))
+        current states, Lagrange multipliers, gradient, constraint adjustment).
+        Returns, per state, the projected scalars, the gradient norm, the
+        multipliers and the number of adjustment steps (a dict for a single
+        state, else a list of dicts); the states on the device become the
+        evolved ones (read them with get_state)."""
+        bufs = self._iterate_buffers()
+        _check(load_library().tg_iterate(self._h, float(gradient_eps), *[_p(b) for b in bufs]),
+               "tg_iterate")
+        out = self._iterate_dicts(*bufs)
+        return out[0] if self.nstates == 1 else out
 
     def get_state(self):
-        n = self.n
-        U = np.zeros((n, n), order="F")
-        V = np.zeros((n, n), order="F")
+        """(U, V) of the device state, or lists of them for several states."""
+        n, S = self.n, self.nstates
+        U = np.zeros(S * n * n)
+        V = np.zeros(S * n * n)
         _check(load_library().tg_get_state(self._h, _p(U), _p(V)), "tg_get_state")
-        return U, V
+        Us = [U[k * n * n:(k + 1) * n * n].reshape((n, n), order="F") for k in range(S)]
+        Vs = [V[k * n * n:(k + 1) * n * n].reshape((n, n), order="F") for k in range(S)]
+        return (Us[0], Vs[0]) if S == 1 else (Us, Vs)
 
     def enable_timing(self, on=True):
         _check(load_library().tg_enable_timing(self._h, 1 if on else 0), "tg_enable_timing")
@@ -280,6 +338,46 @@ class Context:
         _check(load_library().tg_get_timing(self._h, key.encode(), ctypes.byref(ms), ctypes.byref(cnt)),
                "tg_get_timing")
         return ms.value, cnt.value
+
+
+class Group:
+    """Contexts (of any basis sizes) whose projections are issued together as
+    one hipGraph launch (graph=True) or eagerly on their own streams."""
+
+    def __init__(self, contexts, graph=True):
+        self.contexts = list(contexts)
+        arr = (ctypes.c_void_p * len(self.contexts))(*[c._h for c in self.contexts])
+        h = ctypes.c_void_p()
+        _check(load_library().tg_group_create(ctypes.byref(h), len(self.contexts),
+                                              ctypes.cast(arr, ctypes.c_void_p), 1 if graph else 0),
+               "tg_group_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().tg_group_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def project(self):
+        _check(load_library().tg_group_project(self._h), "tg_group_project")
+
+    def iterate(self, gradient_eps=0.0):
+        """tg_iterate of every context around one group projection; returns, per
+        context, the list of per-state result dicts."""
+        _check(load_library().tg_group_iterate(self._h, float(gradient_eps)), "tg_group_iterate")
+        res = []
+        for i, c in enumerate(self.contexts):
+            bufs = c._iterate_buffers()
+            _check(load_library().tg_group_iterate_results(self._h, i, *[_p(b) for b in bufs]),
+                   "tg_group_iterate_results")
+            res.append(c._iterate_dicts(*bufs))
+        return res
 
 
 def zskpfa(A):

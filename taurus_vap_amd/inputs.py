@@ -341,11 +341,26 @@ WORKLOADS = {
     # no-core e_max=4 (140 states), 8p/8n, 11x11
     "emax4": Workload("emax4", "emax4", 8, 8, 11, 2.0, 0.0, "hw", decay=4.0, iters=10,
                       seed_sigma=0.02),
+    # sdpf (64 states), 6p/6n, Q20, 9x9 angles: the largest basis of the batched solves
+    "sdpf": Workload("sdpf", "sdpf", 6, 6, 9, 1.0, -1.5,
+                     [0.0, 1.0, 2.0, 4.0, 16.0 / 3.0, 20.0 / 3.0, 8.0], iters=50,
+                     constraints={"Constraint multipole Q20": 0.2}),
     # e_max=6 (336 states), 10p/12n, Q20 + Jx, 15x15 angles
     "emax6": Workload("emax6", "emax6", 10, 12, 15, 2.0, 0.0, "hw", decay=4.0, iters=100,
                       constraints={"Constraint multipole Q20": 0.2,
                                    "Constraint ang. mom. Jx": 0.0}, seed_sigma=0.01),
 }
+
+
+# Independent constrained solves of uneven basis size run together: (workload,
+# number of solves) with Q20 values on a beta2 grid from -0.4 to 0.6, 9x9 angles.
+BATCHED = [("sd", 24), ("pf", 24), ("sdpf", 16)]
+BATCHED_BETA2 = (-0.4, 0.6)
+
+
+def batched_betas(count: int) -> list[float]:
+    lo, hi = BATCHED_BETA2
+    return [lo + (hi - lo) * k / max(1, count - 1) for k in range(count)]
 
 
 def workload_spe(w: Workload, shells):

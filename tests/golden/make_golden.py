@@ -9,6 +9,7 @@ The fixtures are data only: inputs and the reference's outputs.
 """
 from __future__ import annotations
 
+import dataclasses
 import os
 import shutil
 import subprocess
@@ -92,6 +93,49 @@ def synthetic_case(wname: str, prec: int, M: int, iters: int, angle_sel, seed: i
         return pack_case(d, ham, angle_sel, prec)
 
 
+def batch_case(wname: str, betas, prec: int = 64, M: int = 9, iters: int = 3,
+               seed: int = 1234) -> dict:
+    """Independent Q20-constrained solves of one basis (one reference run per
+    beta2 value, same seed and Hamiltonian): the common inputs once, and per
+    solve its iteration-1 state, constraint values and iteration history."""
+    w0 = I.WORKLOADS[wname]
+    runs = []
+    for b in betas:
+        w = dataclasses.replace(w0, M=M, constraints={"Constraint multipole Q20": float(b)})
+        with tempfile.TemporaryDirectory() as td:
+            I.write_workload(w, td, seed=seed, M=M, iters=iters)
+            d = run_harness(td, prec)
+            n = int(d["ndim"][0])
+            ham = P.read_red(os.path.join(td, w.name + ".red"), n, 4 if prec == 32 else 8)
+        runs.append((d, ham))
+    d0, ham = runs[0]
+    n = int(d0["ndim"][0])
+    for d, _ in runs:
+        assert int(d["niter"][0]) == iters
+        for k in ("facpi", "constraint_dim", "constraint_pair", "Mphip", "Mphin"):
+            assert np.array_equal(np.asarray(d[k][0]), np.asarray(d0[k][0])), k
+        assert np.array_equal(np.asarray(d["constraint_HO"][0]), np.asarray(d0["constraint_HO"][0]))
+    st = lambda key: np.stack([np.asarray(d[key][0]) for d, _ in runs])
+    return dict(
+        ndim=n, prec=prec, betas=np.asarray(betas, dtype=np.float64),
+        sp_2mj=d0["sp_2mj"][0], sp_2mt=d0["sp_2mt"][0], sp_l=d0["sp_l"][0], sp_tr=d0["sp_tr"][0],
+        valence_Z=d0["valence_Z"][0], valence_N=d0["valence_N"][0],
+        Mphip=d0["Mphip"][0], Mphin=d0["Mphin"][0], facpi=d0["facpi"][0],
+        gradient_type=d0["gradient_type"][0], gradient_eta=d0["gradient_eta"][0],
+        gradient_mu=d0["gradient_mu"][0], constraint_dim=d0["constraint_dim"][0],
+        constraint_pair=d0["constraint_pair"][0], factor_delta=d0["factor_delta"][0],
+        H0=d0["H0"][0], H1=d0["H1"][0],
+        abcd=(ham["abcd"] + 1).astype(np.int16),
+        H2=ham["H2"].astype(np.float32 if prec == 32 else np.float64),
+        trperm=ham["trperm"].astype(np.int8),
+        constraint_HO=d0["constraint_HO"][0],
+        U0=st("U0"), V0=st("V0"), constraint_val=st("constraint_val"),
+        lagrange_lambda1=st("lagrange_lambda1"), niter=iters,
+        hist_ener=st("hist_ener"), hist_prot=st("hist_prot"), hist_neut=st("hist_neut"),
+        hist_gradnorm=st("hist_gradnorm"),
+    )
+
+
 def mg24_case(prec: int, M: int, iters: int, angle_sel, xo: int = 1) -> dict:
     with tempfile.TemporaryDirectory() as td:
         shutil.copy(os.path.join(REF, "int", "usdb.sho"), td)
@@ -153,6 +197,14 @@ def main():
         "sd64_m4": lambda: synthetic_case("sd", 64, 4, 1, [0, 5]),
         # reference's own example: Mg24, USDB, Q21 constraint, 5x5 angles
         "mg24_32": lambda: mg24_case(32, 5, 3, [0, 12]),
+        # independent Q20-constrained solves of three basis sizes (9x9 angles,
+        # 3 iterations each), the cases of the batched many-solve path.  The
+        # beta2 values are ones whose constraint adjustment converges: where
+        # adjust_constraints runs out its 100 steps the trajectory depends on
+        # rounding and no implementation can reproduce it.
+        "batch_sd": lambda: batch_case("sd", [-0.1, 0.0, 0.2]),
+        "batch_pf": lambda: batch_case("pf", [-0.1, 0.0, 0.2]),
+        "batch_sdpf": lambda: batch_case("sdpf", [-0.1, 0.0, 0.2]),
     }
     only = sys.argv[1:]
     for name, fn in cases.items():
