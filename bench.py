@@ -144,6 +144,107 @@ def run_gpu(args, ham, plain, fp32, steps, warmup, timing=True, graph=False, con
                 nb=nbloc, nocc=cb["nocc"], nemp=cb["nemp"], UV=(U, V), boundary=boundary)
 
 
+def batched_solves(args):
+    """Independent solves of uneven basis size in one launch per step: 24 sd
+    (n = 24), 24 pf (n = 40) and 16 sdpf (n = 64) seeded states, 9x9 angles,
+    one context per basis holding its solves, the three contexts one group
+    replayed as a captured hipGraph.  A step = the projected gradient of every
+    solve (same hot path as the main line).  Also the largest |dE| of the FP32-
+    TBME run against the FP64-TBME run over the 64 solves."""
+    from taurus_vap_amd import gpu, hamiltonian as H, host, inputs as I
+
+    def build(fp32):
+        ctxs, info = [], []
+        for wname, count in I.BATCHED:
+            w = I.WORKLOADS[wname]
+            h = H.workload_hamiltonian(w, args.seed)
+            b = h["basis"]
+            H2 = h["H2"].astype(np.float32) if fp32 else h["H2"]
+            c = gpu.Context(b.dim, b.sp_2mj, b.sp_2mt, b.sp_l, b.sp_tr, h["H0"], h["H1"],
+                            h["abcd"], H2, h["trperm"], tbme_fp32=fp32)
+            sts = []
+            for k in range(count):
+                U, V = host.seed_state(w, b, args.seed + 101 * k)
+                sts.append((U, V, host.construct_canonical_basis(U, V)))
+            c.set_gradient(args.eta, args.mu)
+            c.set_states(sts, 9, 9, FACPI, float(w.Z), float(w.N))
+            ctxs.append(c)
+            info.append((wname, b.dim, count))
+        return ctxs, info
+
+    def energies(ctxs):
+        out = []
+        for c in ctxs:
+            for k in range(c.nstates):
+                r = c.get_results(k)
+                out.append((r["ener"] / r["over"]).real)
+        return np.array(out)
+
+    ctxs, info = build(True)
+    grp = gpu.Group(ctxs, graph=True)
+    for _ in range(3):          # eager, captured, replayed
+        grp.project()
+    ctxs[0].sync()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        grp.project()
+    for c in ctxs:
+        c.sync()
+    t1 = time.perf_counter()
+    E32 = energies(ctxs)
+    grp.close()
+    for c in ctxs:
+        c.close()
+    ctxs64, _ = build(False)
+    grp = gpu.Group(ctxs64, graph=False)
+    grp.project()
+    for c in ctxs64:
+        c.sync()
+    E64 = energies(ctxs64)
+    grp.close()
+    for c in ctxs64:
+        c.close()
+    nsol = sum(cnt for _, _, cnt in info)
+    sec = (t1 - t0) / args.batch_steps
+    return {"solves": nsol,
+            "bases": " + ".join("%d x %s (n=%d)" % (cnt, wn, nd) for wn, nd, cnt in info),
+            "angles": "9x9", "graph": True,
+            "value": nsol / sec, "unit": "solve-iterations/s", "ms_per_step": 1e3 * sec,
+            "max_abs_dE_fp32_vs_fp64_keV": float(np.max(np.abs(E32 - E64))) * 1e3}
+
+
+def device_iteration(args, ham):
+    """Full gradient iterations of the workload with the constraint machinery
+    on the device (tg_iterate: constraint_20, projection, Lagrange multipliers,
+    gradient, adjust_constraints) and the canonical basis of each new state on
+    the host, with <Z>, <N> constrained.  Not part of `value`."""
+    from taurus_vap_amd import host
+    ctx, h, w, U, V = host.workload_context(args.workload, seed=args.seed, tbme_fp32=True, ham=ham)
+    M = args.angles if args.angles else w.M
+    Zop, Nop = host.number_operators(h["basis"])
+    ctx.set_gradient(args.eta, 0.0)
+    ctx.set_constraints(np.stack([Zop.T, Nop.T]).reshape(-1), [float(w.Z), float(w.N)], 0, 1e-6)
+    dev, hst, nadj = [], [], []
+    for it in range(args.iterate_steps + 1):
+        t0 = time.perf_counter()
+        cb = host.construct_canonical_basis(U, V)
+        ctx.set_state(U, V, cb["D"], cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"], M, M,
+                      FACPI, float(w.Z), float(w.N))
+        t1 = time.perf_counter()
+        r = ctx.iterate(0.0)
+        U, V = ctx.get_state()
+        t2 = time.perf_counter()
+        if it > 0:                  # the first call allocates and tunes
+            hst.append(t1 - t0)
+            dev.append(t2 - t1)
+            nadj.append(r["adjust_iterations"])
+    ctx.close()
+    per = np.mean(dev) + np.mean(hst)
+    return {"value": 1.0 / per, "unit": "iterations/s", "iterations": args.iterate_steps,
+            "device_ms": 1e3 * float(np.mean(dev)), "host_canonical_basis_ms": 1e3 * float(np.mean(hst)),
+            "adjust_steps": nadj, "constraints": "<Z>, <N>"}
+
+
 def cpu_baseline(args, ham, threads, UV):
     """Reference TAURUS_vap built from its own sources (oracle/_ref, FP64 TBMEs,
     OpenMP + threaded OpenBLAS) on the same deck, .red file and state: inside one
@@ -230,6 +331,10 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="replay the projection as a captured hipGraph (no per-kernel timing)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--batch-steps", type=int, default=5,
+                    help="timed steps of the 64 batched independent solves (0: skip)")
+    ap.add_argument("--iterate-steps", type=int, default=3,
+                    help="timed full device iterations with the constraint machinery (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (calibration "
                          "of the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -298,6 +403,24 @@ def main():
                      dE_vs_all_angles_keV=abs(cp["E"] - main_run["E"]) * 1e3)
         log("conjugate pairs: %.3f it/s, E = %.9f MeV" % (pairs["value"], cp["E"]))
 
+    batched = None
+    if args.batch_steps > 0:
+        try:
+            batched = batched_solves(args)
+            log("batched solves: %.1f solve-iterations/s (%.3f ms per step), |dE| fp32 vs fp64 "
+                "<= %.3e keV" % (batched["value"], batched["ms_per_step"],
+                                 batched["max_abs_dE_fp32_vs_fp64_keV"]))
+        except Exception as e:      # a failed side measurement must not hide the main one
+            log("batched solves failed: %r" % (e,))
+    devit = None
+    if args.iterate_steps > 0:
+        try:
+            devit = device_iteration(args, ham)
+            log("device iteration: %.3f it/s (device %.2f ms, host canonical basis %.2f ms)"
+                % (devit["value"], devit["device_ms"], devit["host_canonical_basis_ms"]))
+        except Exception as e:
+            log("device iteration failed: %r" % (e,))
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -337,6 +460,8 @@ def main():
         "dE_vs_fp64_keV": dE_keV,
         "E_proj_MeV": main_run["E"],
         "conj_pairs": pairs,
+        "batched_solves": batched,
+        "device_iteration": devit,
     }
     print(json.dumps(out), flush=True)
 

@@ -28,7 +28,6 @@
 namespace {
 
 constexpr int NT = 256;
-constexpr int NT1 = 1024;   // single-workgroup factorisations
 constexpr int MAXPAIRS = TG_MAX_CONS * (TG_MAX_CONS + 1) / 2;
 
 inline unsigned nblk(long long tot) { return (unsigned)((tot + NT - 1) / NT); }
@@ -111,60 +110,75 @@ __global__ void axpy_cons_kernel(long long nn, int ncons, const double* __restri
   Z[i] = s;
 }
 
-// In-place lower Cholesky factor of a symmetric positive matrix (column-major,
-// right-looking, one workgroup per matrix: blockIdx.x, stride n^2); the strict
-// upper triangle is zeroed.  info = 0, or the first column with a non-positive
-// pivot (1-based).
-__global__ void __launch_bounds__(NT1)
-cholesky_kernel(int n, double* __restrict__ A0, int* __restrict__ info) {
-  __shared__ int bad;
-  double* A = A0 + blockIdx.x * (long long)n * n;
-  const int t = threadIdx.x;
-  if (t == 0) bad = 0;
-  __syncthreads();
-  for (int k = 0; k < n; ++k) {
-    if (t == 0) {
-      const double d = A[k + (long long)k * n];
-      if (!(d > 0.0)) {
-        if (bad == 0) bad = k + 1;
-        A[k + (long long)k * n] = 1.0;
-      } else {
-        A[k + (long long)k * n] = sqrt(d);
-      }
-    }
-    __syncthreads();
-    const double dk = A[k + (long long)k * n];
-    for (int i = k + 1 + t; i < n; i += NT1) A[i + (long long)k * n] /= dk;
-    __syncthreads();
-    const int m = n - k - 1;
-    for (long long e = t; e < (long long)m * m; e += NT1) {
-      const int i = k + 1 + (int)(e % m), j = k + 1 + (int)(e / m);
-      if (i >= j) A[i + (long long)j * n] -= A[i + (long long)k * n] * A[j + (long long)k * n];
-    }
-    __syncthreads();
-  }
-  for (long long e = t; e < (long long)n * n; e += NT1) {
-    const int i = (int)(e % n), j = (int)(e / n);
-    if (i < j) A[e] = 0.0;
-  }
-  if (t == 0) info[blockIdx.x] = bad;
+__global__ void fill_kernel(long long len, double v, double* __restrict__ x) {
+  const long long i = (long long)blockIdx.x * NT + threadIdx.x;
+  if (i < len) x[i] = v;
 }
 
-// X = L^-1 for lower-triangular L (dtrtri 'L','N'): thread j solves column j;
-// one workgroup per matrix (blockIdx.x, stride n^2)
-__global__ void __launch_bounds__(NT1)
-trtri_lower_kernel(int n, const double* __restrict__ L0, double* __restrict__ X0) {
-  const double* L = L0 + blockIdx.x * (long long)n * n;
-  double* X = X0 + blockIdx.x * (long long)n * n;
-  for (int j = threadIdx.x; j < n; j += NT1) {
-    for (int i = 0; i < j; ++i) X[i + (long long)j * n] = 0.0;
-    X[j + (long long)j * n] = 1.0 / L[j + (long long)j * n];
-    for (int i = j + 1; i < n; ++i) {
-      double s = 0.0;
-      for (int k = j; k < i; ++k) s = fma(L[i + (long long)k * n], X[k + (long long)j * n], s);
-      X[i + (long long)j * n] = -s / L[i + (long long)i * n];
+__global__ void fill_int_kernel(int len, int v, int* __restrict__ x) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i < len) x[i] = v;
+}
+
+// Diagonal block of the blocked Cholesky factorisation (one wave per state,
+// blockIdx.x; matrices at + s n^2): factors the b x b block A(k0:, k0:) (lower
+// triangle read, b <= CB) in LDS, writes it to L (zeros above the diagonal)
+// and its inverse to X, which is the diagonal block of L^-1.  A non-positive
+// pivot is replaced by 1 and its column (1-based) recorded in info[s] if none
+// was before.
+constexpr int CB = 32;
+__global__ void __launch_bounds__(64)
+chol_block_kernel(int n, int k0, int b, const double* __restrict__ A0, double* __restrict__ L0,
+                  double* __restrict__ X0, int* __restrict__ info) {
+  __shared__ double a[CB][CB + 1], x[CB][CB + 1];
+  const long long nn = (long long)n * n;
+  const double* A = A0 + blockIdx.x * nn;
+  double* L = L0 + blockIdx.x * nn;
+  double* X = X0 + blockIdx.x * nn;
+  const int t = threadIdx.x;
+  for (int e = t; e < b * b; e += 64) {
+    const int i = e % b, j = e / b;
+    a[i][j] = (i >= j) ? A[(k0 + i) + (long long)(k0 + j) * n] : 0.0;
+  }
+  __syncthreads();
+  int bad = 0;
+  for (int kk = 0; kk < b; ++kk) {
+    double d = a[kk][kk];
+    if (!(d > 0.0)) {
+      if (bad == 0) bad = k0 + kk + 1;
+      d = 1.0;
+    }
+    d = sqrt(d);
+    __syncthreads();
+    if (t == 0) a[kk][kk] = d;
+    for (int i = kk + 1 + t; i < b; i += 64) a[i][kk] /= d;
+    __syncthreads();
+    const int m = b - kk - 1;
+    for (int e = t; e < m * m; e += 64) {
+      const int i = kk + 1 + e % m, j = kk + 1 + e / m;
+      if (i >= j) a[i][j] -= a[i][kk] * a[j][kk];
+    }
+    __syncthreads();
+  }
+  // inverse of the lower-triangular block: thread j solves column j
+  if (t < b) {
+    const int j = t;
+    for (int i = 0; i < j; ++i) x[i][j] = 0.0;
+    x[j][j] = 1.0 / a[j][j];
+    for (int i = j + 1; i < b; ++i) {
+      double sum = 0.0;
+      for (int k = j; k < i; ++k) sum = fma(a[i][k], x[k][j], sum);
+      x[i][j] = -sum / a[i][i];
     }
   }
+  __syncthreads();
+  for (int e = t; e < b * b; e += 64) {
+    const int i = e % b, j = e / b;
+    const long long o = (k0 + i) + (long long)(k0 + j) * n;
+    L[o] = (i >= j) ? a[i][j] : 0.0;
+    X[o] = x[i][j];
+  }
+  if (t == 0 && bad && info[blockIdx.x] == 0) info[blockIdx.x] = bad;
 }
 
 // Gaussian elimination with partial pivoting of a small dense system (host);
@@ -292,6 +306,43 @@ int gram(const Iter& it, const double* c20, const std::vector<int>& states,
   return TG_OK;
 }
 
+// cholesky_factorization + dtrtri of every state: A (n x n per state, destroyed)
+// = L L^T, X = L^-1 (dense, zero above the diagonal).  Right-looking blocked
+// Cholesky with CB-wide panels (diagonal block in LDS, panel and trailing
+// update as GEMMs), then the off-diagonal blocks of L^-1 row block by row
+// block: X(i, :i) = -X(i, i) L(i, :i) X(:i, :i).  L and T are scratch (T:
+// CB x n per state).  info[s] = 0 or the first non-positive pivot column.
+int cholesky_inverse(const Iter& it, double* A, double* L, double* X, double* T, int* info) {
+  const int n = it.n, S = it.S;
+  const long long nn = it.nn;
+  hipStream_t s = it.s;
+  hipLaunchKernelGGL(fill_kernel, dim3(nblk(S * nn)), dim3(NT), 0, s, S * nn, 0.0, L);
+  hipLaunchKernelGGL(fill_kernel, dim3(nblk(S * nn)), dim3(NT), 0, s, S * nn, 0.0, X);
+  hipLaunchKernelGGL(fill_int_kernel, dim3(nblk(S)), dim3(NT), 0, s, S, 0, info);
+  TG_HIP(hipGetLastError());
+  for (int k0 = 0; k0 < n; k0 += CB) {
+    const int b = std::min(CB, n - k0);
+    hipLaunchKernelGGL(chol_block_kernel, dim3(S), dim3(64), 0, s, n, k0, b, A, L, X, info);
+    TG_HIP(hipGetLastError());
+    const int m = n - k0 - b;
+    if (m == 0) break;
+    const long long pk = k0 + (long long)k0 * n, pb = (k0 + b) + (long long)k0 * n;
+    // panel L(k0+b:, k0:k0+b) = A(k0+b:, k0:k0+b) L(k0:k0+b, k0:k0+b)^-T
+    TG_TRY(gemm_dd(s, OP_N, OP_T, m, b, b, 1.0, A + pb, n, nn, X + pk, n, nn, 0.0, L + pb, n, nn, S));
+    // trailing update A(k0+b:, k0+b:) -= L_panel L_panel^T
+    TG_TRY(gemm_dd(s, OP_N, OP_T, m, m, b, -1.0, L + pb, n, nn, L + pb, n, nn, 1.0,
+                   A + (k0 + b) + (long long)(k0 + b) * n, n, nn, S));
+  }
+  for (int i0 = CB; i0 < n; i0 += CB) {
+    const int bi = std::min(CB, n - i0);
+    // T = L(i0:i0+bi, :i0) X(:i0, :i0);  X(i0:i0+bi, :i0) = -X(i0:, i0:) T
+    TG_TRY(gemm_dd(s, OP_N, OP_N, bi, i0, i0, 1.0, L + i0, n, nn, X, n, nn, 0.0, T, CB, nn, S));
+    TG_TRY(gemm_dd(s, OP_N, OP_N, bi, i0, bi, -1.0, X + i0 + (long long)i0 * n, n, nn, T, CB, nn, 0.0,
+                   X + i0, n, nn, S));
+  }
+  return TG_OK;
+}
+
 // Outputs of the last iteration of a context (per state)
 struct IterOut {
   std::vector<double> scal, gnorm, lambda;
@@ -411,10 +462,8 @@ int iterate_post(tg_ctx* c, double gradient_eps, IterOut& out) {
     // cholesky_factorization: L L^T = 1 + Z^T Z
     TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, n, 1.0, Z, n, nn, Z, n, nn, 0.0, it.M(IT_S), n, nn, S));
     hipLaunchKernelGGL(add_identity_kernel, dim3(nblk(n), S), dim3(NT), 0, s, n, it.M(IT_S));
-    hipLaunchKernelGGL(cholesky_kernel, dim3(S), dim3(NT1), 0, s, n, it.M(IT_S), dinfo);
-    hipLaunchKernelGGL(trtri_lower_kernel, dim3(S), dim3(NT1), 0, s, n, it.M(IT_S),
-                       it.M(IT_LINV));
     TG_HIP(hipGetLastError());
+    TG_TRY(cholesky_inverse(it, it.M(IT_S), it.M(IT_T2), it.M(IT_LINV), it.M(IT_X), dinfo));
     std::vector<int> info(S, 0);
     TG_HIP(hipMemcpyAsync(info.data(), dinfo, sizeof(int) * S, hipMemcpyDeviceToHost, s));
     // evolve_wavefunction: U1 = (U0 + V0 Z) L^-T, V1 = (V0 + U0 Z) L^-T
