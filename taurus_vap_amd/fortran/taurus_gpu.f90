@@ -116,6 +116,56 @@ interface
                                   constraint_20(*), lagrange_lambda1(*), gradient_Zim1(*)
     real(c_double), intent(out) :: gradient_Zi(*), gradient_norm
   end function
+
+  ! whole iterations on the device (loop body of src/taurus_vap.f90:77-113,
+  ! gradient_type 0): constraints once, then per iteration
+  ! tg_set_state -> tg_iterate -> tg_get_state
+  integer(c_int) function tg_set_state(ctx, U0, V0, D0, nocc0, nemp0, ovac0, voveru0, &
+                                       Mphip, Mphin, facpi, valence_Z, valence_N, &
+                                       factor_delta, iprint) bind(C, name='tg_set_state')
+    import
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nocc0, nemp0, Mphip, Mphin, iprint
+    real(c_double), intent(in) :: U0(*), V0(*), D0(*), voveru0(*)
+    real(c_double), value :: ovac0, facpi, valence_Z, valence_N, factor_delta
+  end function
+
+  integer(c_int) function tg_set_gradient(ctx, gradient_eta, gradient_mu, constraint_dim, &
+                                          constraint_20, lagrange_lambda1) &
+                                          bind(C, name='tg_set_gradient')
+    import
+    type(c_ptr), value :: ctx
+    real(c_double), value :: gradient_eta, gradient_mu
+    integer(c_int), value :: constraint_dim
+    real(c_double), intent(in) :: constraint_20(*), lagrange_lambda1(*)
+  end function
+
+  integer(c_int) function tg_set_constraints(ctx, constraint_dim, constraint_pair, &
+                                             constraint_HO, nsets, constraint_val, &
+                                             constraint_eps) bind(C, name='tg_set_constraints')
+    import
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: constraint_dim, constraint_pair, nsets
+    real(c_double), intent(in) :: constraint_HO(*), constraint_val(*)
+    real(c_double), value :: constraint_eps
+  end function
+
+  integer(c_int) function tg_iterate(ctx, gradient_eps, pnp_scalars, gradient_norm, &
+                                     lagrange_lambda1, adjust_iterations, converged) &
+                                     bind(C, name='tg_iterate')
+    import
+    type(c_ptr), value :: ctx
+    real(c_double), value :: gradient_eps
+    complex(c_double_complex), intent(out) :: pnp_scalars(*)
+    real(c_double), intent(out) :: gradient_norm(*), lagrange_lambda1(*)
+    integer(c_int), intent(out) :: adjust_iterations(*), converged(*)
+  end function
+
+  integer(c_int) function tg_get_state(ctx, U0, V0) bind(C, name='tg_get_state')
+    import
+    type(c_ptr), value :: ctx
+    real(c_double), intent(out) :: U0(*), V0(*)
+  end function
 end interface
 
 CONTAINS
