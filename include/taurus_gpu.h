@@ -245,6 +245,34 @@ int tg_iterate(tg_ctx* ctx, double gradient_eps, double* pnp_scalars, double* gr
 int tg_get_state(tg_ctx* ctx, double* U0, double* V0);
 
 /*
+ * tg_project_observables -- the gauge-angle loop of project_wavefunction(1, 0,
+ * Mphip, Mphin, ndim) (src/module_projection.f90:236-356, the final printing
+ * projection of print_results :990-1062) for the resident state (tg_set_state,
+ * one state, batched mode): every angle of the grid, with the parity overlap.
+ * Per angle, in the reference's angle order, a record of R = 19 + 2*nshell +
+ * 2*nobo + 4*nobos complex values:
+ *   [0..6]   rot_over, rot_pari, rot_prot, rot_neut, rot_prot2, rot_neut2, rot_ener
+ *   [7..18]  rot_ecomp (calculate_decompo_energy, module_hamiltonian.f90:1134-1260)
+ *   then     rot_occnum per shell s: (s, p), (s, n) (calculate_occupation_number,
+ *            module_particlenumber.f90:157-176; sp_shell[i] = 0-based shell of
+ *            state i, the first ndim/2 states being protons)
+ *   then     for each real one-body operator k of obo_ops (nobo x ndim x ndim):
+ *            the proton and neutron parts of calculate_expectval_obo_cplx
+ *            (module_operators.f90:146-164), without the operator's scalar factor
+ *   then     for each pair (O, O2) of obos_ops (nobos x 2 x ndim x ndim): the traces
+ *            T1 = Tr(O rho), T2 = Tr(O2 rho), T5 = Tr(O rho O rho),
+ *            T6 = Tr(O^T kappaRL^T O kappaLR) of calculate_expectval_obos_cplx
+ *            (:227-257); for factors c1 O, c2 O2 the expectation value is
+ *            c1^2 T1^2 + c2 T2 - c1^2 T5 + c1^2 T6.
+ * pnp_rho (complex ndim x ndim, may be NULL): sum over angles of the
+ * weight x overlap x rho_LR term of sum_gauge (:425-458).  The caller
+ * accumulates the other pnp_* with the reference's sum_gauge.
+ */
+int tg_project_observables(tg_ctx* ctx, int nobo, const double* obo_ops, int nobos,
+                           const double* obos_ops, int nshell, const int* sp_shell,
+                           double* per_angle, double* pnp_rho);
+
+/*
  * Groups: the projections of several contexts (each with its own basis size,
  * Hamiltonian and states) issued together as one hipGraph launch per call
  * (use_graph = 1: captured the second time a configuration is seen, replayed

@@ -255,6 +255,7 @@ struct tg_ctx {
   int cons_nsets = 1;
   DevBuf d_cHO;                                   // constraint operators, internal order
   DevBuf d_it;                                    // iteration work (n*n real matrices)
+  DevBuf d_obs;                                   // observables of the final projection
   std::vector<double> lambda_prev;                // multipliers of the previous iteration (ns x dim)
   // timing: event pairs recorded without synchronisation, summed at collection
   struct Pending { std::string key; hipEvent_t e0, e1; };

@@ -494,6 +494,144 @@ __global__ void scale_det_kernel(int na, double s, const cplx* __restrict__ det,
   if (a < na) pre[a] = cscale(det[a], s);
 }
 
+// ---- observables of the final projection (project_wavefunction iopt = 1)
+
+// block-wide sum of K complex accumulators into out[0..K) (NT threads)
+template <int K>
+__device__ void block_sum(cplx (&acc)[K], cplx* __restrict__ out) {
+  __shared__ cplx red[NT];
+  for (int k = 0; k < K; ++k) {
+    red[threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0];
+    __syncthreads();
+  }
+}
+
+// calculate_decompo_energy (module_hamiltonian.f90:1134-1260) of angle blockIdx.x:
+// out[12] = 1b_p, 1b_n, PH_pp, PH_pn, PH_np, PH_nn, PP_pp, PP_pn, PP_np, PP_nn,
+// E_p, E_n with PH_xy = 1/2 Tr(Gamma_xy rho_yx), PP_xy = 1/2 Tr(Delta_xy kRL_xy^T)
+// (x, y = the proton / neutron halves of the single-particle index)
+__global__ void __launch_bounds__(NT)
+ecomp_kernel(int n, const cplx* __restrict__ H1, const cplx* __restrict__ G0,
+             const cplx* __restrict__ rho0, const cplx* __restrict__ dLR0,
+             const cplx* __restrict__ kRL0, cplx* __restrict__ out) {
+  const long long nn = (long long)n * n, a = blockIdx.x;
+  const cplx* G = G0 + a * nn;
+  const cplx* rho = rho0 + a * nn;
+  const cplx* dLR = dLR0 + a * nn;
+  const cplx* kRL = kRL0 + a * nn;
+  const int h = n / 2;
+  cplx acc[10];   // OB_p, OB_n, PH[2][2], PP[2][2]
+  for (int k = 0; k < 10; ++k) acc[k] = cmk(0.0, 0.0);
+  for (long long idx = threadIdx.x; idx < nn; idx += NT) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    const int bi = i >= h, bj = j >= h;
+    const cplx rji = rho[j + (long long)i * n];
+    cfma(acc[bi], H1[idx], rji);
+    cfma(acc[2 + 2 * bi + bj], G[idx], rji);
+    cfma(acc[6 + 2 * bi + bj], dLR[idx], kRL[idx]);
+  }
+  __shared__ cplx res[10];
+  block_sum<10>(acc, res);
+  if (threadIdx.x == 0) {
+    cplx* o = out + a * 12;
+    o[0] = res[0];
+    o[1] = res[1];
+    for (int k = 0; k < 4; ++k) o[2 + k] = cscale(res[2 + k], 0.5);
+    for (int k = 0; k < 4; ++k) o[6 + k] = cscale(res[6 + k], 0.5);
+    o[10] = cadd(res[0], cscale(cadd(cadd(res[2], res[3]), cadd(res[6], res[7])), 0.5));
+    o[11] = cadd(res[1], cscale(cadd(cadd(res[4], res[5]), cadd(res[8], res[9])), 0.5));
+  }
+}
+
+// calculate_occupation_number (module_particlenumber.f90:157-176): diagonal of
+// rho summed per shell and isospin.  shell[i], iso[i] of each (internal) state;
+// out[a * 2 * nsh + 2 * s + t]
+__global__ void occnum_kernel(int n, int nsh, const int* __restrict__ shell,
+                              const int* __restrict__ iso, const cplx* __restrict__ rho0,
+                              cplx* __restrict__ out) {
+  const long long nn = (long long)n * n, a = blockIdx.x;
+  const cplx* rho = rho0 + a * nn;
+  for (int q = threadIdx.x; q < 2 * nsh; q += NT) {
+    const int sh = q / 2, t = q % 2;
+    cplx sum = cmk(0.0, 0.0);
+    for (int i = 0; i < n; ++i)
+      if (shell[i] == sh && iso[i] == t) sum = cadd(sum, rho[i + (long long)i * n]);
+    out[a * 2 * nsh + q] = sum;
+  }
+}
+
+// calculate_expectval_obo_cplx (module_operators.f90:146-164) for real operator
+// blockIdx.y of angle blockIdx.x: out = (sum_{i in p} (O rho)_ii, sum_{i in n} (O rho)_ii)
+__global__ void __launch_bounds__(NT)
+obo_trace_kernel(int n, const double* __restrict__ ops, const cplx* __restrict__ rho0,
+                 cplx* __restrict__ out) {
+  const long long nn = (long long)n * n, a = blockIdx.x, k = blockIdx.y;
+  const double* O = ops + k * nn;
+  const cplx* rho = rho0 + a * nn;
+  const int h = n / 2;
+  cplx acc[2] = {cmk(0.0, 0.0), cmk(0.0, 0.0)};
+  for (long long idx = threadIdx.x; idx < nn; idx += NT) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    const cplx r = rho[j + (long long)i * n];
+    const double o = O[idx];
+    acc[i >= h].x = fma(o, r.x, acc[i >= h].x);
+    acc[i >= h].y = fma(o, r.y, acc[i >= h].y);
+  }
+  __shared__ cplx res[2];
+  block_sum<2>(acc, res);
+  if (threadIdx.x == 0) {
+    out[(a * gridDim.y + k) * 2] = res[0];
+    out[(a * gridDim.y + k) * 2 + 1] = res[1];
+  }
+}
+
+// traces of calculate_expectval_obos_cplx (module_operators.f90:227-257) for a
+// real operator O (scalar factors applied by the caller), angle blockIdx.x:
+//   T1 = Tr(O rho), T2 = Tr(O2 rho), T5 = Tr(O rho O rho) = sum (O rho)_ij (O rho)_ji,
+//   T6 = Tr(O^T kRL^T O kLR) = sum_ij (kRL O)_ij (O kLR)_ij
+// with X = O rho, Y = kRL O, Z = O kLR per angle.
+__global__ void __launch_bounds__(NT)
+obos_trace_kernel(int n, const cplx* __restrict__ X0, const cplx* __restrict__ Y0,
+                  const cplx* __restrict__ Z0, const double* __restrict__ O2,
+                  const cplx* __restrict__ rho0, cplx* __restrict__ out) {
+  const long long nn = (long long)n * n, a = blockIdx.x;
+  const cplx* X = X0 + a * nn;
+  const cplx* Y = Y0 + a * nn;
+  const cplx* Z = Z0 + a * nn;
+  const cplx* rho = rho0 + a * nn;
+  cplx acc[4];
+  for (int k = 0; k < 4; ++k) acc[k] = cmk(0.0, 0.0);
+  for (long long idx = threadIdx.x; idx < nn; idx += NT) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    const long long t = j + (long long)i * n;
+    if (i == j) acc[0] = cadd(acc[0], X[idx]);
+    const cplx r = rho[t];
+    acc[1].x = fma(O2[idx], r.x, acc[1].x);
+    acc[1].y = fma(O2[idx], r.y, acc[1].y);
+    cfma(acc[2], X[idx], X[t]);
+    cfma(acc[3], Y[idx], Z[idx]);
+  }
+  __shared__ cplx res[4];
+  block_sum<4>(acc, res);
+  if (threadIdx.x < 4) out[a * 4 + threadIdx.x] = res[threadIdx.x];
+}
+
+// pnp_rho of sum_gauge (iopt = 1): sum_a f1_a rho_a (f1 = weight x overlap)
+__global__ void rho_sum_kernel(long long nn, int na, const cplx* __restrict__ f12,
+                               const cplx* __restrict__ rho, cplx* __restrict__ out) {
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= nn) return;
+  cplx s = cmk(0.0, 0.0);
+  for (int a = 0; a < na; ++a) cfma(s, rho[a * nn + idx], f12[2 * a]);
+  out[idx] = s;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1116,6 +1254,8 @@ void permute_out(const tg_ctx* c, const cplx* src, cplx* dst) {
       dst[c->sp_perm[i] + (size_t)c->sp_perm[j] * n] = src[i + (size_t)j * n];
 }
 
+void assign_angles(tg_ctx* c, bool conj);
+
 // S states of this basis (state k: U0 + k n^2, V0 + k n^2, D0 + k n^2,
 // voveru0 + k n holding n - nocc0[k] values) and the common projection set-up.
 int upload_states(tg_ctx* c, int S, const double* U0, const double* V0, const double* D0,
@@ -1175,12 +1315,22 @@ int upload_states(tg_ctx* c, int S, const double* U0, const double* V0, const do
   c->factor_delta = factor_delta;
   c->iprint = iprint ? 1 : 0;
   if (c->grad_on) TG_TRY(tg_ensure_z(c));
-  // assign_angles (:112-185).  With TG_FLAG_CONJ_PAIRS and the full [0, 2 pi)
-  // grid, the grid is closed under phi -> -phi (mod 2 pi): for real U, V every
-  // per-angle quantity at -phi is the complex conjugate of that at phi, and so
-  // is the weight, so only one angle of each pair is computed and enters the
-  // gauge sums twice, as 2 Re(.); self-conjugate angles enter once.
-  const bool pairs = c->conj_pairs && Mphip * Mphin > 1 &&
+  assign_angles(c, c->conj_pairs != 0);
+  c->have_state = 1;
+  c->state_dirty = 1;
+  return TG_OK;
+}
+
+// assign_angles (:112-185) for the context's Mphip, Mphin, facpi and valence
+// numbers.  With `conj` (TG_FLAG_CONJ_PAIRS) and the full [0, 2 pi) grid, the
+// grid is closed under phi -> -phi (mod 2 pi): for real U, V every per-angle
+// quantity at -phi is the complex conjugate of that at phi, and so is the
+// weight, so only one angle of each pair is computed and enters the gauge sums
+// twice, as 2 Re(.); self-conjugate angles enter once.
+void assign_angles(tg_ctx* c, bool conj) {
+  const int Mphip = c->Mphip, Mphin = c->Mphin;
+  const double facpi = c->facpi, valZ = c->valZ, valN = c->valN;
+  const bool pairs = conj && Mphip * Mphin > 1 &&
                      fabs(facpi - 2.0 * M_PI) <= 1e-12 * 2.0 * M_PI;
   auto partner = [](int j, int M) { return (M % 2 == 1) ? (j == 1 ? 1 : M - j + 2) : M - j + 1; };
   c->phip.clear();
@@ -1206,9 +1356,6 @@ int upload_states(tg_ctx* c, int S, const double* U0, const double* V0, const do
       c->mult.push_back(mult);
     }
   c->A = (int)c->phip.size();
-  c->have_state = 1;
-  c->state_dirty = 1;
-  return TG_OK;
 }
 
 int fetch_results(tg_ctx* c, int state, double* scal, double* pA, double* pH, double* pAH,
@@ -1395,7 +1542,7 @@ int tg_finalize(tg_ctx* c) {
   graph_release(c);
   for (DevBuf* b : {&c->d_H1, &c->d_U, &c->d_V, &c->d_D, &c->d_ang, &c->d_real6, &c->d_work,
                     &c->d_blk, &c->d_scal, &c->d_out, &c->d_ovl, &c->d_int, &c->d_pfw, &c->d_c20,
-                    &c->d_Z, &c->d_psign, &c->d_cHO, &c->d_it})
+                    &c->d_Z, &c->d_psign, &c->d_cHO, &c->d_it, &c->d_obs})
     b->release();
   for (Channel* ch : {&c->ph, &c->pp})
     for (DevBuf* b : {&ch->d_perm, &ch->d_groups, &ch->d_W, &ch->d_tile_group, &ch->d_tile_n0})
@@ -1808,6 +1955,135 @@ int tg_set_gradient(tg_ctx* c, double gradient_eta, double gradient_mu, int cons
 
 int tg_get_gradient(tg_ctx* c, double* gradient_Zi, double* gradient_norm) {
   return tg_get_gradient_state(c, 0, gradient_Zi, gradient_norm);
+}
+
+int tg_project_observables(tg_ctx* c, int nobo, const double* obo_ops, int nobos,
+                           const double* obos_ops, int nshell, const int* sp_shell,
+                           double* per_angle, double* pnp_rho) {
+  if (!c || !c->have_state || c->plain || c->ns != 1 || nobo < 0 || nobos < 0 || nshell < 0 ||
+      (nobo > 0 && !obo_ops) || (nobos > 0 && !obos_ops) || (nshell > 0 && !sp_shell) ||
+      !per_angle) {
+    tg_set_error("tg_project_observables: invalid argument (one resident state, batched mode)");
+    return TG_E_ARG;
+  }
+  const int n = c->n, h = n / 2;
+  const long long nn = (long long)n * n;
+  for (int i = 0; i < n && nshell > 0; ++i)
+    if (sp_shell[i] < 0 || sp_shell[i] >= nshell) {
+      tg_set_error("tg_project_observables: shell index out of range");
+      return TG_E_ARG;
+    }
+  // every angle of the grid (no pairing of conjugate angles) and the parity
+  // overlaps (iopt + iprint > 0, :279-284)
+  if (c->A != c->Mphip * c->Mphin) assign_angles(c, false);
+  c->iprint = 1;
+  c->state_dirty = 1;
+  const int grad_was = c->grad_on;
+  c->grad_on = 0;
+  const int r = project_device(c);
+  c->grad_on = grad_was;
+  TG_TRY(r);
+  const RunPlan P = make_plan(c);
+  const int A = P.A;
+  hipStream_t s = c->stream;
+  // device layout: ops (nobo n^2) | obos (2 nobos n^2) | ecomp (12 A) | occ (2 nshell A)
+  //               | obo (2 nobo A) | obos (4 nobos A) | rho sum (n^2) [complex] | shell, iso (int)
+  const size_t nd_ops = (size_t)(nobo + 2 * nobos) * nn;
+  const size_t nc_out = (size_t)A * (12 + 2 * nshell + 2 * nobo + 4 * nobos) + nn;
+  TG_TRY(c->d_obs.ensure(sizeof(double) * nd_ops + sizeof(cplx) * nc_out + sizeof(int) * 2 * n + 256));
+  double* d_ops = c->d_obs.as<double>();
+  double* d_obos = d_ops + (size_t)nobo * nn;
+  cplx* d_ec = reinterpret_cast<cplx*>(d_ops + nd_ops);
+  cplx* d_occ = d_ec + 12 * (size_t)A;
+  cplx* d_obo = d_occ + 2 * (size_t)nshell * A;
+  cplx* d_obs2 = d_obo + 2 * (size_t)nobo * A;
+  cplx* d_rho = d_obs2 + 4 * (size_t)nobos * A;
+  int* d_shell = reinterpret_cast<int*>(d_rho + nn);
+  int* d_iso = d_shell + n;
+  {
+    std::vector<double> hops(nd_ops);
+    auto perm_op = [&](const double* src, double* dst) {
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i)
+          dst[i + (size_t)j * n] = src[c->sp_perm[i] + (size_t)c->sp_perm[j] * n];
+    };
+    for (int k = 0; k < nobo; ++k) perm_op(obo_ops + k * nn, hops.data() + k * nn);
+    for (int k = 0; k < 2 * nobos; ++k) perm_op(obos_ops + k * nn, hops.data() + (nobo + k) * nn);
+    std::vector<int> hint(2 * n, 0);
+    for (int i = 0; i < n; ++i) {
+      hint[i] = nshell > 0 ? sp_shell[c->sp_perm[i]] : 0;
+      hint[n + i] = c->sp_perm[i] >= h ? 1 : 0;
+    }
+    if (nd_ops) TG_HIP(hipMemcpyAsync(d_ops, hops.data(), sizeof(double) * nd_ops,
+                                      hipMemcpyHostToDevice, s));
+    TG_HIP(hipMemcpyAsync(d_shell, hint.data(), sizeof(int) * 2 * n, hipMemcpyHostToDevice, s));
+    TG_HIP(hipStreamSynchronize(s));
+  }
+  auto Wb = [&](int k) { return P.work + (size_t)k * P.chunk * nn; };
+  const cplx* rho = Wb(W_RHO);
+  const cplx* kLR = Wb(W_KLR);
+  const cplx* kRL = Wb(W_KRL);
+  const cplx one = cmk(1, 0), zero = cmk(0, 0);
+  hipLaunchKernelGGL(ecomp_kernel, dim3(A), dim3(NT), 0, s, n, c->d_H1.as<cplx>(), Wb(W_G), rho,
+                     Wb(W_DLR), kRL, d_ec);
+  if (nshell > 0)
+    hipLaunchKernelGGL(occnum_kernel, dim3(A), dim3(NT), 0, s, n, nshell, d_shell, d_iso, rho, d_occ);
+  if (nobo > 0)
+    hipLaunchKernelGGL(obo_trace_kernel, dim3(A, nobo), dim3(NT), 0, s, n, d_ops, rho, d_obo);
+  TG_HIP(hipGetLastError());
+  for (int k = 0; k < nobos; ++k) {
+    const double* O = d_obos + (size_t)2 * k * nn;
+    const double* O2 = O + nn;
+    cplx* X = Wb(W_X);
+    cplx* Y = Wb(W_Y);
+    cplx* Z = Wb(W_Z);
+    TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, O, n, 0, rho, n, nn, zero, X, n, nn, A));
+    TG_TRY(gemm_zd(s, OP_N, OP_N, n, n, n, one, kRL, n, nn, O, n, 0, zero, Y, n, nn, A));
+    TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, O, n, 0, kLR, n, nn, zero, Z, n, nn, A));
+    hipLaunchKernelGGL(obos_trace_kernel, dim3(A), dim3(NT), 0, s, n, X, Y, Z, O2, rho,
+                       d_obs2 + (size_t)4 * k * A);
+    TG_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(rho_sum_kernel, dim3(nblk(nn)), dim3(NT), 0, s, nn, A, P.d_f12, rho, d_rho);
+  TG_HIP(hipGetLastError());
+  TG_HIP(hipStreamSynchronize(s));
+  // per-angle records
+  std::vector<cplx> ov(A), pa(A), tr(9 * (size_t)A), en(A), out(nc_out);
+  TG_HIP(hipMemcpy(ov.data(), P.d_over, sizeof(cplx) * A, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(pa.data(), P.d_pari, sizeof(cplx) * A, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(tr.data(), P.d_tr, sizeof(cplx) * 9 * A, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(en.data(), P.d_ener, sizeof(cplx) * A, hipMemcpyDeviceToHost));
+  TG_HIP(hipMemcpy(out.data(), d_ec, sizeof(cplx) * nc_out, hipMemcpyDeviceToHost));
+  const size_t R = 19 + 2 * (size_t)nshell + 2 * (size_t)nobo + 4 * (size_t)nobos;
+  cplx* rec = reinterpret_cast<cplx*>(per_angle);
+  const cplx* o_ec = out.data();
+  const cplx* o_occ = o_ec + 12 * (size_t)A;
+  const cplx* o_obo = o_occ + 2 * (size_t)nshell * A;
+  const cplx* o_obs = o_obo + 2 * (size_t)nobo * A;
+  for (int a = 0; a < A; ++a) {
+    cplx* q = rec + R * a;
+    const cplx* t = tr.data() + 9 * (size_t)a;
+    q[0] = ov[a];
+    q[1] = pa[a];
+    q[2] = t[0];
+    q[3] = t[1];
+    // calculate_particle_number (module_particlenumber.f90:78-143): <Z^2>, <N^2>
+    q[4] = csub(cadd(t[0], cmul(t[0], t[0])), csub(t[2], t[3]));
+    q[5] = csub(cadd(t[1], cmul(t[1], t[1])), csub(t[4], t[5]));
+    q[6] = en[a];
+    for (int k = 0; k < 12; ++k) q[7 + k] = o_ec[12 * (size_t)a + k];
+    for (int k = 0; k < 2 * nshell; ++k) q[19 + k] = o_occ[2 * (size_t)nshell * a + k];
+    for (int k = 0; k < 2 * nobo; ++k) q[19 + 2 * nshell + k] = o_obo[2 * (size_t)nobo * a + k];
+    for (int k = 0; k < nobos; ++k)
+      for (int m = 0; m < 4; ++m)
+        q[19 + 2 * nshell + 2 * nobo + 4 * k + m] = o_obs[(size_t)4 * k * A + 4 * a + m];
+  }
+  if (pnp_rho) {
+    std::vector<cplx> hr(nn);
+    for (long long i = 0; i < nn; ++i) hr[i] = out[nc_out - nn + i];
+    permute_out(c, hr.data(), reinterpret_cast<cplx*>(pnp_rho));
+  }
+  return TG_OK;
 }
 
 int tg_get_gradient_state(tg_ctx* c, int state, double* gradient_Zi, double* gradient_norm) {

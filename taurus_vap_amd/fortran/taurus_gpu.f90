@@ -22,6 +22,7 @@ integer(c_int), parameter :: TG_OK = 0, TG_E_IMAG_GRADIENT = -5, &
                              TG_FLAG_PLAIN = 1, TG_FLAG_TBME_FP32 = 2, &
                              TG_FLAG_GRAPH = 4, TG_FLAG_CONJ_PAIRS = 8
 type(c_ptr) :: taurus_gpu_ctx = c_null_ptr
+integer(c_int) :: taurus_gpu_flags = 0
 
 interface
   function tg_last_error() bind(C, name='tg_last_error') result(p)
@@ -166,6 +167,18 @@ interface
     type(c_ptr), value :: ctx
     real(c_double), intent(out) :: U0(*), V0(*)
   end function
+
+  ! per-angle observables of project_wavefunction(1, ...) (final printing)
+  integer(c_int) function tg_project_observables(ctx, nobo, obo_ops, nobos, obos_ops, &
+                                                 nshell, sp_shell, per_angle, pnp_rho) &
+                                                 bind(C, name='tg_project_observables')
+    import
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nobo, nobos, nshell
+    real(c_double), intent(in) :: obo_ops(*), obos_ops(*)
+    integer(c_int), intent(in) :: sp_shell(*)
+    complex(c_double_complex), intent(out) :: per_angle(*), pnp_rho(*)
+  end function
 end interface
 
 CONTAINS
@@ -214,6 +227,7 @@ subroutine taurus_gpu_setup(ndim)
   endif
   h2b = int(storage_size(hamil_H2) / 8, c_int)
   if ( h2b == 4 ) flags = ior(flags, TG_FLAG_TBME_FP32)
+  taurus_gpu_flags = flags
   status = tg_init(taurus_gpu_ctx, int(ndim,c_int), int(HOsp_2mj,c_int32_t), &
                    int(HOsp_2mt,c_int32_t), int(HOsp_l,c_int32_t), int(HOsp_tr,c_int32_t), &
                    real(hamil_H0,c_double), aimag(hamil_H0), hamil_H1, &

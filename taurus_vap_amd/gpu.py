@@ -38,7 +38,7 @@ EXPORTS = [
     "tg_enable_timing", "tg_reset_timing", "tg_get_timing", "tg_set_constraints",
     "tg_iterate", "tg_get_state", "tg_set_states", "tg_get_results_state",
     "tg_get_gradient_state", "tg_group_create", "tg_group_destroy", "tg_group_project",
-    "tg_group_iterate", "tg_group_iterate_results",
+    "tg_group_iterate", "tg_group_iterate_results", "tg_project_observables",
 ]
 
 _lib = None
@@ -89,6 +89,7 @@ def load_library():
     lib.tg_group_project.argtypes = [P]
     lib.tg_group_iterate.argtypes = [P, D]
     lib.tg_group_iterate_results.argtypes = [P, I, P, P, P, P, P]
+    lib.tg_project_observables.argtypes = [P, I, P, I, P, I, P, P, P]
     _lib = lib
     return lib
 
@@ -200,6 +201,7 @@ class Context:
             vou = np.zeros(1)
         self._state_keep = (U0, V0, D0, vou)
         self.nstates = 1
+        self._angles = int(Mphip) * int(Mphin)
         _check(load_library().tg_set_state(self._h, _p(U0), _p(V0), _p(D0), int(nocc0), int(nemp0),
                                            float(ovac0), _p(vou), int(Mphip), int(Mphin),
                                            float(facpi), float(valence_Z), float(valence_N),
@@ -324,6 +326,27 @@ class Context:
         Us = [U[k * n * n:(k + 1) * n * n].reshape((n, n), order="F") for k in range(S)]
         Vs = [V[k * n * n:(k + 1) * n * n].reshape((n, n), order="F") for k in range(S)]
         return (Us[0], Vs[0]) if S == 1 else (Us, Vs)
+
+    def project_observables(self, obo_ops=(), obos_pairs=(), sp_shell=None, nshell=0):
+        """Per-angle observables of the final projection (project_wavefunction
+        iopt = 1) of the resident state: returns (records A x R complex, pnp_rho)
+        with the record layout of tg_project_observables."""
+        n = self.n
+        obo = np.ascontiguousarray(np.asarray([np.asarray(o, float).reshape(n, n).T for o in obo_ops]
+                                              or np.zeros((1, n, n))))
+        obs = np.ascontiguousarray(np.asarray([[np.asarray(o, float).reshape(n, n).T,
+                                                 np.asarray(o2, float).reshape(n, n).T]
+                                                for o, o2 in obos_pairs] or np.zeros((1, 2, n, n))))
+        shell = np.ascontiguousarray(np.asarray(sp_shell if sp_shell is not None else np.zeros(n),
+                                                dtype=np.int32))
+        A = self._angles
+        R = 19 + 2 * nshell + 2 * len(obo_ops) + 4 * len(obos_pairs)
+        rec = np.zeros((A, R), np.complex128)
+        rho = np.zeros((n, n), np.complex128, order="F")
+        _check(load_library().tg_project_observables(self._h, len(obo_ops), _p(obo), len(obos_pairs),
+                                                     _p(obs), int(nshell), _p(shell), _p(rec),
+                                                     _p(rho)), "tg_project_observables")
+        return rec, rho
 
     def enable_timing(self, on=True):
         _check(load_library().tg_enable_timing(self._h, 1 if on else 0), "tg_enable_timing")

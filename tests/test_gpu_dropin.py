@@ -130,3 +130,73 @@ def test_dropin_mg24_hfb_published():
     print("drop-in HFB: %d iterations, E = %.6f (published %d, %.6f)" % (rows[-1][0], e, it_pub, e_pub))
     assert abs(e - e_pub) <= 1.5e-6
     assert abs(rows[-1][0] - it_pub) <= 10
+
+
+REF_EXE = os.path.join(ROOT, "oracle", "_ref", "taurus_vap32.exe")
+
+
+def _run_exe(exe: str, M: int, iters: int, extra_env=None):
+    """stdout and occupation_numbers.dat of a run of `exe` on the Mg24 deck."""
+    with tempfile.TemporaryDirectory() as td:
+        for f in ("usdb.sho", "initial_wf.txt"):
+            with open(os.path.join(MG24, f)) as src, open(os.path.join(td, f), "w") as dst:
+                dst.write(src.read())
+        deck = open(os.path.join(MG24, "template_input.txt")).read()
+        deck = (deck.replace("ETA", "000").replace("MUU", "000").replace("XGGG", str(M))
+                .replace("XO", "1"))
+        deck = deck.replace("Maximum no. of iterations     3000",
+                            "Maximum no. of iterations     %4d" % iters)
+        with open(os.path.join(td, "input.txt"), "w") as f:
+            f.write(deck)
+        env = dict(os.environ, OMP_NUM_THREADS="4")
+        env.update(extra_env or {})
+        with open(os.path.join(td, "input.txt")) as fin:
+            r = subprocess.run([exe], stdin=fin, capture_output=True, text=True, cwd=td, env=env,
+                               timeout=600)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        occ = open(os.path.join(td, "occupation_numbers.dat")).read()
+    return r.stdout, occ
+
+
+def _numbers(text: str):
+    """(label, numbers) of every line that carries numbers."""
+    rows = []
+    for line in text.splitlines():
+        toks = line.split()
+        nums, words = [], []
+        for t in toks:
+            try:
+                nums.append(float(t))
+            except ValueError:
+                words.append(t)
+        if nums:
+            rows.append((" ".join(words), nums))
+    return rows
+
+
+@pytest.mark.parametrize("M", [5, 1])
+def test_dropin_final_printout_matches_reference(M):
+    """print_results (projected and quasiparticle-state tables: energy
+    decomposition, multipoles, radii, angular momentum, parity, particle
+    numbers) and occupation_numbers.dat of the drop-in, whose final projection
+    runs on the GPU (tg_project_observables), against the reference's own CPU
+    build (oracle/_ref/taurus_vap32.exe) on the same deck after 3 iterations.
+    Numbers agree to the printed precision (2e-6 absolute, 2e-6 relative)."""
+    if not os.path.exists(EXE) or not os.path.exists(REF_EXE):
+        pytest.skip("integration build or reference build not present")
+    out_gpu, occ_gpu = _run_exe(EXE, M, 3)
+    out_ref, occ_ref = _run_exe(REF_EXE, M, 3)
+    checked = 0
+    start = "PROJECTED STATE PROPERTIES" if M > 1 else "QUASIPARTICLE STATE PROPERTIES"
+    tables = lambda out: out[out.index(start):out.index("COMPLEMENTARY FILES")]
+    for text_g, text_r, min_rows in ((tables(out_gpu), tables(out_ref), 30), (occ_gpu, occ_ref, 4)):
+        rg, rr = _numbers(text_g), _numbers(text_r)
+        assert len(rg) == len(rr) and len(rg) >= min_rows, (len(rg), len(rr))
+        for (lg, ng), (lr, nr) in zip(rg, rr):
+            if lg.startswith("Label"):
+                continue
+            assert lg == lr and len(ng) == len(nr), (lg, lr)
+            for a, b in zip(ng, nr):
+                assert abs(a - b) <= 2e-6 + 2e-6 * abs(b), (lg, ng, nr)
+                checked += 1
+    print("final printout: %d numbers agree" % checked)
