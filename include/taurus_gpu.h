@@ -74,6 +74,17 @@ int tg_finalize(tg_ctx* ctx);
 int tg_set_mode(tg_ctx* ctx, int plain);
 
 /*
+ * tg_set_batch_limit -- at most max_items (state, angle) items per launch
+ * sequence of the batched projection (0: no limit, the default).  Independently
+ * of it, when the per-item work matrices of all items do not fit into the free
+ * device memory, the projection runs the items chunk by chunk and keeps only
+ * A_phi and H20_phi of every item for sum_gauge (the memory-short mode).  With
+ * several states a chunk holds whole states.  tg_project_observables needs all
+ * angles at once and returns TG_E_NOMEM otherwise.
+ */
+int tg_set_batch_limit(tg_ctx* ctx, int max_items);
+
+/*
  * tg_calculate_fields -- calculate_fields(rhoLR,kappaLR,kappaRL,gammaLR,hspLR,
  * deltaLR,deltaRL,ndim), src/module_fields.f90:99-189.  All complex ndim x ndim.
  */

@@ -232,6 +232,9 @@ struct tg_ctx {
   DevBuf d_pfw;                                   // pfaffian work
   int have_state = 0;
   int tuned_n = 0, tuned_chunk = 0, tuned_nb = -1;  // sizes the GEMM tiles were tuned for
+  // items per launch sequence: batch_limit from tg_set_batch_limit (0: none),
+  // mem_chunk chosen from the free device memory by the upload (0: all items)
+  int batch_limit = 0, mem_chunk = 0;
   int state_dirty = 0;                            // set-up of a new state not uploaded yet
   // hipGraph replay of the projection (TG_FLAG_GRAPH)
   int use_graph = 0, graph_valid = 0, graph_seen = 0;

@@ -608,6 +608,8 @@ GraphKey ctx_key(tg_ctx* c) {
   std::memcpy(&k.v[12], d, sizeof(d));
   k.v[15] = c->plain;
   k.v[16] = c->w_fp32;
+  k.v[17] = c->batch_limit;
+  k.v[18] = c->mem_chunk;
   return k;
 }
 

@@ -758,6 +758,19 @@ struct RunPlan {
   size_t work_bytes, scal_bytes, int_bytes, ovl_bytes, pfw_bytes;
 };
 
+// Items per launch sequence.  All items at once unless a limit applies (the
+// caller's batch limit, or the free-memory chunk of the upload).  A chunk holds
+// whole states when the context has several, so that every launch's per-state
+// operands are runs of whole states; with one state any chunk size works.
+int chunk_items(const tg_ctx* c, int B, int A) {
+  int lim = B;
+  if (c->batch_limit > 0) lim = std::min(lim, c->batch_limit);
+  if (c->mem_chunk > 0) lim = std::min(lim, c->mem_chunk);
+  if (lim >= B) return B;
+  if (c->ns > 1) return std::max(1, lim / A) * A;
+  return std::max(1, lim);
+}
+
 RunPlan make_plan(tg_ctx* c) {
   RunPlan P{};
   P.n = c->n;
@@ -766,7 +779,7 @@ RunPlan make_plan(tg_ctx* c) {
   P.S = c->ns;
   P.B = P.S * P.A;
   P.nn = (long long)P.n * P.n;
-  P.chunk = c->plain ? 1 : P.B;
+  P.chunk = c->plain ? 1 : chunk_items(c, P.B, P.A);
   P.os = 3 * (long long)P.n + 8;
   P.outs = tg_out_stride(P.n);
   P.is = tg_part_len(P.n);
@@ -838,6 +851,24 @@ RunPlan make_plan(tg_ctx* c) {
 // scaling, ln-norm and prefactor of every state) and, on the first call at a
 // size, the GEMM tile autotuning.  Runs eagerly, never under stream capture.
 int project_upload(tg_ctx* c) {
+  // When all items do not fit into the free device memory, the per-item work
+  // matrices are reused chunk by chunk (A_phi and H20 of every item are kept
+  // for sum_gauge): the memory-short mode of the batched path.
+  c->mem_chunk = 0;
+  if (!c->plain) {
+    RunPlan F = make_plan(c);
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+      const size_t have = free_b + c->d_work.bytes + c->d_pfw.bytes + c->d_blk.bytes;
+      const size_t nn = (size_t)F.nn;
+      const size_t per_item = sizeof(cplx) * (W_COUNT * nn) + (F.chunk ? F.pfw_bytes / F.chunk : 0) +
+                              sizeof(cplx) * 4 * nn;   // contraction layouts (upper bound)
+      const size_t fixed = sizeof(cplx) * 2 * (size_t)F.B * nn;
+      const size_t budget = (size_t)(0.85 * (double)have);
+      if (fixed + per_item * (size_t)F.chunk > budget && budget > fixed + per_item)
+        c->mem_chunk = (int)std::max<size_t>(1, (budget - fixed) / per_item);
+    }
+  }
   RunPlan P = make_plan(c);
   const int n = P.n, A = P.A, S = P.S;
   const long long nn = P.nn;
@@ -926,8 +957,8 @@ int project_run(tg_ctx* c) {
   double* R6 = c->d_real6.as<double>();      // Kp Kn Lp Ln Mp Mn Pp Pn [parity Pp Pn] [Ds]
   const long long ks = (long long)S * nn;    // stride of one product over the S states
   const cplx one = cmk(1, 0), zero = cmk(0, 0);
-  if (chunk != B && S != 1) {
-    tg_set_error("projection: angle chunks need a single state");
+  if (chunk != B && S != 1 && chunk % A != 0) {
+    tg_set_error("projection: a chunk must hold whole states");
     return TG_E_ARG;
   }
 
@@ -1082,9 +1113,12 @@ int project_run(tg_ctx* c) {
       TG_TRY(batched_inverse(s, n, Uphi, n, nn, na, P.d_det, P.d_info, P.d_perm));
     }
     // the real U, V of an item's state: batch element b of a launch over the
-    // chunk reads U + (b / A) * nn (one chunk holds whole states, or S = 1)
+    // chunk reads Uc + (b / A) * nn, Uc = U of the chunk's first state (a chunk
+    // holds whole states, or S = 1)
     {
       ScopedTiming tm(c, "gemm");
+      const double* Uc = U + (size_t)(a0 / A) * nn;
+      const double* Vc = V + (size_t)(a0 / A) * nn;
       TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, Vphi, n, nn, Uphi, n, nn, zero, Aphi, n, nn, na));
       hipLaunchKernelGGL(conj_kernel, dim3(nblk(tot)), dim3(NT), 0, s, tot, Aphi);
       hipLaunchKernelGGL(bcast_real_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, (long long)a0, A,
@@ -1092,12 +1126,12 @@ int project_run(tg_ctx* c) {
       hipLaunchKernelGGL(bcast_real_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, (long long)a0, A,
                          V, Vt);
       TG_HIP(hipGetLastError());
-      TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, V, n, nn, Aphi, n, nn, one, Ut, n, nn, na, A, 1));
-      TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, U, n, nn, Aphi, n, nn, one, Vt, n, nn, na, A, 1));
+      TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, Vc, n, nn, Aphi, n, nn, one, Ut, n, nn, na, A, 1));
+      TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, Uc, n, nn, Aphi, n, nn, one, Vt, n, nn, na, A, 1));
       // densities (calculate_densities with conj(U~), conj(V~))
-      TG_TRY(gemm_zd(s, OP_N, OP_T, n, n, n, one, Vt, n, nn, V, n, nn, zero, rho, n, nn, na, 1, A));
-      TG_TRY(gemm_zd(s, OP_N, OP_T, n, n, n, one, Vt, n, nn, U, n, nn, zero, kLR, n, nn, na, 1, A));
-      TG_TRY(gemm_dz(s, OP_N, OP_T, n, n, n, one, V, n, nn, Ut, n, nn, zero, kRL, n, nn, na, A, 1));
+      TG_TRY(gemm_zd(s, OP_N, OP_T, n, n, n, one, Vt, n, nn, Vc, n, nn, zero, rho, n, nn, na, 1, A));
+      TG_TRY(gemm_zd(s, OP_N, OP_T, n, n, n, one, Vt, n, nn, Uc, n, nn, zero, kLR, n, nn, na, 1, A));
+      TG_TRY(gemm_dz(s, OP_N, OP_T, n, n, n, one, Vc, n, nn, Ut, n, nn, zero, kRL, n, nn, na, A, 1));
     }
     // fields
     {
@@ -1180,7 +1214,8 @@ GraphKey graph_key(const tg_ctx* c) {
     }
   const long long vals[20] = {c->n, c->A, c->Mphip, c->Mphin, c->plain, c->ns, (long long)shapes,
                               c->iprint, c->grad_on, c->grad_ncons, b1, b2, b3,
-                              (long long)devbuf_epoch(), c->w_fp32, c->c20_shared, 1, 0, 0, 0};
+                              (long long)devbuf_epoch(), c->w_fp32, c->c20_shared, 1,
+                              chunk_items(c, c->ns * c->A, c->A), 0, 0};
   for (int i = 0; i < 20; ++i) k.v[i] = vals[i];
   return k;
 }
@@ -1553,6 +1588,16 @@ int tg_finalize(tg_ctx* c) {
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  return TG_OK;
+}
+
+int tg_set_batch_limit(tg_ctx* c, int max_items) {
+  if (!c || max_items < 0) {
+    tg_set_error("tg_set_batch_limit: invalid argument");
+    return TG_E_ARG;
+  }
+  c->batch_limit = max_items;
+  c->state_dirty = 1;
   return TG_OK;
 }
 
@@ -1986,6 +2031,11 @@ int tg_project_observables(tg_ctx* c, int nobo, const double* obo_ops, int nobos
   const RunPlan P = make_plan(c);
   const int A = P.A;
   hipStream_t s = c->stream;
+  if (P.chunk != P.B) {
+    // the observables read every angle's densities and fields after the loop
+    tg_set_error("tg_project_observables: the angles do not fit into device memory at once");
+    return TG_E_NOMEM;
+  }
   // device layout: ops (nobo n^2) | obos (2 nobos n^2) | ecomp (12 A) | occ (2 nshell A)
   //               | obo (2 nobo A) | obos (4 nobos A) | rho sum (n^2) [complex] | shell, iso (int)
   const size_t nd_ops = (size_t)(nobo + 2 * nobos) * nn;

@@ -18,7 +18,7 @@ use Hamiltonian
 implicit none
 public
 
-integer(c_int), parameter :: TG_OK = 0, TG_E_IMAG_GRADIENT = -5, &
+integer(c_int), parameter :: TG_OK = 0, TG_E_IMAG_GRADIENT = -5, TG_E_NOMEM = -3, &
                              TG_FLAG_PLAIN = 1, TG_FLAG_TBME_FP32 = 2, &
                              TG_FLAG_GRAPH = 4, TG_FLAG_CONJ_PAIRS = 8
 type(c_ptr) :: taurus_gpu_ctx = c_null_ptr

@@ -39,6 +39,7 @@ EXPORTS = [
     "tg_iterate", "tg_get_state", "tg_set_states", "tg_get_results_state",
     "tg_get_gradient_state", "tg_group_create", "tg_group_destroy", "tg_group_project",
     "tg_group_iterate", "tg_group_iterate_results", "tg_project_observables",
+    "tg_set_batch_limit",
 ]
 
 _lib = None
@@ -90,6 +91,7 @@ def load_library():
     lib.tg_group_iterate.argtypes = [P, D]
     lib.tg_group_iterate_results.argtypes = [P, I, P, P, P, P, P]
     lib.tg_project_observables.argtypes = [P, I, P, I, P, I, P, P, P]
+    lib.tg_set_batch_limit.argtypes = [P, I]
     _lib = lib
     return lib
 
@@ -148,6 +150,10 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_batch_limit(self, max_items: int):
+        """At most max_items (state, angle) items per launch sequence (0: all)."""
+        _check(load_library().tg_set_batch_limit(self._h, int(max_items)), "tg_set_batch_limit")
 
     def set_mode(self, plain: bool):
         _check(load_library().tg_set_mode(self._h, 1 if plain else 0), "tg_set_mode")
