@@ -304,11 +304,12 @@ int tg_group_iterate_results(tg_group* group, int index, double* pnp_scalars,
 
 /*
  * Device timing: with timing enabled, every stage and the dominant kernels of
- * tg_project_resident record HIP event pairs on the context stream (no
+ * tg_project_resident record HIP event pairs on the stream they run on (no
  * synchronisation is added); tg_get_timing returns the summed milliseconds and
  * the number of intervals recorded under `key` ("total", "kernel:gj_inverse",
- * "kernel:pfaffian", "kernel:grouped_ph", "kernel:grouped_pp", "contraction",
- * "gemm", "gemm_h20", "overlap", "thouless_inverse").
+ * "kernel:pfaffian", "kernel:grouped_ph", "kernel:grouped_pp", "kernel:zgemm"
+ * (every complex batched GEMM of the loop body), "contraction", "gemm",
+ * "gemm_h20", "overlap", "thouless_inverse").
  */
 int tg_enable_timing(tg_ctx* ctx, int on);
 int tg_reset_timing(tg_ctx* ctx);

@@ -449,8 +449,10 @@ call set_gradient
 open(ud, file=outfile, status='replace', form='formatted')
 do k = 1, kmax
   do isamp = 1, 2
-    mp = isamp
-    mn = 2
+    ! samples of 1x2 and 3x3 gauge angles (a wide spread of the angle count
+    ! keeps the per-angle slope clear of host timing noise)
+    mp = 1 + 2 * (isamp - 1)
+    mn = 2 + (isamp - 1)
     call system_clock(c0, rate)
     field_H20v = zero
     call calculate_densities_real(bogo_U0,bogo_V0,dens_rhoRR,dens_kappaRR,HOsp_dim)

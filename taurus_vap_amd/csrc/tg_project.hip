@@ -1035,7 +1035,10 @@ int project_run(tg_ctx* c) {
         hipLaunchKernelGGL(build_q_kernel, dim3(nblk(mm * cnt)), dim3(NT), 0, s2, n, o.nemp, nb, cnt,
                            b0, A, Pa, Pb, P.d_T, P.d_sg, os, P.d_ph, Q, JQ);
         TG_HIP(hipGetLastError());
-        TG_TRY(gemm_zz(s2, OP_T, OP_N, nb, nb, nb, one, Q, nb, mm, JQ, nb, mm, zero, K, nb, mm, cnt));
+        {
+          ScopedTiming tz(c, "kernel:zgemm", s2);
+          TG_TRY(gemm_zz(s2, OP_T, OP_N, nb, nb, nb, one, Q, nb, mm, JQ, nb, mm, zero, K, nb, mm, cnt));
+        }
         hipLaunchKernelGGL(add_j_kernel, dim3(nblk((long long)(nb / 2) * cnt)), dim3(NT), 0, s2, nb,
                            cnt, b0, A, P.d_sg, os, K);
         TG_HIP(hipGetLastError());
@@ -1119,7 +1122,10 @@ int project_run(tg_ctx* c) {
       ScopedTiming tm(c, "gemm");
       const double* Uc = U + (size_t)(a0 / A) * nn;
       const double* Vc = V + (size_t)(a0 / A) * nn;
-      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, Vphi, n, nn, Uphi, n, nn, zero, Aphi, n, nn, na));
+      {
+        ScopedTiming tz(c, "kernel:zgemm");
+        TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, Vphi, n, nn, Uphi, n, nn, zero, Aphi, n, nn, na));
+      }
       hipLaunchKernelGGL(conj_kernel, dim3(nblk(tot)), dim3(NT), 0, s, tot, Aphi);
       hipLaunchKernelGGL(bcast_real_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, (long long)a0, A,
                          U, Ut);
@@ -1150,11 +1156,15 @@ int project_run(tg_ctx* c) {
     {
       ScopedTiming tm(c, "gemm_h20");
       const cplx hfd = cmk(0.5 * c->factor_delta, 0.0), mhfd = cmk(-0.5 * c->factor_delta, 0.0);
-      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, hsp, n, nn, Vt, n, nn, zero, X, n, nn, na));
-      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, hfd, dLR, n, nn, Ut, n, nn, one, X, n, nn, na));
-      TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, mhfd, dRL, n, nn, Vt, n, nn, zero, Y, n, nn, na));
-      TG_TRY(gemm_zz(s, OP_T, OP_N, n, n, n, one, Ut, n, nn, X, n, nn, zero, Z, n, nn, na));
-      TG_TRY(gemm_zz(s, OP_T, OP_N, n, n, n, one, Vt, n, nn, Y, n, nn, one, Z, n, nn, na));
+      auto zg = [&](TgOp oa, const cplx* Aa, cplx al, const cplx* Bb, cplx be, cplx* Cc) -> int {
+        ScopedTiming tz(c, "kernel:zgemm");
+        return gemm_zz(s, oa, OP_N, n, n, n, al, Aa, n, nn, Bb, n, nn, be, Cc, n, nn, na);
+      };
+      TG_TRY(zg(OP_N, hsp, one, Vt, zero, X));
+      TG_TRY(zg(OP_N, dLR, hfd, Ut, one, X));
+      TG_TRY(zg(OP_N, dRL, mhfd, Vt, zero, Y));
+      TG_TRY(zg(OP_T, Ut, one, X, zero, Z));
+      TG_TRY(zg(OP_T, Vt, one, Y, one, Z));
       hipLaunchKernelGGL(skew_part_kernel, dim3((n + 31) / 32, (n + 31) / 32, na), dim3(256), 0, s,
                          n, Z, H20);
       TG_HIP(hipGetLastError());
