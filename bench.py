@@ -48,17 +48,18 @@ def log(msg):
 
 
 def algorithmic_flops(ctx_info, n, A, nb):
-    """Algorithmic FP64 flops per launch of the timed kernels (real flops;
-    complex multiply-add = 8, real x complex = 4)."""
+    """Algorithmic FP64 flops per step (one projection of all A angles) of the
+    timed kernels (real flops; complex multiply-add = 8, real x complex = 4)."""
     return {
         "kernel:grouped_ph": 4.0 * ctx_info["wph"] * A,
         "kernel:grouped_pp": 4.0 * ctx_info["wpp"] * 2 * A,
         "kernel:gj_inverse": 8.0 * n ** 3 * A,
         "kernel:pfaffian": (4.0 / 3.0) * nb ** 3 * A,
-        # complex batched GEMMs of the loop body per launch, averaged over the
-        # launches of one step: A_phi and the five H20 products (n^3 each) and
-        # the reduced-overlap product (nb^3)
-        "kernel:zgemm": 8.0 * A * (6 * n ** 3 + nb ** 3) / 7.0,
+        # complex batched GEMMs of the loop body on the main stream: A_phi (n^3),
+        # the two H20 products over the inner dimension 2n and the one over n
+        "kernel:zgemm": 8.0 * A * 6 * n ** 3,
+        # the reduced-overlap product Q^T (J Q) on the second stream (nb^3)
+        "kernel:zgemm_q": 8.0 * A * nb ** 3,
     }
 
 
@@ -123,7 +124,8 @@ def run_gpu(args, ham, plain, fp32, steps, warmup, timing=True, graph=False, con
     kt = {}
     if timing:
         for key in ("total", "kernel:grouped_ph", "kernel:grouped_pp", "kernel:gj_inverse",
-                    "kernel:pfaffian", "kernel:zgemm", "contraction", "gemm", "gemm_h20",
+                    "kernel:pfaffian", "kernel:zgemm", "kernel:zgemm_q", "contraction", "gemm",
+                    "gemm_h20",
                     "overlap", "thouless_inverse"):
             kt[key] = ctx.timing(key)
         ctx.enable_timing(False)
@@ -384,11 +386,13 @@ def main():
     by_kernel = {}
     for k, (ms, cnt) in kern.items():
         avg_s = ms / cnt / 1e3
-        ach = fl[k] / avg_s / 1e12
+        per_step = cnt / main_run["steps"]
+        flop_launch = fl[k] / per_step
+        ach = flop_launch / avg_s / 1e12
         name = k.split(":", 1)[1]
         by_kernel[name] = {"ms_per_step": ms / main_run["steps"],
-                           "launches_per_step": cnt / main_run["steps"],
-                           "avg_launch_ms": avg_s * 1e3, "flop_per_launch": fl[k],
+                           "launches_per_step": per_step,
+                           "avg_launch_ms": avg_s * 1e3, "flop_per_launch": flop_launch,
                            "achieved": ach, "frac": ach / PEAK_FP64_TFLOPS,
                            "traffic": measured_traffic(args.workload, A, name)}
     if kern:

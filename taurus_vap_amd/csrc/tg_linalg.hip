@@ -147,7 +147,8 @@ __device__ inline void wave_argmax(double& v, int& i) {
   }
 }
 
-// block-wide argmax (first index on ties) for GJ_THREADS threads
+// block-wide argmax (first index on ties) for TH threads
+template <int TH>
 __device__ inline void gj_argmax(double v, int i, double* wv, int* wi, double& bv, int& bi) {
   wave_argmax(v, i);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -155,17 +156,18 @@ __device__ inline void gj_argmax(double v, int i, double* wv, int* wi, double& b
   __syncthreads();
   bv = wv[0];
   bi = wi[0];
-  for (int w = 1; w < GJ_THREADS / 64; ++w)
+  for (int w = 1; w < TH / 64; ++w)
     if (wv[w] > bv || (wv[w] == bv && wi[w] < bi)) { bv = wv[w]; bi = wi[w]; }
   __syncthreads();
 }
 
-template <int GJ_NB, int RP, int TH>
+// TH threads per copy (n <= TH), RP copies, WIN columns per staged window
+template <int GJ_NB, int RP, int TH, int WIN>
 __global__ void __launch_bounds__(TH * RP)
 gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* __restrict__ det,
                   int* __restrict__ info, int* __restrict__ perm_all) {
   __shared__ cplx sw[2][2][GJ_NB];
-  __shared__ cplx Pw[GJ_NB][GJ_WIN + GJ_U];   // GJ_U columns of slack: no clamping below
+  __shared__ cplx Pw[GJ_NB][WIN + GJ_U];   // GJ_U columns of slack: no clamping below
   __shared__ int srcs[GJ_NB];
   __shared__ double wv[2][TH * RP / 64];
   __shared__ int wi[2][TH * RP / 64];
@@ -273,8 +275,8 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
     // swapped once, with a row at or below it), so after the panel source rows
     // are staged in LDS every thread only reads and writes its own row.
     const int nrest = n - w;
-    for (int c0 = 0; c0 < nrest; c0 += GJ_WIN) {
-      const int cw = min(GJ_WIN, nrest - c0);
+    for (int c0 = 0; c0 < nrest; c0 += WIN) {
+      const int cw = min(WIN, nrest - c0);
       for (int e = tt; e < w * cw; e += TH * RP) {
         const int m = e % w, c = e / w;
         const int jj = c0 + c;
@@ -435,23 +437,25 @@ constexpr int PF_THREADS = 512;
 constexpr int PF_PS = 8;
 constexpr int PF_WIN = 256;   // columns whose pending vectors are staged in LDS at once
 
-__global__ void __launch_bounds__(PF_THREADS)
+// TH threads (n <= TH), WIN columns per staged window
+template <int TH, int WIN>
+__global__ void __launch_bounds__(TH)
 pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long sA,
                         cplx* __restrict__ work, double* __restrict__ logabs,
                         cplx* __restrict__ phase, int* __restrict__ zero) {
   __shared__ cplx xk[PF_PS], yk[PF_PS];
   __shared__ cplx swx[2][PF_PS], swy[2][PF_PS];
   __shared__ cplx swc[2];
-  __shared__ cplx cx[PF_WIN][PF_PS], cy[PF_WIN][PF_PS];
-  __shared__ double wv[PF_THREADS / 64];
-  __shared__ int wi[PF_THREADS / 64];
+  __shared__ cplx cx[WIN][PF_PS], cy[WIN][PF_PS];
+  __shared__ double wv[TH / 64];
+  __shared__ int wi[TH / 64];
   __shared__ cplx spiv;
   const int t = threadIdx.x;
   const bool own = t < n;
   const cplx* A = Aall + (long long)blockIdx.x * sA;
   cplx* W = work + (long long)blockIdx.x * n * n;
   const long long nn = (long long)n * n;
-  for (long long idx = t; idx < nn; idx += PF_THREADS) {
+  for (long long idx = t; idx < nn; idx += TH) {
     const int i = (int)(idx % n), j = (int)(idx / n);
     cplx v = cmk(0.0, 0.0);
     if (i < j) v = A[i + (long long)j * lda];
@@ -491,7 +495,7 @@ pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long
       int vi = (own && t < k) ? t : n;
       double mx;
       int kp;
-      gj_argmax(v, vi, wv, wi, mx, kp);
+      gj_argmax<TH>(v, vi, wv, wi, mx, kp);
       if (!(mx > 0.0) || kp < 0 || kp >= k) { z = 1; break; }
       if (kp != kk) {
         if (t == kp) {
@@ -505,7 +509,7 @@ pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long
           swc[1] = cik;
         }
         // rows kp <-> kk of the leading columns
-        for (int j = t; j < k; j += PF_THREADS) {
+        for (int j = t; j < k; j += TH) {
           cplx* r1 = W + kp + (long long)j * n;
           cplx* r2 = W + kk + (long long)j * n;
           const cplx x1 = *r1;
@@ -566,9 +570,9 @@ pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long
     }
     if (z) break;
     // rank-2*ns update of the leading block [0, m): the pending vectors of
-    // PF_WIN columns are staged in LDS, then every thread streams its own row
-    for (int j0 = 0; j0 < m; j0 += PF_WIN) {
-      const int cw = min(PF_WIN, m - j0);
+    // WIN columns are staged in LDS, then every thread streams its own row
+    for (int j0 = 0; j0 < m; j0 += WIN) {
+      const int cw = min(WIN, m - j0);
       if (t >= j0 && t < j0 + cw)
 #pragma unroll
         for (int s2 = 0; s2 < PF_PS; ++s2) { cx[t - j0][s2] = X[s2]; cy[t - j0][s2] = Y[s2]; }
@@ -603,19 +607,30 @@ int batched_inverse(hipStream_t s, int n, cplx* A, int lda, long long sA, int ba
                     int* info, int* perm) {
   if (n <= 0 || batch <= 0) return TG_OK;
   if (n <= GJ_THREADS) {
-    // panel width: 16 or 24 columns (TG_GJ_NB), 16 by default
-    // TG_GJ=1: one thread per row; default: two threads per row (the second
-    // mirrors the panel reduction and takes half of the column updates)
+    // one thread per row in the smallest workgroup that holds the rows (the
+    // LDS window shrinks with it, so small matrices share a CU); TG_GJ=2: two
+    // threads per row (the second mirrors the panel reduction and takes half
+    // of the column updates), n <= 384
     static const int rp = [] {
       const char* e = std::getenv("TG_GJ");
-      return (e && std::atoi(e) == 1) ? 1 : 2;
+      return (e && std::atoi(e) == 2) ? 2 : 1;
     }();
+#define TG_GJ_LAUNCH(RP, TH, WIN)                                                              \
+  hipLaunchKernelGGL((gj_blocked_kernel<16, RP, TH, WIN>), dim3(batch), dim3(TH * RP), 0, s, n, A, \
+                     lda, sA, det, info, perm)
     if (rp == 2 && n <= 384)
-      hipLaunchKernelGGL((gj_blocked_kernel<16, 2, 384>), dim3(batch), dim3(768), 0, s, n, A, lda,
-                         sA, det, info, perm);
+      TG_GJ_LAUNCH(2, 384, GJ_WIN);
+    else if (n <= 64)
+      TG_GJ_LAUNCH(1, 64, 64);
+    else if (n <= 128)
+      TG_GJ_LAUNCH(1, 128, 128);
+    else if (n <= 256)
+      TG_GJ_LAUNCH(1, 256, 256);
+    else if (n <= 384)
+      TG_GJ_LAUNCH(1, 384, GJ_WIN);
     else
-      hipLaunchKernelGGL((gj_blocked_kernel<16, 1, GJ_THREADS>), dim3(batch), dim3(GJ_THREADS), 0, s,
-                         n, A, lda, sA, det, info, perm);
+      TG_GJ_LAUNCH(1, GJ_THREADS, GJ_WIN);
+#undef TG_GJ_LAUNCH
     TG_HIP(hipGetLastError());
     return TG_OK;
   }
@@ -634,8 +649,21 @@ int batched_pfaffian(hipStream_t s, int n, const cplx* A, int lda, long long sA,
     return TG_E_ARG;
   }
   if (n <= PF_THREADS) {
-    hipLaunchKernelGGL(pfaffian_blocked_kernel, dim3(batch), dim3(PF_THREADS), 0, s, n, A, lda, sA,
-                       work, logabs, phase, zero);
+    // one thread per row in the smallest workgroup that holds the rows
+#define TG_PF_LAUNCH(TH, WIN)                                                                 \
+  hipLaunchKernelGGL((pfaffian_blocked_kernel<TH, WIN>), dim3(batch), dim3(TH), 0, s, n, A, lda, \
+                     sA, work, logabs, phase, zero)
+    if (n <= 64)
+      TG_PF_LAUNCH(64, 64);
+    else if (n <= 128)
+      TG_PF_LAUNCH(128, 128);
+    else if (n <= 256)
+      TG_PF_LAUNCH(256, PF_WIN);
+    else if (n <= 384)
+      TG_PF_LAUNCH(384, PF_WIN);
+    else
+      TG_PF_LAUNCH(PF_THREADS, PF_WIN);
+#undef TG_PF_LAUNCH
     TG_HIP(hipGetLastError());
     return TG_OK;
   }

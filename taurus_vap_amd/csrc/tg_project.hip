@@ -1036,7 +1036,7 @@ int project_run(tg_ctx* c) {
                            b0, A, Pa, Pb, P.d_T, P.d_sg, os, P.d_ph, Q, JQ);
         TG_HIP(hipGetLastError());
         {
-          ScopedTiming tz(c, "kernel:zgemm", s2);
+          ScopedTiming tz(c, "kernel:zgemm_q", s2);
           TG_TRY(gemm_zz(s2, OP_T, OP_N, nb, nb, nb, one, Q, nb, mm, JQ, nb, mm, zero, K, nb, mm, cnt));
         }
         hipLaunchKernelGGL(add_j_kernel, dim3(nblk((long long)(nb / 2) * cnt)), dim3(NT), 0, s2, nb,
