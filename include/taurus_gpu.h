@@ -308,8 +308,9 @@ int tg_group_iterate_results(tg_group* group, int index, double* pnp_scalars,
  * synchronisation is added); tg_get_timing returns the summed milliseconds and
  * the number of intervals recorded under `key` ("total", "kernel:gj_inverse",
  * "kernel:pfaffian", "kernel:grouped_ph", "kernel:grouped_pp", "kernel:zgemm"
- * (every complex batched GEMM of the loop body), "contraction", "gemm",
- * "gemm_h20", "overlap", "thouless_inverse").
+ * (the complex batched GEMMs of the loop body on the context stream),
+ * "kernel:zgemm_q" (the reduced-overlap product on the second stream),
+ * "contraction", "gemm", "gemm_h20", "overlap", "thouless_inverse").
  */
 int tg_enable_timing(tg_ctx* ctx, int on);
 int tg_reset_timing(tg_ctx* ctx);
