@@ -25,49 +25,94 @@ namespace {
 
 constexpr int NT = 256;
 
-// X[a + p*na] = src_a[perm[p]]   (src_a at src + a*nn)
-__global__ void gather_kernel(int na, int npos, const int* __restrict__ perm,
-                              const cplx* __restrict__ src, long long nn, int ld_out, int row0,
-                              cplx* __restrict__ X) {
-  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
-  long long tot = (long long)na * npos;
-  if (idx >= tot) return;
-  int a = (int)(idx % na);
-  int p = (int)(idx / na);
-  X[row0 + a + (long long)p * ld_out] = src[(long long)a * nn + perm[p]];
-}
+// Tiled transposes between the angle-major matrices (angle a at src + a*nn,
+// element (x, y) at linear index x + y*n) and the position-major contraction
+// layouts (X[row0 + a + p*ld], position p of a channel).  A workgroup moves a
+// tile of TL linear indices x TA angles through LDS: the matrix side is read or
+// written as TA runs of TL consecutive elements, the contraction side as TL runs
+// of TA consecutive angles, so both sides are coalesced.  inv maps linear
+// indices to positions (Channel::inv).
+constexpr int TL = 64;   // linear indices per tile
+constexpr int TA = 32;   // angles per tile
 
-// Gamma_a[perm[p]] = C[a + p*na]
-__global__ void scatter_ph_kernel(int na, int npos, const int* __restrict__ perm,
-                                  const cplx* __restrict__ C, long long nn,
-                                  cplx* __restrict__ dst) {
-  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
-  long long tot = (long long)na * npos;
-  if (idx >= tot) return;
-  int a = (int)(idx % na);
-  int p = (int)(idx / na);
-  dst[(long long)a * nn + perm[p]] = C[a + (long long)p * na];
-}
-
-// Delta_a(x,y) = C, Delta_a(y,x) = -C (x>y); the diagonal is set to zero.
-__global__ void scatter_pp_kernel(int n, int na, int npos, const int* __restrict__ perm,
-                                  const cplx* __restrict__ C, int ldc, int row0, long long nn,
-                                  cplx* __restrict__ dst) {
-  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
-  long long tot = (long long)na * npos;
-  if (idx < tot) {
-    int a = (int)(idx % na);
-    int p = (int)(idx / na);
-    int lin = perm[p];
-    int x = lin % n, y = lin / n;
-    cplx v = C[row0 + a + (long long)p * ldc];
-    dst[(long long)a * nn + lin] = v;
-    dst[(long long)a * nn + y + (long long)x * n] = cmk(-v.x, -v.y);
-  } else if (idx < tot + (long long)na * n) {
-    long long r = idx - tot;
-    int a = (int)(r % na), i = (int)(r / na);
-    dst[(long long)a * nn + i + (long long)i * n] = cmk(0.0, 0.0);
+// X[row0 + a + inv[lin]*ld] = src_a[lin] for every lin with inv[lin] >= 0
+__global__ void __launch_bounds__(NT)
+gather_tiled_kernel(int na, long long nn, const int* __restrict__ inv,
+                    const cplx* __restrict__ src, int ld, int row0, cplx* __restrict__ X) {
+  __shared__ cplx tile[TL][TA + 1];
+  const long long lin0 = (long long)blockIdx.x * TL;
+  const int a0 = blockIdx.y * TA;
+  const int t = threadIdx.x;
+  {
+    const int l = t % TL;
+    const long long lin = lin0 + l;
+    for (int r = t / TL; r < TA; r += NT / TL) {
+      const int a = a0 + r;
+      if (a < na && lin < nn) tile[l][r] = src[(long long)a * nn + lin];
+    }
   }
+  __syncthreads();
+  const int r = t % TA;
+  for (int l = t / TA; l < TL; l += NT / TA) {
+    const long long lin = lin0 + l;
+    const int a = a0 + r;
+    if (lin < nn && a < na) {
+      const int q = inv[lin];
+      if (q >= 0) X[row0 + a + (long long)q * ld] = tile[l][r];
+    }
+  }
+}
+
+// dst_a[lin] = field at lin from C[row0 + a + q*ld]:  q = inv[lin] >= 0: C;
+// pp (q <= -2): -C at position -2 - q; the diagonal (q == -1): 0.  With hsp:
+// hsp_a[lin] = dst_a[lin] + H1[lin].
+__global__ void __launch_bounds__(NT)
+scatter_tiled_kernel(int na, long long nn, const int* __restrict__ inv,
+                     const cplx* __restrict__ C, int ld, int row0, cplx* __restrict__ dst,
+                     const cplx* __restrict__ H1, cplx* __restrict__ hsp) {
+  __shared__ cplx tile[TL][TA + 1];
+  const long long lin0 = (long long)blockIdx.x * TL;
+  const int a0 = blockIdx.y * TA;
+  const int t = threadIdx.x;
+  {
+    const int r = t % TA;
+    for (int l = t / TA; l < TL; l += NT / TA) {
+      const long long lin = lin0 + l;
+      const int a = a0 + r;
+      if (lin < nn && a < na) {
+        const int q = inv[lin];
+        cplx v = cmk(0.0, 0.0);
+        if (q >= 0) {
+          v = C[row0 + a + (long long)q * ld];
+        } else if (q <= -2) {
+          v = C[row0 + a + (long long)(-2 - q) * ld];
+          v = cmk(-v.x, -v.y);
+        }
+        tile[l][r] = v;
+      }
+    }
+  }
+  __syncthreads();
+  const int l = t % TL;
+  const long long lin = lin0 + l;
+  if (lin >= nn) return;
+  const cplx h = hsp ? H1[lin] : cmk(0.0, 0.0);
+  for (int r = t / TL; r < TA; r += NT / TL) {
+    const int a = a0 + r;
+    if (a < na) {
+      const cplx v = tile[l][r];
+      dst[(long long)a * nn + lin] = v;
+      if (hsp) hsp[(long long)a * nn + lin] = cadd(v, h);
+    }
+  }
+}
+
+// hsp_a = Gamma_a + H1
+__global__ void add_h1_kernel(long long nn, int na, const cplx* __restrict__ H1,
+                              const cplx* __restrict__ G, cplx* __restrict__ h) {
+  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= nn * na) return;
+  h[idx] = cadd(G[idx], H1[idx % nn]);
 }
 
 // calculate_fields_diag: Gamma(i,j) = conj(Gamma(j,i)) for i > j; DeltaRL = conj(DeltaLR)
@@ -123,6 +168,10 @@ void plan_channel(Channel& ch, int n, const std::vector<int>& m, const std::vect
   ch.npos = off;
   ch.ngroups = (int)ch.groups.size();
   ch.wsize = woff;
+  ch.inv = pos_of;
+  if (pp)
+    for (int y = 0; y < n; ++y)
+      for (int x = 0; x < y; ++x) ch.inv[x + (size_t)y * n] = -2 - pos_of[y + (size_t)x * n];
 }
 
 }  // namespace
@@ -219,8 +268,8 @@ int contraction_build(tg_ctx* c, int64_t ntb, const int16_t* abcd, const void* H
   // upload
   for (Channel* ch : {&c->ph, &c->pp}) {
     std::vector<double>& W = (ch == &c->ph) ? Wph : Wpp;
-    TG_TRY(ch->d_perm.ensure(sizeof(int) * ch->perm.size()));
-    TG_HIP(hipMemcpy(ch->d_perm.p, ch->perm.data(), sizeof(int) * ch->perm.size(),
+    TG_TRY(ch->d_inv.ensure(sizeof(int) * ch->inv.size()));
+    TG_HIP(hipMemcpy(ch->d_inv.p, ch->inv.data(), sizeof(int) * ch->inv.size(),
                      hipMemcpyHostToDevice));
     TG_TRY(ch->d_groups.ensure(sizeof(GroupDesc) * ch->groups.size()));
     TG_HIP(hipMemcpy(ch->d_groups.p, ch->groups.data(), sizeof(GroupDesc) * ch->groups.size(),
@@ -251,7 +300,7 @@ int contraction_build(tg_ctx* c, int64_t ntb, const int16_t* abcd, const void* H
 }
 
 int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const cplx* kRL,
-                    cplx* gamma, cplx* dLR, cplx* dRL, int hermitian) {
+                    cplx* gamma, cplx* dLR, cplx* dRL, int hermitian, const cplx* H1, cplx* hsp) {
   const int n = c->n;
   const long long nn = (long long)n * n;
   hipStream_t s = c->stream;
@@ -265,15 +314,15 @@ int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const c
   cplx* Xpp = Cph + (size_t)na * nph;
   cplx* Cpp = Xpp + (size_t)two * na * npp;
 
-  const int* pph = c->ph.d_perm.as<int>();
-  const int* ppp = c->pp.d_perm.as<int>();
-  hipLaunchKernelGGL(gather_kernel, dim3(nblk((long long)na * nph)), dim3(NT), 0, s, na, nph, pph,
-                     rho, nn, na, 0, Xph);
-  hipLaunchKernelGGL(gather_kernel, dim3(nblk((long long)na * npp)), dim3(NT), 0, s, na, npp, ppp,
-                     kLR, nn, two * na, 0, Xpp);
+  const int* iph = c->ph.d_inv.as<int>();
+  const int* ipp = c->pp.d_inv.as<int>();
+  const dim3 tiles((unsigned)((nn + TL - 1) / TL), (unsigned)((na + TA - 1) / TA));
+  hipLaunchKernelGGL(gather_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, iph, rho, na, 0, Xph);
+  hipLaunchKernelGGL(gather_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, ipp, kLR, two * na, 0,
+                     Xpp);
   if (two == 2)
-    hipLaunchKernelGGL(gather_kernel, dim3(nblk((long long)na * npp)), dim3(NT), 0, s, na, npp,
-                       ppp, kRL, nn, two * na, na, Xpp);
+    hipLaunchKernelGGL(gather_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, ipp, kRL, two * na, na,
+                       Xpp);
   TG_HIP(hipGetLastError());
   // first call for this batch size: pick the tile shapes (outputs recomputed below)
   TG_TRY(gemm_tune_grouped(s, na, Xph, Cph, c->ph.d_W.p, c->w_fp32, c->ph.d_groups.as<GroupDesc>(),
@@ -294,13 +343,15 @@ int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const c
                         c->pp.d_groups.as<GroupDesc>(), c->pp.d_tile_group.as<int>(),
                         c->pp.d_tile_n0.as<int>(), 1, c->pp.ntiles_total));
   }
-  hipLaunchKernelGGL(scatter_ph_kernel, dim3(nblk((long long)na * nph)), dim3(NT), 0, s, na, nph,
-                     pph, Cph, nn, gamma);
-  hipLaunchKernelGGL(scatter_pp_kernel, dim3(nblk((long long)na * npp + (long long)na * n)),
-                     dim3(NT), 0, s, n, na, npp, ppp, Cpp, two * na, 0, nn, dLR);
+  // Gamma (and h = Gamma + H1 when no hermitian completion follows), DeltaLR, DeltaRL
+  const bool fuse_h = hsp != nullptr && !hermitian;
+  hipLaunchKernelGGL(scatter_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, iph, Cph, na, 0, gamma,
+                     fuse_h ? H1 : nullptr, fuse_h ? hsp : nullptr);
+  hipLaunchKernelGGL(scatter_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, ipp, Cpp, two * na, 0,
+                     dLR, (const cplx*)nullptr, (cplx*)nullptr);
   if (two == 2)
-    hipLaunchKernelGGL(scatter_pp_kernel, dim3(nblk((long long)na * npp + (long long)na * n)),
-                       dim3(NT), 0, s, n, na, npp, ppp, Cpp, two * na, na, nn, dRL);
+    hipLaunchKernelGGL(scatter_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, ipp, Cpp, two * na, na,
+                       dRL, (const cplx*)nullptr, (cplx*)nullptr);
   TG_HIP(hipGetLastError());
   if (hermitian)
     hipLaunchKernelGGL(hermitize_kernel, dim3(nblk((long long)na * nn)), dim3(NT), 0, s, n, na, nn,
@@ -308,6 +359,9 @@ int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const c
   else if (two == 1 && dRL != nullptr)
     hipLaunchKernelGGL(hermitize_kernel, dim3(nblk((long long)na * nn)), dim3(NT), 0, s, n, na, nn,
                        (cplx*)nullptr, dLR, dRL);
+  if (hsp != nullptr && !fuse_h)
+    hipLaunchKernelGGL(add_h1_kernel, dim3(nblk((long long)na * nn)), dim3(NT), 0, s, nn, na, H1,
+                       gamma, hsp);
   TG_HIP(hipGetLastError());
   return TG_OK;
 }

@@ -163,8 +163,12 @@ struct Channel {
   int ngroups = 0;
   std::vector<GroupDesc> groups;
   std::vector<int> perm;        // position -> linear index (x + y*n) of the element
+  // linear index -> position: >= 0 for a stored element; pp: -2 - position of
+  // the transposed element for x < y (the field there is minus it), -1 on the
+  // diagonal
+  std::vector<int> inv;
   long long wsize = 0;          // sum of s_g^2
-  DevBuf d_perm, d_groups, d_W, d_tile_group, d_tile_n0;
+  DevBuf d_inv, d_groups, d_W, d_tile_group, d_tile_n0;
   int ntiles_n = 0, ntiles_total = 0;
 };
 
@@ -300,6 +304,8 @@ int contraction_build(tg_ctx* c, int64_t ntb, const int16_t* abcd, const void* H
 // Gamma, DeltaLR, DeltaRL for `na` angles whose densities are stored per angle
 // (complex n*n, stride n*n).  Writes gamma (without H1), deltaLR, deltaRL.
 // hermitian != 0 reproduces calculate_fields_diag (Gamma made hermitian from its
-// upper triangle, DeltaRL = conj(DeltaLR)).
+// upper triangle, DeltaRL = conj(DeltaLR)).  With hsp != nullptr it also writes
+// hsp = gamma + H1 (H1 one complex n*n matrix shared by the angles).
 int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const cplx* kRL,
-                    cplx* gamma, cplx* dLR, cplx* dRL, int hermitian);
+                    cplx* gamma, cplx* dLR, cplx* dRL, int hermitian,
+                    const cplx* H1 = nullptr, cplx* hsp = nullptr);

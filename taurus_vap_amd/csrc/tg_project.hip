@@ -1142,10 +1142,8 @@ int project_run(tg_ctx* c) {
     // fields
     {
       ScopedTiming tm(c, "contraction");
-      TG_TRY(contraction_run(c, na, rho, kLR, proj ? kRL : nullptr, G, dLR, dRL, proj ? 0 : 1));
-      hipLaunchKernelGGL(add_h1_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, c->d_H1.as<cplx>(),
-                         G, hsp);
-      TG_HIP(hipGetLastError());
+      TG_TRY(contraction_run(c, na, rho, kLR, proj ? kRL : nullptr, G, dLR, dRL, proj ? 0 : 1,
+                             c->d_H1.as<cplx>(), hsp));
     }
     // particle numbers and energy
     hipLaunchKernelGGL(traces_kernel, dim3(na), dim3(NT), 0, s, n, c->d_H1.as<cplx>(), rho, kLR,
@@ -1590,7 +1588,7 @@ int tg_finalize(tg_ctx* c) {
                     &c->d_Z, &c->d_psign, &c->d_cHO, &c->d_it, &c->d_obs})
     b->release();
   for (Channel* ch : {&c->ph, &c->pp})
-    for (DevBuf* b : {&ch->d_perm, &ch->d_groups, &ch->d_W, &ch->d_tile_group, &ch->d_tile_n0})
+    for (DevBuf* b : {&ch->d_inv, &ch->d_groups, &ch->d_W, &ch->d_tile_group, &ch->d_tile_n0})
       b->release();
   if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
