@@ -221,34 +221,47 @@ def batched_solves(args):
 
 
 def device_iteration(args, ham):
-    """Full gradient iterations of the workload with the constraint machinery
-    on the device (tg_iterate: constraint_20, projection, Lagrange multipliers,
-    gradient, adjust_constraints) and the canonical basis of each new state on
-    the host, with <Z>, <N> constrained.  Not part of `value`."""
-    from taurus_vap_amd import host
+    """Full gradient iterations of the workload on the device: tg_iterate
+    (constraint_20, projection, Lagrange multipliers, gradient, adjust_constraints)
+    then tg_canonical_basis of each new state, which never leaves the device,
+    with <Z>, <N> constrained.  A state the device canonical basis refuses
+    (degenerate kappa subspace wider than one pair) takes the host route and is
+    counted.  Not part of `value`."""
+    from taurus_vap_amd import host, gpu
     ctx, h, w, U, V = host.workload_context(args.workload, seed=args.seed, tbme_fp32=True, ham=ham)
     M = args.angles if args.angles else w.M
     Zop, Nop = host.number_operators(h["basis"])
     ctx.set_gradient(args.eta, 0.0)
     ctx.set_constraints(np.stack([Zop.T, Nop.T]).reshape(-1), [float(w.Z), float(w.N)], 0, 1e-6)
-    dev, hst, nadj = [], [], []
+    cb = host.construct_canonical_basis(U, V)
+    ctx.set_state(U, V, cb["D"], cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"], M, M,
+                  FACPI, float(w.Z), float(w.N))
+    it_ms, cb_ms, nadj, host_route = [], [], [], 0
     for it in range(args.iterate_steps + 1):
         t0 = time.perf_counter()
-        cb = host.construct_canonical_basis(U, V)
-        ctx.set_state(U, V, cb["D"], cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"], M, M,
-                      FACPI, float(w.Z), float(w.N))
-        t1 = time.perf_counter()
         r = ctx.iterate(0.0)
-        U, V = ctx.get_state()
+        t1 = time.perf_counter()
+        try:
+            ctx.canonical_basis()
+        except gpu.TaurusGPUError as e:
+            if "(-7)" not in str(e):
+                raise
+            U, V = ctx.get_state()
+            cb = host.construct_canonical_basis(U, V)
+            ctx.set_state(U, V, cb["D"], cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"], M, M,
+                          FACPI, float(w.Z), float(w.N))
+            host_route += 1
+        ctx.sync()
         t2 = time.perf_counter()
         if it > 0:                  # the first call allocates and tunes
-            hst.append(t1 - t0)
-            dev.append(t2 - t1)
+            it_ms.append(1e3 * (t1 - t0))
+            cb_ms.append(1e3 * (t2 - t1))
             nadj.append(r["adjust_iterations"])
     ctx.close()
-    per = np.mean(dev) + np.mean(hst)
+    per = (np.mean(it_ms) + np.mean(cb_ms)) / 1e3
     return {"value": 1.0 / per, "unit": "iterations/s", "iterations": args.iterate_steps,
-            "device_ms": 1e3 * float(np.mean(dev)), "host_canonical_basis_ms": 1e3 * float(np.mean(hst)),
+            "iterate_ms": float(np.mean(it_ms)), "canonical_basis_ms": float(np.mean(cb_ms)),
+            "canonical_basis": "device", "host_route_states": host_route,
             "adjust_steps": nadj, "constraints": "<Z>, <N>"}
 
 
@@ -438,8 +451,10 @@ def main():
     if args.iterate_steps > 0:
         try:
             devit = device_iteration(args, ham)
-            log("device iteration: %.3f it/s (device %.2f ms, host canonical basis %.2f ms)"
-                % (devit["value"], devit["device_ms"], devit["host_canonical_basis_ms"]))
+            log("device iteration: %.3f it/s (tg_iterate %.2f ms, device canonical basis %.2f ms, "
+                "%d states on the host route)" % (devit["value"], devit["iterate_ms"],
+                                                  devit["canonical_basis_ms"],
+                                                  devit["host_route_states"]))
         except Exception as e:
             log("device iteration failed: %r" % (e,))
 

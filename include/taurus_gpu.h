@@ -35,6 +35,7 @@ extern "C" {
 #define TG_E_NODEVICE       -4   /* no HIP device */
 #define TG_E_IMAG_GRADIENT  -5   /* calculate_projected_H20: Im(gradient) > 1e-8 */
 #define TG_E_SINGULAR       -6   /* singular overlap matrix (zgetrf info /= 0) */
+#define TG_E_UNSUPPORTED    -7   /* a case the device routine leaves to the host */
 
 /* Context flags */
 #define TG_FLAG_PLAIN        1   /* one gauge angle per launch sequence, no batching */
@@ -243,7 +244,8 @@ int tg_set_constraints(tg_ctx* ctx, int constraint_dim, int constraint_pair,
  * construct_canonical_basis, src/module_wavefunctions.f90) must precede the
  * next tg_iterate.  A failed Cholesky factorisation returns TG_E_SINGULAR where
  * the reference stops; a singular multiplier system keeps the previous
- * multipliers (:471-477).
+ * multipliers (:471-477).  tg_canonical_basis sets the new states without
+ * leaving the device.
  */
 int tg_iterate(tg_ctx* ctx, double gradient_eps, double* pnp_scalars, double* gradient_norm,
                double* lagrange_lambda1, int* adjust_iterations, int* converged);
@@ -254,6 +256,24 @@ int tg_iterate(tg_ctx* ctx, double gradient_eps, double* pnp_scalars, double* gr
  * tg_init).
  */
 int tg_get_state(tg_ctx* ctx, double* U0, double* V0);
+
+/*
+ * tg_canonical_basis -- construct_canonical_basis (src/module_wavefunctions.f90:
+ * 777-945) of every resident state on the device, in place of tg_get_state +
+ * the host routine + tg_set_states between two tg_iterate calls: rho = V V^T is
+ * diagonalised (ascending eigenvalues), nocc / nemp count |rhoc_ii| >= 1 - eps /
+ * <= eps, ovacc = sum log u over the others, kappac = D^T V U^T D gives the
+ * signs of v/u.  D replaces the resident D0 and the states are set up as
+ * tg_set_states does (the gauge grid, Mphip, Mphin, ... of the last
+ * tg_set_states).  Outputs per state (each pointer may be NULL): nocc[nstates],
+ * nemp[nstates], ovacc[nstates], voveru real[nstates*ndim] (zero past
+ * ndim - nocc).  eps: the reference's cut, 1e-8 by default (seed_occeps).
+ * Returns TG_E_UNSUPPORTED when a degenerate subspace of kappac spans more than
+ * one canonical pair (the reference takes its real Schur form, dgees :877-900):
+ * the states are then left unset and the caller takes the host route.
+ */
+int tg_canonical_basis(tg_ctx* ctx, double eps, int* nocc, int* nemp, double* ovacc,
+                       double* voveru);
 
 /*
  * tg_project_observables -- the gauge-angle loop of project_wavefunction(1, 0,

@@ -264,6 +264,9 @@ struct tg_ctx {
   DevBuf d_it;                                    // iteration work (n*n real matrices)
   DevBuf d_obs;                                   // observables of the final projection
   std::vector<double> lambda_prev;                // multipliers of the previous iteration (ns x dim)
+  // canonical basis on the device (tg_canon.hip): work buffers, rocBLAS handle
+  DevBuf d_canon;
+  void* rb = nullptr;
   // timing: event pairs recorded without synchronisation, summed at collection
   struct Pending { std::string key; hipEvent_t e0, e1; };
   int timing = 0;
@@ -298,6 +301,11 @@ int tg_fetch_results(tg_ctx* c, int state, double* scal, double* pA, double* pH,
                      double* H20v, double* imag);
 // ensures the Z buffers for c->ns states (zeroed when they are (re)sized)
 int tg_ensure_z(tg_ctx* c);
+// The resident U, V, D and c->st of c->ns states are set: the tail of
+// tg_set_states (Z buffers, gauge grid, state marked for the set-up upload).
+int tg_states_ready(tg_ctx* c);
+// releases the rocBLAS handle of the device canonical basis
+void tg_canon_release(tg_ctx* c);
 
 int contraction_build(tg_ctx* c, int64_t ntb, const int16_t* abcd, const void* H2,
                       int h2_bytes, const int8_t* trperm);

@@ -1357,11 +1357,7 @@ int upload_states(tg_ctx* c, int S, const double* U0, const double* V0, const do
   c->valN = valN;
   c->factor_delta = factor_delta;
   c->iprint = iprint ? 1 : 0;
-  if (c->grad_on) TG_TRY(tg_ensure_z(c));
-  assign_angles(c, c->conj_pairs != 0);
-  c->have_state = 1;
-  c->state_dirty = 1;
-  return TG_OK;
+  return tg_states_ready(c);
 }
 
 // assign_angles (:112-185) for the context's Mphip, Mphin, facpi and valence
@@ -1450,6 +1446,13 @@ int get_device_ready() {
 int tg_project_device(tg_ctx* c) { return project_device(c); }
 int tg_project_upload(tg_ctx* c) { return c->state_dirty ? project_upload(c) : TG_OK; }
 int tg_project_run(tg_ctx* c) { return project_run(c); }
+int tg_states_ready(tg_ctx* c) {
+  if (c->grad_on) TG_TRY(tg_ensure_z(c));
+  assign_angles(c, c->conj_pairs != 0);
+  c->have_state = 1;
+  c->state_dirty = 1;
+  return TG_OK;
+}
 int tg_ensure_z(tg_ctx* c) {
   const long long nn = (long long)c->n * c->n;
   const int S = c->ns;
@@ -1585,8 +1588,9 @@ int tg_finalize(tg_ctx* c) {
   graph_release(c);
   for (DevBuf* b : {&c->d_H1, &c->d_U, &c->d_V, &c->d_D, &c->d_ang, &c->d_real6, &c->d_work,
                     &c->d_blk, &c->d_scal, &c->d_out, &c->d_ovl, &c->d_int, &c->d_pfw, &c->d_c20,
-                    &c->d_Z, &c->d_psign, &c->d_cHO, &c->d_it, &c->d_obs})
+                    &c->d_Z, &c->d_psign, &c->d_cHO, &c->d_it, &c->d_obs, &c->d_canon})
     b->release();
+  tg_canon_release(c);
   for (Channel* ch : {&c->ph, &c->pp})
     for (DevBuf* b : {&ch->d_inv, &ch->d_groups, &ch->d_W, &ch->d_tile_group, &ch->d_tile_n0})
       b->release();

@@ -18,7 +18,7 @@ use Hamiltonian
 implicit none
 public
 
-integer(c_int), parameter :: TG_OK = 0, TG_E_IMAG_GRADIENT = -5, TG_E_NOMEM = -3, &
+integer(c_int), parameter :: TG_OK = 0, TG_E_IMAG_GRADIENT = -5, TG_E_NOMEM = -3, TG_E_UNSUPPORTED = -7, &
                              TG_FLAG_PLAIN = 1, TG_FLAG_TBME_FP32 = 2, &
                              TG_FLAG_GRAPH = 4, TG_FLAG_CONJ_PAIRS = 8
 type(c_ptr) :: taurus_gpu_ctx = c_null_ptr
@@ -166,6 +166,16 @@ interface
     import
     type(c_ptr), value :: ctx
     real(c_double), intent(out) :: U0(*), V0(*)
+  end function
+
+  ! construct_canonical_basis of the resident states on the device
+  integer(c_int) function tg_canonical_basis(ctx, eps, nocc, nemp, ovacc, voveru) &
+                                             bind(C, name='tg_canonical_basis')
+    import
+    type(c_ptr), value :: ctx
+    real(c_double), value :: eps
+    integer(c_int), intent(out) :: nocc(*), nemp(*)
+    real(c_double), intent(out) :: ovacc(*), voveru(*)
   end function
 
   ! per-angle observables of project_wavefunction(1, ...) (final printing)

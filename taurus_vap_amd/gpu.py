@@ -39,7 +39,7 @@ EXPORTS = [
     "tg_iterate", "tg_get_state", "tg_set_states", "tg_get_results_state",
     "tg_get_gradient_state", "tg_group_create", "tg_group_destroy", "tg_group_project",
     "tg_group_iterate", "tg_group_iterate_results", "tg_project_observables",
-    "tg_set_batch_limit",
+    "tg_set_batch_limit", "tg_canonical_basis",
 ]
 
 _lib = None
@@ -92,6 +92,7 @@ def load_library():
     lib.tg_group_iterate_results.argtypes = [P, I, P, P, P, P, P]
     lib.tg_project_observables.argtypes = [P, I, P, I, P, I, P, P, P]
     lib.tg_set_batch_limit.argtypes = [P, I]
+    lib.tg_canonical_basis.argtypes = [P, D, P, P, P, P]
     _lib = lib
     return lib
 
@@ -332,6 +333,22 @@ class Context:
         Us = [U[k * n * n:(k + 1) * n * n].reshape((n, n), order="F") for k in range(S)]
         Vs = [V[k * n * n:(k + 1) * n * n].reshape((n, n), order="F") for k in range(S)]
         return (Us[0], Vs[0]) if S == 1 else (Us, Vs)
+
+    def canonical_basis(self, eps=1.0e-8):
+        """construct_canonical_basis of every resident state on the device
+        (tg_canonical_basis): the states are set for the next projection or
+        iteration without leaving the device.  Returns a list of dict(nocc, nemp,
+        ovac, voveru), one per state.  Raises TaurusGPUError (code -7) when a
+        degenerate kappa subspace needs its real Schur form (host route)."""
+        n, S = self.n, self.nstates
+        nocc = np.zeros(S, np.int32)
+        nemp = np.zeros(S, np.int32)
+        ovac = np.zeros(S)
+        vou = np.zeros(S * n)
+        _check(load_library().tg_canonical_basis(self._h, float(eps), _p(nocc), _p(nemp), _p(ovac),
+                                                 _p(vou)), "tg_canonical_basis")
+        return [dict(nocc=int(nocc[k]), nemp=int(nemp[k]), ovac=float(ovac[k]),
+                     voveru=vou[k * n:k * n + n - int(nocc[k])].copy()) for k in range(S)]
 
     def project_observables(self, obo_ops=(), obos_pairs=(), sp_shell=None, nshell=0):
         """Per-angle observables of the final projection (project_wavefunction
