@@ -118,19 +118,19 @@ __global__ void row_sign_kernel(long long tot, int n, const double* __restrict__
   Ds[idx] = sgn[idx % n] * D[idx];
 }
 
-__global__ void conj_kernel(long long tot, cplx* __restrict__ X) {
-  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
-  if (idx < tot) X[idx].y = -X[idx].y;
-}
-
-// dst_a = complex(src of the item's state) for every item (state (b0 + a) / A,
-// state s at src + s * nn)
-__global__ void bcast_real_kernel(long long nn, int na, long long b0, int A,
-                                  const double* __restrict__ src, cplx* __restrict__ dst) {
+// One pass after the A_phi product: A_phi = conj(A_phi), and U~, V~ preset to the
+// real U, V of the item's state (state (b0 + a) / A at U + s * nn) for the
+// GEMMs that add V A_phi and U A_phi
+__global__ void thouless_prep_kernel(long long nn, int na, long long b0, int A,
+                                     const double* __restrict__ U, const double* __restrict__ V,
+                                     cplx* __restrict__ Aphi, cplx* __restrict__ Ut,
+                                     cplx* __restrict__ Vt) {
   long long idx = (long long)blockIdx.x * NT + threadIdx.x;
   if (idx >= nn * na) return;
-  const long long st = (b0 + idx / nn) / A;
-  dst[idx] = cmk(src[st * nn + idx % nn], 0.0);
+  const long long st = (b0 + idx / nn) / A, e = st * nn + idx % nn;
+  Aphi[idx].y = -Aphi[idx].y;
+  Ut[idx] = cmk(U[e], 0.0);
+  Vt[idx] = cmk(V[e], 0.0);
 }
 
 // h_a = Gamma_a + H1
@@ -173,7 +173,10 @@ skew_part_kernel(int n, const cplx* __restrict__ G, cplx* __restrict__ H) {
 // module_hamiltonian.f90:1098-1126).  out[a*9 + k]:
 // 0 prot, 1 neut, 2 Tr(rho_p^2), 3 Tr(kRL_p^T kLR_p), 4 Tr(rho_n^2),
 // 5 Tr(kRL_n^T kLR_n), 6 Tr(H1 rho), 7 Tr(Gamma rho), 8 Tr(DeltaLR kRL^T)
-__global__ void __launch_bounds__(NT)
+// One workgroup of TR_NT threads per angle (sixteen waves keep enough loads in
+// flight for the one CU that sums an angle).
+constexpr int TR_NT = 1024;
+__global__ void __launch_bounds__(TR_NT)
 traces_kernel(int n, const cplx* __restrict__ H1, const cplx* __restrict__ rho,
               const cplx* __restrict__ kLR, const cplx* __restrict__ kRL,
               const cplx* __restrict__ G, const cplx* __restrict__ dLR, long long stride,
@@ -188,7 +191,7 @@ traces_kernel(int n, const cplx* __restrict__ H1, const cplx* __restrict__ rho,
   cplx acc[9];
   for (int k = 0; k < 9; ++k) acc[k] = cmk(0.0, 0.0);
   const long long nn = (long long)n * n;
-  for (long long idx = threadIdx.x; idx < nn; idx += NT) {
+  for (long long idx = threadIdx.x; idx < nn; idx += TR_NT) {
     const int i = (int)(idx % n), j = (int)(idx / n);
     const long long t = j + (long long)i * n;   // (j,i)
     const cplx rji = r_[t];
@@ -202,11 +205,11 @@ traces_kernel(int n, const cplx* __restrict__ H1, const cplx* __restrict__ rho,
     cfma(acc[7], g_[idx], rji);
     cfma(acc[8], dl[idx], kr[idx]);
   }
-  __shared__ cplx red[NT];
+  __shared__ cplx red[TR_NT];
   for (int k = 0; k < 9; ++k) {
     red[threadIdx.x] = acc[k];
     __syncthreads();
-    for (int s = NT / 2; s > 0; s >>= 1) {
+    for (int s = TR_NT / 2; s > 0; s >>= 1) {
       if ((int)threadIdx.x < s) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + s]);
       __syncthreads();
     }
@@ -1126,11 +1129,8 @@ int project_run(tg_ctx* c) {
         ScopedTiming tz(c, "kernel:zgemm");
         TG_TRY(gemm_zz(s, OP_N, OP_N, n, n, n, one, Vphi, n, nn, Uphi, n, nn, zero, Aphi, n, nn, na));
       }
-      hipLaunchKernelGGL(conj_kernel, dim3(nblk(tot)), dim3(NT), 0, s, tot, Aphi);
-      hipLaunchKernelGGL(bcast_real_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, (long long)a0, A,
-                         U, Ut);
-      hipLaunchKernelGGL(bcast_real_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na, (long long)a0, A,
-                         V, Vt);
+      hipLaunchKernelGGL(thouless_prep_kernel, dim3(nblk(tot)), dim3(NT), 0, s, nn, na,
+                         (long long)a0, A, U, V, Aphi, Ut, Vt);
       TG_HIP(hipGetLastError());
       TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, Vc, n, nn, Aphi, n, nn, one, Ut, n, nn, na, A, 1));
       TG_TRY(gemm_dz(s, OP_N, OP_N, n, n, n, one, Uc, n, nn, Aphi, n, nn, one, Vt, n, nn, na, A, 1));
@@ -1146,7 +1146,7 @@ int project_run(tg_ctx* c) {
                              c->d_H1.as<cplx>(), hsp));
     }
     // particle numbers and energy
-    hipLaunchKernelGGL(traces_kernel, dim3(na), dim3(NT), 0, s, n, c->d_H1.as<cplx>(), rho, kLR,
+    hipLaunchKernelGGL(traces_kernel, dim3(na), dim3(TR_NT), 0, s, n, c->d_H1.as<cplx>(), rho, kLR,
                        kRL, G, dLR, nn, P.d_tr + 9 * (size_t)a0);
     TG_HIP(hipGetLastError());
     // H20 (calculate_H20 with U~, V~) as G - G^T,
