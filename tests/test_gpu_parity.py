@@ -3,7 +3,7 @@
 The expected values are the reference TAURUS_vap's own outputs (golden fixtures
 written by oracle/_ref through oracle/ref_harness.f90) and, where a fixture does
 not cover a case, the CPU oracle (oracle/pnvap.py) on the same seeded inputs.
-Tolerances (north-star bar): projected energy 1e-8 relative, fields and the
+Tolerances: projected energy 1e-8 relative, fields and the
 gradient 1e-10 absolute.
 """
 import numpy as np
