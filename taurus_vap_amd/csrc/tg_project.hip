@@ -80,34 +80,46 @@ inline unsigned nblk(long long tot) { return (unsigned)((tot + NT - 1) / NT); }
 // U_phi, V_phi of every item from the six real matrices of its state
 // (R6 = [Kp Kn Lp Ln Mp Mn], Kx = U^T Pi_x U, Lx = V^T Pi_x V, Mx = V^T Pi_x U;
 // product k of state s at (k * S + s) * n^2).  Item b0 + a is angle (b0 + a) % A
-// of state (b0 + a) / A.
+// of state (b0 + a) / A.  A thread takes one element for UV_AG consecutive
+// items, so the real matrices (two of them read transposed) are loaded once per
+// state rather than once per item.
+constexpr int UV_AG = 16;
 __global__ void build_uv_phi_kernel(int n, int na, long long b0, int A, int S,
                                     const double* __restrict__ R6, const cplx* __restrict__ ph,
                                     cplx* __restrict__ Uphi, cplx* __restrict__ Vphi,
                                     long long stride) {
   const long long nn = (long long)n * n;
-  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
-  if (idx >= nn * na) return;
-  const long long a = idx / nn;
-  const long long b = b0 + a;
-  const long long st = b / A, ang = b % A;
-  const int r = (int)(idx % nn);
-  const int i = r % n, j = r / n;
+  const long long r = (long long)blockIdx.x * NT + threadIdx.x;
+  if (r >= nn) return;
+  const int i = (int)(r % n), j = (int)(r / n);
   const long long rt = j + (long long)i * n;
-  const cplx ep = ph[2 * ang], en = ph[2 * ang + 1];
-  const double* R = R6 + st * nn;
   const long long ks = (long long)S * nn;
-  const double Kp = R[r], Kn = R[ks + r], Lp = R[2 * ks + r], Ln = R[3 * ks + r];
-  const double Mp = R[4 * ks + r], Mn = R[5 * ks + r], Mpt = R[4 * ks + rt], Mnt = R[5 * ks + rt];
-  cplx u, v;
-  // e_p Kp + e_n Kn + conj(e_p) Lp + conj(e_n) Ln
-  u.x = ep.x * (Kp + Lp) + en.x * (Kn + Ln);
-  u.y = ep.y * (Kp - Lp) + en.y * (Kn - Ln);
-  // e_p Mp + e_n Mn + conj(e_p) Mp^T + conj(e_n) Mn^T
-  v.x = ep.x * (Mp + Mpt) + en.x * (Mn + Mnt);
-  v.y = ep.y * (Mp - Mpt) + en.y * (Mn - Mnt);
-  Uphi[a * stride + r] = u;
-  Vphi[a * stride + r] = v;
+  const int a1 = min(na, (int)(blockIdx.y + 1) * UV_AG);
+  long long cur = -1;
+  double kp = 0, km = 0, kn = 0, knm = 0, mp = 0, mpm = 0, mn = 0, mnm = 0;
+  for (int a = blockIdx.y * UV_AG; a < a1; ++a) {
+    const long long b = b0 + a;
+    const long long st = b / A, ang = b % A;
+    if (st != cur) {
+      const double* R = R6 + st * nn;
+      const double Kp = R[r], Kn = R[ks + r], Lp = R[2 * ks + r], Ln = R[3 * ks + r];
+      const double Mp = R[4 * ks + r], Mn = R[5 * ks + r];
+      const double Mpt = R[4 * ks + rt], Mnt = R[5 * ks + rt];
+      kp = Kp + Lp; km = Kp - Lp; kn = Kn + Ln; knm = Kn - Ln;
+      mp = Mp + Mpt; mpm = Mp - Mpt; mn = Mn + Mnt; mnm = Mn - Mnt;
+      cur = st;
+    }
+    const cplx ep = ph[2 * ang], en = ph[2 * ang + 1];
+    cplx u, v;
+    // e_p Kp + e_n Kn + conj(e_p) Lp + conj(e_n) Ln
+    u.x = ep.x * kp + en.x * kn;
+    u.y = ep.y * km + en.y * knm;
+    // e_p Mp + e_n Mn + conj(e_p) Mp^T + conj(e_n) Mn^T
+    v.x = ep.x * mp + en.x * mn;
+    v.y = ep.y * mpm + en.y * mnm;
+    Uphi[a * stride + r] = u;
+    Vphi[a * stride + r] = v;
+  }
 }
 
 // Ds(i,j) = sgn(i) D(i,j)
@@ -173,49 +185,76 @@ skew_part_kernel(int n, const cplx* __restrict__ G, cplx* __restrict__ H) {
 // module_hamiltonian.f90:1098-1126).  out[a*9 + k]:
 // 0 prot, 1 neut, 2 Tr(rho_p^2), 3 Tr(kRL_p^T kLR_p), 4 Tr(rho_n^2),
 // 5 Tr(kRL_n^T kLR_n), 6 Tr(H1 rho), 7 Tr(Gamma rho), 8 Tr(DeltaLR kRL^T)
-// One workgroup of TR_NT threads per angle (sixteen waves keep enough loads in
-// flight for the one CU that sums an angle).
-constexpr int TR_NT = 1024;
-__global__ void __launch_bounds__(TR_NT)
+// A workgroup sums the elements (i, j) of one band of 32 rows i of one angle,
+// 32 columns j at a time; rho(j, i) comes from a 32 x 32 tile of rho staged in
+// LDS, so every global read is coalesced.  part[(a * bands + band) * 9 + k];
+// traces_sum_kernel adds the bands.
+constexpr int TRT = 32;
+__global__ void __launch_bounds__(NT)
 traces_kernel(int n, const cplx* __restrict__ H1, const cplx* __restrict__ rho,
               const cplx* __restrict__ kLR, const cplx* __restrict__ kRL,
               const cplx* __restrict__ G, const cplx* __restrict__ dLR, long long stride,
-              cplx* __restrict__ out) {
-  const long long a = blockIdx.x;
-  const int h = n / 2;
+              cplx* __restrict__ part) {
+  const int band = blockIdx.x, bands = gridDim.x;
+  const long long a = blockIdx.y;
+  const int i0 = band * TRT, h = n / 2;
+  const int tx = threadIdx.x % TRT, ty = threadIdx.x / TRT;
   const cplx* r_ = rho + a * stride;
   const cplx* kl = kLR + a * stride;
   const cplx* kr = kRL + a * stride;
   const cplx* g_ = G + a * stride;
   const cplx* dl = dLR + a * stride;
+  __shared__ cplx T[TRT][TRT + 1];   // T[jj][ii] = rho(j0 + jj, i0 + ii)
   cplx acc[9];
   for (int k = 0; k < 9; ++k) acc[k] = cmk(0.0, 0.0);
-  const long long nn = (long long)n * n;
-  for (long long idx = threadIdx.x; idx < nn; idx += TR_NT) {
-    const int i = (int)(idx % n), j = (int)(idx / n);
-    const long long t = j + (long long)i * n;   // (j,i)
-    const cplx rji = r_[t];
-    const bool pi = i < h, pj = j < h;
-    if (i == j) acc[pi ? 0 : 1] = cadd(acc[pi ? 0 : 1], r_[idx]);
-    if (pi == pj) {
-      cfma(acc[pi ? 2 : 4], r_[idx], rji);
-      cfma(acc[pi ? 3 : 5], kr[idx], kl[idx]);
+  const int i = i0 + tx;
+  const bool pi = i < h;
+  for (int j0 = 0; j0 < n; j0 += TRT) {
+    for (int ii = ty; ii < TRT; ii += NT / TRT) {
+      const int jr = j0 + tx, ir = i0 + ii;
+      T[tx][ii] = (ir < n && jr < n) ? r_[jr + (long long)ir * n] : cmk(0.0, 0.0);
     }
-    cfma(acc[6], H1[idx], rji);
-    cfma(acc[7], g_[idx], rji);
-    cfma(acc[8], dl[idx], kr[idx]);
+    __syncthreads();
+    if (i < n)
+      for (int jj = ty; jj < TRT; jj += NT / TRT) {
+        const int j = j0 + jj;
+        if (j >= n) break;
+        const long long idx = i + (long long)j * n;
+        const cplx rji = T[jj][tx];
+        const bool pj = j < h;
+        if (i == j) acc[pi ? 0 : 1] = cadd(acc[pi ? 0 : 1], r_[idx]);
+        if (pi == pj) {
+          cfma(acc[pi ? 2 : 4], r_[idx], rji);
+          cfma(acc[pi ? 3 : 5], kr[idx], kl[idx]);
+        }
+        cfma(acc[6], H1[idx], rji);
+        cfma(acc[7], g_[idx], rji);
+        cfma(acc[8], dl[idx], kr[idx]);
+      }
+    __syncthreads();
   }
-  __shared__ cplx red[TR_NT];
+  __shared__ cplx red[NT];
   for (int k = 0; k < 9; ++k) {
     red[threadIdx.x] = acc[k];
     __syncthreads();
-    for (int s = TR_NT / 2; s > 0; s >>= 1) {
+    for (int s = NT / 2; s > 0; s >>= 1) {
       if ((int)threadIdx.x < s) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + s]);
       __syncthreads();
     }
-    if (threadIdx.x == 0) out[a * 9 + k] = red[0];
+    if (threadIdx.x == 0) part[(a * bands + band) * 9 + k] = red[0];
     __syncthreads();
   }
+}
+
+// out[a*9 + k] = sum over the bands of part
+__global__ void traces_sum_kernel(int na, int bands, const cplx* __restrict__ part,
+                                  cplx* __restrict__ out) {
+  const int idx = blockIdx.x * NT + threadIdx.x;
+  if (idx >= na * 9) return;
+  const int a = idx / 9, k = idx % 9;
+  cplx s = cmk(0.0, 0.0);
+  for (int b = 0; b < bands; ++b) s = cadd(s, part[((long long)a * bands + b) * 9 + k]);
+  out[idx] = s;
 }
 
 // Q = T (e_p Pp + e_n Pn)[nemp:, nemp:] T and JQ = J Q (reduced overlap).
@@ -754,6 +793,7 @@ struct RunPlan {
   std::vector<OvlShape> sh;
   cplx *work, *Akeep, *Hkeep;
   cplx *d_ph, *d_tr, *d_over, *d_ener, *d_f12, *d_pnp, *d_pre, *d_det, *d_phase, *d_w, *d_pari;
+  cplx* d_trp;   // per-band partial traces of one chunk
   double *d_logabs, *d_imag, *d_mult;
   int *d_info, *d_zero, *d_perm;
   double *d_T, *d_sg, *d_ovac, *d_vou;      // overlap set-up of state 0 (state s at + s * os)
@@ -799,9 +839,12 @@ RunPlan make_plan(tg_ctx* c) {
   P.work_bytes = sizeof(cplx) * (wsz + 2 * (size_t)B * nn);
   // scalars: ph (2A) | traces (9B) | over (B) | ener (B) | f12 (2B) | pnp (8S) | pre (B)
   //          | det (B) | phase (B) | w (A) | pari (B) | logabs (B doubles)
-  //          | imag (S * is doubles) | mult (A doubles)
+  //          | imag (S * is doubles) | mult (A doubles) | pad to 16 bytes
+  //          | partial traces (9 * chunk * bands)
+  const size_t bands = (n + TRT - 1) / TRT;
   P.scal_bytes = sizeof(cplx) * (3 * (size_t)A + 17 * (size_t)B + 8 * (size_t)S) +
-                 sizeof(double) * ((size_t)B + (size_t)S * P.is + A);
+                 sizeof(double) * ((size_t)B + (size_t)S * P.is + A + 1) +
+                 sizeof(cplx) * 9 * (size_t)P.chunk * bands;
   P.int_bytes = sizeof(int) * ((size_t)B * (n + 2) + 8);
   // overlap set-up per state: T (n) | sg (n) | ovac (2) | pre0 (cplx) | v/u (n)
   P.ovl_bytes = sizeof(double) * (size_t)S * P.os;
@@ -839,6 +882,10 @@ RunPlan make_plan(tg_ctx* c) {
   P.d_logabs = reinterpret_cast<double*>(P.d_pari + B);
   P.d_imag = P.d_logabs + B;
   P.d_mult = P.d_imag + (size_t)S * P.is;
+  {
+    const size_t end = (size_t)((const char*)(P.d_mult + A) - (const char*)c->d_scal.p);
+    P.d_trp = reinterpret_cast<cplx*>((char*)c->d_scal.p + ((end + 15) / 16) * 16);
+  }
   P.d_info = c->d_int.as<int>();
   P.d_zero = P.d_info + B;
   P.d_perm = P.d_zero + B;
@@ -1112,7 +1159,8 @@ int project_run(tg_ctx* c) {
     // Thouless matrix A_phi (calculate_thouless :684-718)
     {
       ScopedTiming tm(c, "thouless_inverse");
-      hipLaunchKernelGGL(build_uv_phi_kernel, dim3(nblk(tot)), dim3(NT), 0, s, n, na, (long long)a0,
+      hipLaunchKernelGGL(build_uv_phi_kernel, dim3(nblk(nn), (na + UV_AG - 1) / UV_AG), dim3(NT), 0,
+                         s, n, na, (long long)a0,
                          A, S, R6, P.d_ph, Uphi, Vphi, nn);
       TG_HIP(hipGetLastError());
       ScopedTiming tk(c, "kernel:gj_inverse");
@@ -1146,8 +1194,13 @@ int project_run(tg_ctx* c) {
                              c->d_H1.as<cplx>(), hsp));
     }
     // particle numbers and energy
-    hipLaunchKernelGGL(traces_kernel, dim3(na), dim3(TR_NT), 0, s, n, c->d_H1.as<cplx>(), rho, kLR,
-                       kRL, G, dLR, nn, P.d_tr + 9 * (size_t)a0);
+    {
+      const int bands = (n + TRT - 1) / TRT;
+      hipLaunchKernelGGL(traces_kernel, dim3(bands, na), dim3(NT), 0, s, n, c->d_H1.as<cplx>(), rho,
+                         kLR, kRL, G, dLR, nn, P.d_trp);
+      hipLaunchKernelGGL(traces_sum_kernel, dim3(nblk(9LL * na)), dim3(NT), 0, s, na, bands, P.d_trp,
+                         P.d_tr + 9 * (size_t)a0);
+    }
     TG_HIP(hipGetLastError());
     // H20 (calculate_H20 with U~, V~) as G - G^T,
     //   X1 = h V~ + fd/2 DeltaLR U~,  X2 = -fd/2 DeltaRL V~,  G = U~^T X1 + V~^T X2
