@@ -268,8 +268,10 @@ int tg_get_state(tg_ctx* ctx, double* U0, double* V0);
  * tg_set_states).  Outputs per state (each pointer may be NULL): nocc[nstates],
  * nemp[nstates], ovacc[nstates], voveru real[nstates*ndim] (zero past
  * ndim - nocc).  eps: the reference's cut, 1e-8 by default (seed_occeps).
- * Returns TG_E_UNSUPPORTED when a degenerate subspace of kappac spans more than
- * one canonical pair (the reference takes its real Schur form, dgees :877-900):
+ * A degenerate subspace of kappac wider than one canonical pair (the
+ * reference's real Schur step, dgees :877-900) is brought to canonical form by
+ * an orthogonal transformation of its block (any canonical form gives the same
+ * projected quantities).  Returns TG_E_UNSUPPORTED for a subspace wider than 64:
  * the states are then left unset and the caller takes the host route.
  */
 int tg_canonical_basis(tg_ctx* ctx, double eps, int* nocc, int* nemp, double* ovacc,
