@@ -58,9 +58,17 @@ def algorithmic_flops(ctx_info, n, A, nb):
         # complex batched GEMMs of the loop body on the main stream: A_phi (n^3),
         # the two H20 products over the inner dimension 2n and the one over n
         "kernel:zgemm": 8.0 * A * 6 * n ** 3,
-        # the reduced-overlap product Q^T (J Q) on the second stream (nb^3)
-        "kernel:zgemm_q": 8.0 * A * nb ** 3,
+        # the reduced-overlap product Q^T (J Q) on the second stream: its upper
+        # row bands (the skew result's upper triangle is what the Pfaffian reads)
+        "kernel:zgemm_q": 8.0 * A * nb * sum(min(rb, nb - r0) * (nb - r0)
+                                             for rb in [q_band_rows(nb)]
+                                             for r0 in range(0, nb, rb)),
     }
+
+
+def q_band_rows(nb):
+    """Row band height of the reduced-overlap product (as the library's)."""
+    return (((nb + 2) // 3 + 15) // 16) * 16 if nb >= 128 else nb
 
 
 def measured_traffic(workload, angles_per_launch, kernel):

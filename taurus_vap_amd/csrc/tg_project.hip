@@ -181,6 +181,11 @@ skew_part_kernel(int n, const cplx* __restrict__ G, cplx* __restrict__ H) {
   }
 }
 
+// Row bands of the reduced-overlap product K = Q^T (J Q) (nb x nb): three bands of
+// a multiple of 16 rows (even, so no canonical pair straddles two bands) from
+// nb = 128 on, one band below.
+inline int q_band_rows(int nb) { return nb >= 128 ? (((nb + 2) / 3 + 15) / 16) * 16 : nb; }
+
 // Per-angle traces (calculate_particle_number :78-143, calculate_expectval_energy
 // module_hamiltonian.f90:1098-1126).  out[a*9 + k]:
 // 0 prot, 1 neut, 2 Tr(rho_p^2), 3 Tr(kRL_p^T kLR_p), 4 Tr(rho_n^2),
@@ -980,9 +985,12 @@ int project_upload(tg_ctx* c) {
     TG_TRY(gemm_tune_dz(s, OP_N, OP_T, n, n, n, U, n, 0, Xa, n, nn, Ya, n, nn, chunk));
     TG_TRY(gemm_tune_zd(s, OP_N, OP_T, n, n, n, Xa, n, nn, U, n, 0, Ya, n, nn, chunk));
     if (nb0 > 0) {
-      const int nb = nb0;
+      const int nb = nb0, rb = q_band_rows(nb);
       const long long mm = (long long)nb * nb;
-      TG_TRY(gemm_tune_zz(s, OP_T, OP_N, nb, nb, nb, Xa, nb, mm, Ta, nb, mm, Ya, nb, mm, chunk));
+      for (int r0 = 0; r0 < nb; r0 += rb)
+        TG_TRY(gemm_tune_zz(s, OP_T, OP_N, std::min(rb, nb - r0), nb - r0, nb, Xa + (size_t)r0 * nb, nb,
+                            mm, Ta + (size_t)r0 * nb, nb, mm, Ya + r0 + (size_t)r0 * nb, nb, mm,
+                            chunk));
     }
     c->tuned_n = n;
     c->tuned_chunk = chunk;
@@ -1086,8 +1094,14 @@ int project_run(tg_ctx* c) {
                            b0, A, Pa, Pb, P.d_T, P.d_sg, os, P.d_ph, Q, JQ);
         TG_HIP(hipGetLastError());
         {
+          // K = Q^T (J Q) is skew and the Pfaffian reads its upper triangle only:
+          // the product runs in bands of rows, each from its diagonal block on
           ScopedTiming tz(c, "kernel:zgemm_q", s2);
-          TG_TRY(gemm_zz(s2, OP_T, OP_N, nb, nb, nb, one, Q, nb, mm, JQ, nb, mm, zero, K, nb, mm, cnt));
+          const int rb = q_band_rows(nb);
+          for (int r0 = 0; r0 < nb; r0 += rb)
+            TG_TRY(gemm_zz(s2, OP_T, OP_N, std::min(rb, nb - r0), nb - r0, nb, one,
+                           Q + (size_t)r0 * nb, nb, mm, JQ + (size_t)r0 * nb, nb, mm, zero,
+                           K + r0 + (size_t)r0 * nb, nb, mm, cnt));
         }
         hipLaunchKernelGGL(add_j_kernel, dim3(nblk((long long)(nb / 2) * cnt)), dim3(NT), 0, s2, nb,
                            cnt, b0, A, P.d_sg, os, K);
