@@ -340,6 +340,28 @@ def copy_probe(mib: int):
         % (mib, nel * 8, nel * 8))
 
 
+def dgemm_probe(n: int):
+    """The vendor FP64 GEMM (torch.matmul -> rocBLAS / hipBLASLt) on two n x n
+    matrices: the FP64 matrix-core rate a library GEMM sustains on this part."""
+    import torch
+    a = torch.rand(n, n, dtype=torch.float64, device="cuda")
+    b = torch.rand(n, n, dtype=torch.float64, device="cuda")
+    c = torch.empty_like(a)
+    for _ in range(3):
+        torch.matmul(a, b, out=c)
+    torch.cuda.synchronize()
+    reps = 10
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch.matmul(a, b, out=c)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    log("dgemm probe: n = %d, %.3f ms per product, %.2f TFLOP/s (FP64)"
+        % (n, ms, 2.0 * n ** 3 / ms / 1e9))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -366,6 +388,8 @@ def main():
                     help="timed steps of the 64 batched independent solves (0: skip)")
     ap.add_argument("--iterate-steps", type=int, default=3,
                     help="timed full device iterations with the constraint machinery (0: skip)")
+    ap.add_argument("--dgemm-probe", type=int, default=0, metavar="N",
+                    help="time the vendor FP64 GEMM on N x N matrices and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (calibration "
                          "of the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -374,6 +398,9 @@ def main():
         print("bench.py runs on one GPU only: the batched gauge-angle path is measured on a "
               "single MI355X (--gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.dgemm_probe:
+        dgemm_probe(args.dgemm_probe)
+        return
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
