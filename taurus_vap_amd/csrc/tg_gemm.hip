@@ -280,6 +280,9 @@ int tune(hipStream_t s, const GemmKey& key, int nvar, F run) {
     if (hipEventSynchronize(e1) != hipSuccess) rc = TG_E_HIP;
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
+    if (rc == TG_OK && std::getenv("TG_GEMM_VERBOSE"))
+      fprintf(stderr, "[taurus_gpu]   kind %d ops %d%d M %d N %d K %d batch %d tile %d: %.3f ms\n",
+              key.kind, key.opa, key.opb, key.M, key.N, key.K, key.batch, v, ms / 2);
     if (rc == TG_OK && ms < best_ms) {
       best_ms = ms;
       best = v;

@@ -80,7 +80,8 @@ struct Tile {
     if (BJ) { k = e % BK; j = e / BK; } else { j = e % BN; k = e / BN; }
   }
 
-  __device__ static void run(int M, int N, int K, cplx alpha, const TA* __restrict__ A, int lda,
+  // always inlined: several kernels share one tile body
+  __device__ __forceinline__ static void run(int M, int N, int K, cplx alpha, const TA* __restrict__ A, int lda,
                              const TB* __restrict__ B, int ldb, cplx beta, bool use_beta,
                              cplx* __restrict__ C, int ldc, int m0, int n0, Smem& sm) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -201,20 +202,36 @@ struct Tile {
   }
 };
 
-// Batched: grid = XCDS * ceil(batch / XCDS) * tiles_per_matrix workgroups.
-template <typename TA, typename TB, int OPA, int OPB, int BM, int BN, int WM, int WN>
-__global__ void __launch_bounds__(64 * WM * WN)
+// Batched: grid = XCDS * per workgroups.  BAL = false: batch element b goes to
+// XCD b % 8 (per = ceil(batch / 8) * tiles); BAL = true: the batch * tiles
+// (element, tile) pairs are cut into 8 contiguous runs of `per`, one per XCD, so
+// every XCD gets the same number of tiles.  Either way the tiles of one element
+// are consecutive on their XCD.  OCC > 1 bounds the registers for OCC times the
+// workgroup's waves per CU.
+template <typename TA, typename TB, int OPA, int OPB, int BM, int BN, int WM, int WN, bool BAL,
+          int OCC>
+__global__ void __launch_bounds__(64 * WM * WN * OCC)
 mfma_batched_kernel(int M, int N, int K, cplx alpha, const TA* __restrict__ A, int lda,
                     long long sA, const TB* __restrict__ B, int ldb, long long sB, cplx beta,
                     int use_beta, cplx* __restrict__ C, int ldc, long long sC, int batch,
-                    int tiles_m, int tiles, int divA, int divB) {
+                    int tiles_m, int tiles, long long per, int divA, int divB) {
   typedef Tile<TA, TB, OPA, OPB, BM, BN, WM, WN> T;
   __shared__ typename T::Smem sm;
   const int L = blockIdx.x;
   const int xcd = L % XCDS, q = L / XCDS;
-  const int slot = q / tiles, tile = q % tiles;
-  const long long b = xcd + (long long)XCDS * slot;
-  if (b >= batch) return;
+  long long b;
+  int tile;
+  if (BAL) {
+    const long long g = (long long)xcd * per + q;
+    if (g >= (long long)batch * tiles) return;
+    b = g / tiles;
+    tile = (int)(g - b * tiles);
+  } else {
+    const int slot = q / tiles;
+    tile = q % tiles;
+    b = xcd + (long long)XCDS * slot;
+    if (b >= batch) return;
+  }
   const int tm = tile % tiles_m, tn = tile / tiles_m;
   T::run(M, N, K, alpha, A + (b / divA) * sA, lda, B + (b / divB) * sB, ldb, beta, use_beta != 0,
          C + b * sC, ldc, tm * BM, tn * BN, sm);
@@ -235,27 +252,28 @@ mfma_grouped_kernel(int M, const cplx* __restrict__ X, cplx* __restrict__ C,
          cmk(0.0, 0.0), false, C + (long long)g.off * M, M, blockIdx.y * BM, n0, sm);
 }
 
-template <typename TA, typename TB, int OPA, int OPB, int BM, int BN, int WM, int WN>
+template <typename TA, typename TB, int OPA, int OPB, int BM, int BN, int WM, int WN,
+          bool BAL = false, int OCC = 1>
 int launch_batched(hipStream_t s, int M, int N, int K, cplx alpha, const TA* A, int lda,
                    long long sA, const TB* B, int ldb, long long sB, cplx beta, int use_beta,
                    cplx* C, int ldc, long long sC, int batch, int divA, int divB) {
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   const int tiles = tiles_m * tiles_n;
-  const long long slots = (batch + XCDS - 1) / XCDS;
-  const long long grid = (long long)XCDS * slots * tiles;
+  const long long per = BAL ? ((long long)batch * tiles + XCDS - 1) / XCDS
+                            : (long long)((batch + XCDS - 1) / XCDS) * tiles;
+  const long long grid = (long long)XCDS * per;
   if (grid > 0x7fffffffLL) {
     tg_set_error("mfma gemm: grid too large");
     return TG_E_ARG;
   }
-  hipLaunchKernelGGL((mfma_batched_kernel<TA, TB, OPA, OPB, BM, BN, WM, WN>), dim3((unsigned)grid),
-                     dim3(64 * WM * WN), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
-                     use_beta, C, ldc, sC, batch, tiles_m, tiles, divA, divB);
+  if (grid == 0) return TG_OK;
+  hipLaunchKernelGGL((mfma_batched_kernel<TA, TB, OPA, OPB, BM, BN, WM, WN, BAL, OCC>),
+                     dim3((unsigned)grid), dim3(64 * WM * WN), 0, s, M, N, K, alpha, A, lda, sA, B,
+                     ldb, sB, beta, use_beta, C, ldc, sC, batch, tiles_m, tiles, per, divA, divB);
   TG_HIP(hipGetLastError());
   return TG_OK;
 }
 
-// tile variants: 0 = 64x64 (2x2 waves), 1 = 48x48 (1x3 waves), 2 = 112x112 (1x7 waves),
-// 3 = 96x64 (2x2 waves)
 template <typename TA, typename TB, int OPA, int OPB>
 int launch_variant(int variant, hipStream_t s, int M, int N, int K, cplx alpha, const TA* A,
                    int lda, long long sA, const TB* B, int ldb, long long sB, cplx beta,
@@ -275,6 +293,18 @@ int launch_variant(int variant, hipStream_t s, int M, int N, int K, cplx alpha, 
       return launch_batched<TA, TB, OPA, OPB, 96, 64, 2, 2>(s, M, N, K, alpha, A, lda, sA, B, ldb,
                                                             sB, beta, use_beta, C, ldc, sC, batch,
                                                             divA, divB);
+    case 4:
+      return launch_batched<TA, TB, OPA, OPB, 112, 112, 1, 7, true>(
+          s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, use_beta, C, ldc, sC, batch, divA, divB);
+    case 5:
+      return launch_batched<TA, TB, OPA, OPB, 48, 48, 1, 3, true>(
+          s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, use_beta, C, ldc, sC, batch, divA, divB);
+    case 6:
+      return launch_batched<TA, TB, OPA, OPB, 48, 48, 1, 3, false, 4>(
+          s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, use_beta, C, ldc, sC, batch, divA, divB);
+    case 7:
+      return launch_batched<TA, TB, OPA, OPB, 48, 48, 1, 3, true, 4>(
+          s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, use_beta, C, ldc, sC, batch, divA, divB);
   }
   tg_set_error("mfma gemm: bad tile variant");
   return TG_E_ARG;
