@@ -238,8 +238,8 @@ mfma_batched_kernel(int M, int N, int K, cplx alpha, const TA* __restrict__ A, i
 }
 
 // Grouped contraction: C_g (M x s_g) = X_g (M x s_g) W_g (s_g x s_g), W real (float/double).
-template <typename TW, int BM, int BN, int WM, int WN>
-__global__ void __launch_bounds__(64 * WM * WN)
+template <typename TW, int BM, int BN, int WM, int WN, int OCC = 1>
+__global__ void __launch_bounds__(64 * WM * WN * OCC)
 mfma_grouped_kernel(int M, const cplx* __restrict__ X, cplx* __restrict__ C,
                     const TW* __restrict__ W, const GroupDesc* __restrict__ groups,
                     const int* __restrict__ tile_group, const int* __restrict__ tile_n0) {
@@ -383,6 +383,22 @@ int mfma_gemm_grouped(int variant, hipStream_t s, int M, const cplx* X, cplx* C,
                          C, (const float*)W, groups_dev, tile_group_dev, tile_n0_dev);
     else
       hipLaunchKernelGGL((mfma_grouped_kernel<double, 256, 64, 4, 2>), grid, dim3(512), 0, s, M,
+                         X, C, (const double*)W, groups_dev, tile_group_dev, tile_n0_dev);
+  } else if (variant == 3) {
+    dim3 grid(ntiles_total, (M + 127) / 128, 1);
+    if (w_is_fp32)
+      hipLaunchKernelGGL((mfma_grouped_kernel<float, 128, 64, 4, 1, 2>), grid, dim3(256), 0, s, M,
+                         X, C, (const float*)W, groups_dev, tile_group_dev, tile_n0_dev);
+    else
+      hipLaunchKernelGGL((mfma_grouped_kernel<double, 128, 64, 4, 1, 2>), grid, dim3(256), 0, s, M,
+                         X, C, (const double*)W, groups_dev, tile_group_dev, tile_n0_dev);
+  } else if (variant == 4) {
+    dim3 grid(ntiles_total, (M + 63) / 64, 1);
+    if (w_is_fp32)
+      hipLaunchKernelGGL((mfma_grouped_kernel<float, 64, 64, 2, 2, 3>), grid, dim3(256), 0, s, M, X,
+                         C, (const float*)W, groups_dev, tile_group_dev, tile_n0_dev);
+    else
+      hipLaunchKernelGGL((mfma_grouped_kernel<double, 64, 64, 2, 2, 3>), grid, dim3(256), 0, s, M,
                          X, C, (const double*)W, groups_dev, tile_group_dev, tile_n0_dev);
   } else {
     tg_set_error("mfma grouped gemm: bad tile variant");
