@@ -35,6 +35,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "PNVAP gradient iters/s (e_max=6, 15x15 angles) vs plain FP64 GPU, |dE|<1 keV"
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+# timing keys of kernels launched on the second (overlap) stream
+SECOND_STREAM_KERNELS = ("kernel:pfaffian", "kernel:zgemm_q")
 PEAK_HBM_GBS = 8000.0
 # Gauge integral over [0, 2 pi): the seed states carry proton-neutron mixing
 # (their Thouless matrix has proton-neutron entries), so check_symmetries keeps
@@ -428,7 +430,9 @@ def main():
     M = args.angles if args.angles else w.M
     fl = algorithmic_flops(sizes, n, A, nb)
     # every timed kernel (in-run HIP events on the stream each launch ran on);
-    # the dominant one is the kernel with the largest summed time per step
+    # the dominant one is the kernel with the largest summed time per step on the
+    # context stream: the second stream's kernels (overlap chain) share the CUs
+    # with the context stream's, so their event times are not their own
     kern = {k: v for k, v in main_run["timing"].items() if k.startswith("kernel:") and v[1] > 0}
     roofline = None
     by_kernel = {}
@@ -444,7 +448,8 @@ def main():
                            "achieved": ach, "frac": ach / PEAK_FP64_TFLOPS,
                            "traffic": measured_traffic(args.workload, A, name)}
     if kern:
-        dom = max(kern, key=lambda k: kern[k][0])
+        ctx_kern = [k for k in kern if k not in SECOND_STREAM_KERNELS] or list(kern)
+        dom = max(ctx_kern, key=lambda k: kern[k][0])
         d = by_kernel[dom.split(":", 1)[1]]
         roofline = {"bound": "mfma", "kernel": dom.split(":", 1)[1], "achieved": d["achieved"],
                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": d["frac"],
