@@ -348,10 +348,13 @@ int gemm_tune_grouped(hipStream_t s, int M, const cplx* X, cplx* C, const void* 
 
 int gemm_zz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
             const cplx* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
-            cplx beta, cplx* C, int ldc, long long sC, int batch, int divA, int divB) {
+            cplx beta, cplx* C, int ldc, long long sC, int batch, int divA, int divB,
+            int variant) {
   if (M <= 0 || N <= 0 || batch <= 0) return TG_OK;
   if (sel().mfma)
-    return mfma_gemm_zz(variant_for(GemmKey{K_ZZ, (int)opa, (int)opb, M, N, K, batch}), s, opa,
+    return mfma_gemm_zz(variant >= 0 ? variant
+                                     : variant_for(GemmKey{K_ZZ, (int)opa, (int)opb, M, N, K, batch}),
+                        s, opa,
                         opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,
                         !is_zero(beta), C, ldc, sC, batch, divA, divB);
   return dispatch<cplx, cplx, cplx>(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta,

@@ -89,10 +89,11 @@ struct DevBuf {
 enum TgOp { OP_N = 0, OP_T = 1, OP_C = 2 };
 
 // complex = complex x complex
+// variant >= 0 forces a matrix-core tile variant (tg_mfma.h) instead of the tuned one
 int gemm_zz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
             const cplx* A, int lda, long long sA, const cplx* B, int ldb, long long sB,
             cplx beta, cplx* C, int ldc, long long sC, int batch, int divA = 1,
-            int divB = 1);
+            int divB = 1, int variant = -1);
 // complex = real x complex
 int gemm_dz(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, cplx alpha,
             const double* A, int lda, long long sA, const cplx* B, int ldb, long long sB,

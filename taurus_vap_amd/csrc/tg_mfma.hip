@@ -305,6 +305,9 @@ int launch_variant(int variant, hipStream_t s, int M, int N, int K, cplx alpha, 
     case 7:
       return launch_batched<TA, TB, OPA, OPB, 48, 48, 1, 3, true, 4>(
           s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, use_beta, C, ldc, sC, batch, divA, divB);
+    case 8:
+      return launch_batched<TA, TB, OPA, OPB, 64, 32, 1, 2, true, 4>(
+          s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, use_beta, C, ldc, sC, batch, divA, divB);
   }
   tg_set_error("mfma gemm: bad tile variant");
   return TG_E_ARG;

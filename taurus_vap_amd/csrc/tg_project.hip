@@ -1098,10 +1098,16 @@ int project_run(tg_ctx* c) {
           // the product runs in bands of rows, each from its diagonal block on
           ScopedTiming tz(c, "kernel:zgemm_q", s2);
           const int rb = q_band_rows(nb);
+          // The bands run beside the inverse.  The 384-thread inverse (256 < n <= 384)
+          // leaves two SIMDs of each of its CUs half free, which the two-wave 64x32
+          // tile (256 VGPRs, 26 KB LDS) fits; other sizes take the tuned tile.
+          // TG_Q_TILE=k forces variant k (-1: tuned).
+          const char* qe = std::getenv("TG_Q_TILE");
+          const int qtile = qe ? std::atoi(qe) : ((n > 256 && n <= 384) ? 8 : -1);
           for (int r0 = 0; r0 < nb; r0 += rb)
             TG_TRY(gemm_zz(s2, OP_T, OP_N, std::min(rb, nb - r0), nb - r0, nb, one,
                            Q + (size_t)r0 * nb, nb, mm, JQ + (size_t)r0 * nb, nb, mm, zero,
-                           K + r0 + (size_t)r0 * nb, nb, mm, cnt));
+                           K + r0 + (size_t)r0 * nb, nb, mm, cnt, 1, 1, qtile));
         }
         hipLaunchKernelGGL(add_j_kernel, dim3(nblk((long long)(nb / 2) * cnt)), dim3(NT), 0, s2, nb,
                            cnt, b0, A, P.d_sg, os, K);
