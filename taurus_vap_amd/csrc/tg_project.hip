@@ -304,7 +304,7 @@ __global__ void build_q_kernel(int n, int nemp, int nb, int na, long long b0, in
 // K += J (the sgn-weighted symplectic unit; signs of state s at sgn0 + s * os)
 __global__ void add_j_kernel(int nb, int na, long long b0, int A, const double* __restrict__ sgn0,
                              long long os, cplx* __restrict__ K) {
-  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long pairs = (long long)(nb / 2) * na;
   if (idx >= pairs) return;
   const long long a = idx / (nb / 2);
@@ -1109,8 +1109,10 @@ int project_run(tg_ctx* c) {
                            Q + (size_t)r0 * nb, nb, mm, JQ + (size_t)r0 * nb, nb, mm, zero,
                            K + r0 + (size_t)r0 * nb, nb, mm, cnt, 1, 1, qtile));
         }
-        hipLaunchKernelGGL(add_j_kernel, dim3(nblk((long long)(nb / 2) * cnt)), dim3(NT), 0, s2, nb,
-                           cnt, b0, A, P.d_sg, os, K);
+        // one-wave workgroups: they fit beside the inverse's workgroups, so the
+        // Pfaffians can start on the CUs the inverse leaves free
+        hipLaunchKernelGGL(add_j_kernel, dim3((unsigned)(((long long)(nb / 2) * cnt + 63) / 64)),
+                           dim3(64), 0, s2, nb, cnt, b0, A, P.d_sg, os, K);
         TG_HIP(hipGetLastError());
         {
           ScopedTiming tk(c, "kernel:pfaffian", s2);
