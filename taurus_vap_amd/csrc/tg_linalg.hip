@@ -437,12 +437,12 @@ constexpr int PF_THREADS = 512;
 constexpr int PF_PS = 8;
 constexpr int PF_WIN = 256;   // columns whose pending vectors are staged in LDS at once
 
-// TH threads (n <= TH), WIN columns per staged window.  For TH = 384 the launch
-// bound of twice the workgroup caps the kernel at 168 VGPRs (three waves per
-// SIMD), so that the GEMM tiles of the other stream can share its CUs while it
-// waits on memory (at n = 336 the kernel runs longer but the step is 0.8 ms shorter).
+// TH threads (n <= TH), WIN columns per staged window.  (A cap at 168 VGPRs for
+// TH = 384, so that GEMM tiles could share its CUs, paid while the Pfaffians ran
+// beside the particle-hole contraction; since they start beside the inverse the
+// uncapped kernel makes the n = 336 step 0.5 ms shorter.)
 template <int TH, int WIN>
-__global__ void __launch_bounds__(TH == 384 ? 2 * TH : TH)
+__global__ void __launch_bounds__(TH)
 pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long sA,
                         cplx* __restrict__ work, double* __restrict__ logabs,
                         cplx* __restrict__ phase, int* __restrict__ zero) {
