@@ -1,18 +1,22 @@
 #!/usr/bin/env python3
 """PNVAP gradient iterations/s on one MI355X.
 
-Workload (default): e_max = 6 harmonic-oscillator basis (336 single-particle
-states), 15 x 15 Fomenko gauge angles, 10 protons / 12 neutrons, seeded
-synthetic JT-coupled interaction V ~ N(0, 2 MeV) exp(-(Na+Nb+Nc+Nd)/4)
-(12.9M stored m-scheme elements), SPEs hbar*omega (N + 3/2), seed state a real
-Thouless rotation (Z ~ N(0, 0.01)) of the lowest-filled Slater determinant,
-brought to <Z> = 10, <N> = 12.
-TBMEs stored in FP32, all arithmetic FP64.
+Workload: e_max = 6 harmonic-oscillator basis (336 single-particle states),
+15 x 15 Fomenko gauge angles, 10 protons / 12 neutrons, constraints <Z>, <N>,
+Q20 = 0.2 and Jx = 0; seeded synthetic JT-coupled interaction
+V ~ N(0, 2 MeV) exp(-(Na+Nb+Nc+Nd)/4) (12.9M stored m-scheme elements, the list
+the reference builds from the same .sho file, bit for bit), SPEs
+hbar*omega (N + 3/2).  The state and the constraint operators are those of the
+reference's own first iteration on this deck (tests/golden/emax6.npz, written by
+the reference after its adjust_constraints).  TBMEs stored in FP32, all
+arithmetic FP64.
 
-One step = one projected-gradient iteration of the hot path with the state
-resident in HBM: the gauge-angle loop of project_wavefunction over all 225
-angles (Thouless matrices, transition densities, TBME contraction, overlaps,
-energies, H20), sum_gauge, calculate_projected_H20 and the gradient step.
+One step = one full gradient iteration of the reference's loop
+(taurus_vap.f90:77-113) with the state resident in HBM: tg_iterate
+(update_constraints_qpbasis, project_wavefunction over all 225 angles,
+calculate_lagrange_multipliers, calculate_gradient, adjust_constraints with its
+Cholesky evolution) followed by tg_canonical_basis (construct_canonical_basis of
+the new state, which project_wavefunction does first in the reference).
 
 Prints one JSON line.  --gpus N > 1 is not supported (one GPU per run).
 """
@@ -35,8 +39,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "PNVAP gradient iters/s (e_max=6, 15x15 angles) vs plain FP64 GPU, |dE|<1 keV"
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
-# timing keys of kernels launched on the second (overlap) stream
-SECOND_STREAM_KERNELS = ("kernel:pfaffian", "kernel:zgemm_q")
 PEAK_HBM_GBS = 8000.0
 # Gauge integral over [0, 2 pi): the seed states carry proton-neutron mixing
 # (their Thouless matrix has proton-neutron entries), so check_symmetries keeps
@@ -102,63 +104,131 @@ def contraction_sizes(h):
     return dict(wph=float(sum(v * v for v in ph.values())), wpp=float(sum(v * v for v in pp.values())))
 
 
-def run_gpu(args, ham, plain, fp32, steps, warmup, timing=True, graph=False, conj_pairs=False):
-    from taurus_vap_amd import host
-    ctx, h, w, U, V = host.workload_context(args.workload, seed=args.seed, tbme_fp32=fp32,
-                                            plain=plain, ham=ham, graph=graph,
-                                            conj_pairs=conj_pairs)
-    timing = timing and not graph     # per-kernel events run the eager launch sequence
+FIXTURE = os.path.join(ROOT, "tests", "golden", "emax6.npz")
+
+
+def load_state():
+    """The reference's first-iteration state of the e_max = 6 deck, its constraint
+    operators and its FP64 / FP32 projected energies (data written by the
+    reference; no reference code runs here)."""
+    return dict(np.load(FIXTURE))
+
+
+def payload_sha256(ham, bits):
+    """SHA-256 of the element payload (0-based abcd, H2, trperm) in the reference's
+    reduced-file layout, to compare with the reference-written file's."""
+    import hashlib
+    d = hashlib.sha256()
+    d.update(np.ascontiguousarray(np.asarray(ham["abcd"]) - 1, dtype="<i2").tobytes())
+    d.update(np.ascontiguousarray(ham["H2"].astype(np.float32) if bits == 32 else ham["H2"],
+                                  dtype="<f4" if bits == 32 else "<f8").tobytes())
+    d.update(np.ascontiguousarray(ham["trperm"], dtype="<i1").tobytes())
+    return d.hexdigest()
+
+
+def make_context(ham, fix, plain, fp32, graph=False, conj_pairs=False):
+    from taurus_vap_amd import gpu, host
+    b = ham["basis"]
+    H2 = ham["H2"].astype(np.float32) if fp32 else ham["H2"]
+    ctx = gpu.Context(b.dim, b.sp_2mj, b.sp_2mt, b.sp_l, b.sp_tr, ham["H0"], ham["H1"],
+                      ham["abcd"], H2, ham["trperm"], plain=plain, tbme_fp32=fp32, graph=graph,
+                      conj_pairs=conj_pairs)
+    ctx.set_gradient(float(fix["gradient_eta"]), float(fix["gradient_mu"]))
+    ctx.set_constraints(np.asarray(fix["constraint_HO"]).reshape(-1), fix["constraint_val"],
+                        int(fix["constraint_pair"]), 1e-6)
+    U, V = np.asarray(fix["U0"]), np.asarray(fix["V0"])
     cb = host.construct_canonical_basis(U, V)
-    n = h["basis"].dim
-    M = args.angles if args.angles else w.M
+    M = int(fix["Mphip"])
     ctx.set_state(U, V, cb["D"], cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"], M, M,
-                  FACPI, float(w.Z), float(w.N))
-    # gradient step with two constraint arrays (the shape of Q20 + Jx)
-    rng = np.random.default_rng(args.seed + 7)
-    c20 = rng.normal(size=(2, n * n)) * 0.01
-    ctx.set_gradient(args.eta, args.mu, c20, [0.05, -0.02])
+                  float(fix["facpi"]), float(fix["valence_Z"]), float(fix["valence_N"]),
+                  float(fix["factor_delta"]))
+    return ctx, cb
+
+
+def next_basis(ctx, fix):
+    """tg_canonical_basis of the evolved state; a degenerate kappa block wider
+    than the device form handles takes the host route (counted)."""
+    from taurus_vap_amd import gpu, host
+    try:
+        ctx.canonical_basis()
+        return 0
+    except gpu.TaurusGPUError as e:
+        if "(-7)" not in str(e):
+            raise
+    U, V = ctx.get_state()
+    cb = host.construct_canonical_basis(U, V)
+    M = int(fix["Mphip"])
+    ctx.set_state(U, V, cb["D"], cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"], M, M,
+                  float(fix["facpi"]), float(fix["valence_Z"]), float(fix["valence_N"]),
+                  float(fix["factor_delta"]))
+    return 1
+
+
+def iterate_gpu(args, ham, fix, plain, fp32, steps, warmup, timing=True, graph=False,
+                conj_pairs=False):
+    """W + K full iterations from the reference's state; returns the wall time of
+    the K timed ones, the projected energy of the first iteration (the fixture
+    state) and the per-kernel timings of the timed iterations."""
+    ctx, cb = make_context(ham, fix, plain, fp32, graph=graph, conj_pairs=conj_pairs)
+    timing = timing and not graph     # per-kernel events run the eager launch sequence
+    first = None
+    host_route = 0
     for _ in range(warmup):
-        ctx.project_resident()
+        r = ctx.iterate(0.0)
+        first = first or r
+        host_route += next_basis(ctx, fix)
     ctx.sync()
     if timing:
         ctx.enable_timing(True)
         ctx.reset_timing()
+    t_it = t_cb = 0.0
+    nadj, energies = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        r = ctx.iterate(0.0)
+        b_ = time.perf_counter()
+        host_route += next_basis(ctx, fix)
+        ctx.sync()
+        c_ = time.perf_counter()
+        first = first or r
+        t_it += b_ - a
+        t_cb += c_ - b_
+        nadj.append(r["adjust_iterations"])
+        energies.append(float((r["ener"] / r["over"]).real))
+    t1 = time.perf_counter()
+    kt = {}
+    if timing:
+        for key in ("total", "kernel:grouped_ph", "kernel:grouped_pp", "kernel:gj_inverse",
+                    "kernel:pfaffian", "kernel:zgemm", "kernel:zgemm_q", "contraction", "gemm",
+                    "gemm_h20", "overlap", "thouless_inverse"):
+            kt[key] = ctx.timing(key)
+        ctx.enable_timing(False)
+    ctx.close()
+    n = ham["basis"].dim
+    M = int(fix["Mphip"])
+    return dict(seconds=t1 - t0, steps=steps, iterate_s=t_it, canonical_s=t_cb, timing=kt,
+                E_first=float((first["ener"] / first["over"]).real), first=first, n=n,
+                A=((M * M + 1) // 2 if conj_pairs else M * M),
+                nb=n - cb["nocc"] - cb["nemp"], adjust_steps=nadj, energies=energies,
+                host_route_states=host_route)
+
+
+def projection_only(args, ham, fix, steps, warmup):
+    """The projection of the fixture state alone (tg_project_resident: all angles,
+    sum_gauge, projected H20, gradient step), FP32 TBMEs: the loop body without
+    the constraint machinery and the canonical basis."""
+    ctx, _ = make_context(ham, fix, plain=False, fp32=True)
+    for _ in range(warmup):
+        ctx.project_resident()
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
         ctx.project_resident()
     ctx.sync()
     t1 = time.perf_counter()
-    res = ctx.get_results()
-    Zi, gnorm = ctx.get_gradient()
-    kt = {}
-    if timing:
-        for key in ("total", "kernel:grouped_ph", "kernel:grouped_pp", "kernel:gj_inverse",
-                    "kernel:pfaffian", "kernel:zgemm", "kernel:zgemm_q", "contraction", "gemm",
-                    "gemm_h20",
-                    "overlap", "thouless_inverse"):
-            kt[key] = ctx.timing(key)
-        ctx.enable_timing(False)
-    boundary = None
-    if timing:
-        # the host-buffer boundary as the Fortran drop-in calls it: canonical basis
-        # on the host, then tg_project_wavefunction (upload, all angles, sum_gauge,
-        # projected H20, download of pnp_* and field_H20v); not part of `value`
-        tc0 = time.perf_counter()
-        cb2 = host.construct_canonical_basis(U, V)
-        tc1 = time.perf_counter()
-        for _ in range(2):
-            ctx.project_wavefunction(U, V, cb2["D"], cb2["nocc"], cb2["nemp"], cb2["ovac"],
-                                     cb2["voveru"], M, M, FACPI, float(w.Z), float(w.N))
-        tc2 = time.perf_counter()
-        boundary = dict(canonical_basis_ms=1e3 * (tc1 - tc0), host_call_ms=1e3 * (tc2 - tc1) / 2)
     ctx.close()
-    E = (res["ener"] / res["over"]).real
-    nbloc = n - cb["nocc"] - cb["nemp"]
-    return dict(seconds=t1 - t0, steps=steps, E=E, Z=(res["prot"] / res["over"]).real,
-                N=(res["neut"] / res["over"]).real, gnorm=gnorm, timing=kt, n=n,
-                A=((M * M + 1) // 2 if conj_pairs else M * M),   # angles per launch
-                nb=nbloc, nocc=cb["nocc"], nemp=cb["nemp"], UV=(U, V), boundary=boundary)
+    return dict(value=steps / (t1 - t0), unit="projections/s", ms_per_step=1e3 * (t1 - t0) / steps)
 
 
 def batched_solves(args):
@@ -230,59 +300,14 @@ def batched_solves(args):
             "max_abs_dE_fp32_vs_fp64_keV": float(np.max(np.abs(E32 - E64))) * 1e3}
 
 
-def device_iteration(args, ham):
-    """Full gradient iterations of the workload on the device: tg_iterate
-    (constraint_20, projection, Lagrange multipliers, gradient, adjust_constraints)
-    then tg_canonical_basis of each new state, which never leaves the device,
-    with <Z>, <N> constrained.  A state the device canonical basis refuses
-    (degenerate kappa subspace wider than one pair) takes the host route and is
-    counted.  Not part of `value`."""
-    from taurus_vap_amd import host, gpu
-    ctx, h, w, U, V = host.workload_context(args.workload, seed=args.seed, tbme_fp32=True, ham=ham)
-    M = args.angles if args.angles else w.M
-    Zop, Nop = host.number_operators(h["basis"])
-    ctx.set_gradient(args.eta, 0.0)
-    ctx.set_constraints(np.stack([Zop.T, Nop.T]).reshape(-1), [float(w.Z), float(w.N)], 0, 1e-6)
-    cb = host.construct_canonical_basis(U, V)
-    ctx.set_state(U, V, cb["D"], cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"], M, M,
-                  FACPI, float(w.Z), float(w.N))
-    it_ms, cb_ms, nadj, host_route = [], [], [], 0
-    for it in range(args.iterate_steps + 1):
-        t0 = time.perf_counter()
-        r = ctx.iterate(0.0)
-        t1 = time.perf_counter()
-        try:
-            ctx.canonical_basis()
-        except gpu.TaurusGPUError as e:
-            if "(-7)" not in str(e):
-                raise
-            U, V = ctx.get_state()
-            cb = host.construct_canonical_basis(U, V)
-            ctx.set_state(U, V, cb["D"], cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"], M, M,
-                          FACPI, float(w.Z), float(w.N))
-            host_route += 1
-        ctx.sync()
-        t2 = time.perf_counter()
-        if it > 0:                  # the first call allocates and tunes
-            it_ms.append(1e3 * (t1 - t0))
-            cb_ms.append(1e3 * (t2 - t1))
-            nadj.append(r["adjust_iterations"])
-    ctx.close()
-    per = (np.mean(it_ms) + np.mean(cb_ms)) / 1e3
-    return {"value": 1.0 / per, "unit": "iterations/s", "iterations": args.iterate_steps,
-            "iterate_ms": float(np.mean(it_ms)), "canonical_basis_ms": float(np.mean(cb_ms)),
-            "canonical_basis": "device", "host_route_states": host_route,
-            "adjust_steps": nadj, "constraints": "<Z>, <N>"}
-
-
-def cpu_baseline(args, ham, threads, UV):
-    """Reference TAURUS_vap built from its own sources (oracle/_ref, FP64 TBMEs,
-    OpenMP + threaded OpenBLAS) on the same deck, .red file and state: inside one
-    run of oracle/ref_harness.f90 ('time' mode) the reference's
-    project_wavefunction(0,...) is timed at 1x2 and 3x3 gauge angles (best of 3),
-    plus its gradient step; the iteration time at 15x15 angles is the linear fit
-    in the angle count.  The reference's own set-up (reading the 220 MB .red file,
-    ~1 min) is not timed."""
+def cpu_baseline(args, ham, fix, threads):
+    """The reference TAURUS_vap built from its own sources (oracle/_ref, FP64 TBMEs,
+    OpenMP + threaded OpenBLAS) on the same deck, reduced Hamiltonian file and
+    state: oracle/ref_harness.f90 ('timefull' mode) times one full iteration of
+    the reference's loop -- project_wavefunction over the 15 x 15 angles (with
+    its construct_canonical_basis), calculate_lagrange_multipliers,
+    calculate_gradient and adjust_constraints.  The reference's set-up (reading
+    the 220 MB reduced file) is not timed."""
     import resource
     exe = os.path.join(ROOT, "oracle", "_ref", "harness64.exe")
     if not os.path.exists(exe):
@@ -290,43 +315,38 @@ def cpu_baseline(args, ham, threads, UV):
         return None
     from taurus_vap_amd import inputs as I, hamiltonian as H
     w = I.WORKLOADS[args.workload]
-    A = (args.angles or w.M) ** 2
 
     def big_stack():
         soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
         resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
 
     with tempfile.TemporaryDirectory() as td:
-        I.write_workload(w, td, seed=args.seed, hamil_read=1, iters=1, eta=args.eta, mu=args.mu,
-                         grad_eps=1e-14, UV=UV)
+        I.write_workload(w, td, seed=args.seed, hamil_read=1, iters=1,
+                         eta=float(fix["gradient_eta"]), mu=float(fix["gradient_mu"]),
+                         grad_eps=1e-14, UV=(np.asarray(fix["U0"]), np.asarray(fix["V0"])))
         H.write_red(os.path.join(td, w.name + ".red"), "synthetic %s seed %d" % (w.name, args.seed),
                     1, ham["H0"], ham["H1"], ham["abcd"], ham["H2"], ham["trperm"], rh2_bytes=8)
         env = dict(os.environ, OMP_NUM_THREADS=str(threads), OPENBLAS_NUM_THREADS=str(threads),
                    OMP_STACKSIZE="256M")
         t0 = time.perf_counter()
         with open(os.path.join(td, "input.txt")) as fin, open(os.path.join(td, "out.txt"), "w") as fout:
-            r = subprocess.run([exe, "time", "times.txt", "3"], stdin=fin, stdout=fout,
-                               stderr=subprocess.STDOUT, cwd=td, env=env, timeout=1500,
-                               preexec_fn=big_stack)
+            r = subprocess.run([exe, "timefull", "times.txt", str(args.cpu_iterations)], stdin=fin,
+                               stdout=fout, stderr=subprocess.STDOUT, cwd=td, env=env,
+                               timeout=1500, preexec_fn=big_stack)
         log("reference harness run: %.1f s (rc %d)" % (time.perf_counter() - t0, r.returncode))
         if r.returncode != 0:
             return None
         rows = np.loadtxt(os.path.join(td, "times.txt"), ndmin=2)
-    best = {}
-    for k, a, tp, tg, e in rows:
-        a = int(a)
-        best[a] = min(best.get(a, (1e30, 1e30))[0], tp), min(best.get(a, (1e30, 1e30))[1], tg)
-    (t2, g2), (t9, g9) = best[2], best[9]
-    per_angle = (t9 - t2) / 7.0
-    fixed = t2 - 2.0 * per_angle
-    if per_angle <= 0.0 or fixed < 0.0:
-        # timing noise larger than the spread: no fixed part, slope of the larger sample
-        per_angle, fixed = t9 / 9.0, 0.0
-    t_iter = fixed + A * per_angle + min(g2, g9)
+    per = rows[:, 2] + rows[:, 3] + rows[:, 4]
+    t_iter = float(np.mean(per))
     return dict(value=1.0 / t_iter, unit="iterations/s", cores=threads, kind="reference",
-                sample="reference project_wavefunction timed at 1x2 (%.2f s) and 3x3 (%.2f s) gauge "
-                       "angles + gradient step (%.3f s), best of 3, linear fit %.3f s + %.3f s/angle "
-                       "extrapolated to %d angles" % (t2, t9, min(g2, g9), fixed, per_angle, A))
+                sample="%d full iteration(s) of the reference's loop on the same deck, state and "
+                       "Hamiltonian, timed inside the reference build: project_wavefunction over "
+                       "%d angles %.2f s, multipliers + gradient %.3f s, adjust_constraints %.3f s "
+                       "(mean per iteration); set-up not timed"
+                       % (len(rows), int(rows[0, 1]), float(np.mean(rows[:, 2])),
+                          float(np.mean(rows[:, 3])), float(np.mean(rows[:, 4]))),
+                E_first=float(rows[0, 5]))
 
 
 def copy_probe(mib: int):
@@ -370,26 +390,26 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="emax6")
-    ap.add_argument("--angles", type=int, default=0, help="gauge angles per species (default: workload)")
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--eta", type=float, default=0.005)
-    ap.add_argument("--mu", type=float, default=0.3)
-    ap.add_argument("--mode", choices=["batched", "plain"], default="batched")
     ap.add_argument("--tbme", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--plain-steps", type=int, default=1,
-                    help="timed steps of the plain FP64 per-angle path (0: skip)")
+                    help="timed iterations of the plain FP64 per-angle path (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-iterations", type=int, default=1,
+                    help="full reference iterations timed for cpu_baseline")
     ap.add_argument("--conj-pairs", action="store_true",
                     help="compute one gauge angle of each (phi, -phi) pair (exact for real U, V)")
     ap.add_argument("--pair-steps", type=int, default=5,
-                    help="timed steps of the conjugate-pair variant reported beside value (0: skip)")
+                    help="timed iterations of the conjugate-pair variant reported beside value (0: skip)")
     ap.add_argument("--graph", action="store_true",
                     help="replay the projection as a captured hipGraph (no per-kernel timing)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--batch-steps", type=int, default=5,
                     help="timed steps of the 64 batched independent solves (0: skip)")
-    ap.add_argument("--iterate-steps", type=int, default=3,
-                    help="timed full device iterations with the constraint machinery (0: skip)")
+    ap.add_argument("--proj-steps", type=int, default=5,
+                    help="timed projections alone, without the constraint machinery (0: skip)")
+    ap.add_argument("--eta", type=float, default=0.005, help="step of the batched solves")
+    ap.add_argument("--mu", type=float, default=0.3, help="momentum of the batched solves")
     ap.add_argument("--dgemm-probe", type=int, default=0, metavar="N",
                     help="time the vendor FP64 GEMM on N x N matrices and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -409,30 +429,37 @@ def main():
 
     from taurus_vap_amd import hamiltonian as H, inputs as I
     w = I.WORKLOADS[args.workload]
+    fix = load_state()
     t = time.perf_counter()
     ham = H.workload_hamiltonian(w, args.seed)
-    log("hamiltonian %s: n=%d, %d stored TBMEs (%.1f s)" % (args.workload, ham["basis"].dim,
-                                                           len(ham["trperm"]), time.perf_counter() - t))
+    same_list = all(payload_sha256(ham, b) == str(fix["red_sha256_%d" % b]) for b in (64, 32))
+    log("hamiltonian %s: n=%d, %d stored TBMEs (%.1f s), list %s the reference's"
+        % (args.workload, ham["basis"].dim, len(ham["trperm"]), time.perf_counter() - t,
+           "equal to" if same_list else "DIFFERENT FROM"))
     sizes = contraction_sizes(ham)
+    fp32 = args.tbme == "fp32"
 
-    main_run = run_gpu(args, ham, plain=(args.mode == "plain"), fp32=(args.tbme == "fp32"),
-                       steps=args.steps, warmup=args.warmup, graph=args.graph,
-                       conj_pairs=args.conj_pairs)
+    main_run = iterate_gpu(args, ham, fix, plain=False, fp32=fp32, steps=args.steps,
+                           warmup=args.warmup, graph=args.graph, conj_pairs=args.conj_pairs)
     ips = main_run["steps"] / main_run["seconds"]
-    log("optimized: %.3f it/s, E = %.9f MeV, <Z> = %.6f, <N> = %.6f, |grad| = %.4e"
-        % (ips, main_run["E"], main_run["Z"], main_run["N"], main_run["gnorm"]))
+    E64_ref = float((fix["pnp_ener"] / fix["pnp_over"]).real)
+    E32_ref = float((fix["fp32_pnp_ener"] / fix["fp32_pnp_over"]).real)
+    dE_keV = abs(main_run["E_first"] - E64_ref) * 1e3
+    log("iterations: %.3f it/s (tg_iterate %.2f ms + canonical basis %.2f ms per step), first "
+        "E = %.9f MeV (reference FP64 %.9f, |dE| = %.3e keV), adjustment steps %s"
+        % (ips, 1e3 * main_run["iterate_s"] / main_run["steps"],
+           1e3 * main_run["canonical_s"] / main_run["steps"], main_run["E_first"], E64_ref, dE_keV,
+           main_run["adjust_steps"]))
     for k, (ms, c) in main_run["timing"].items():
         if c:
             log("  %-22s %9.3f ms/step  (%d intervals)" % (k, ms / main_run["steps"], c))
 
-    # dominant timed kernel and its roofline
+    # per-kernel rooflines (in-run HIP events on the stream each kernel ran on);
+    # the dominant kernel is the one with the largest summed time per step, on
+    # either stream
     n, A, nb = main_run["n"], main_run["A"], main_run["nb"]
-    M = args.angles if args.angles else w.M
+    M = int(fix["Mphip"])
     fl = algorithmic_flops(sizes, n, A, nb)
-    # every timed kernel (in-run HIP events on the stream each launch ran on);
-    # the dominant one is the kernel with the largest summed time per step on the
-    # context stream: the second stream's kernels (overlap chain) share the CUs
-    # with the context stream's, so their event times are not their own
     kern = {k: v for k, v in main_run["timing"].items() if k.startswith("kernel:") and v[1] > 0}
     roofline = None
     by_kernel = {}
@@ -447,36 +474,45 @@ def main():
                            "avg_launch_ms": avg_s * 1e3, "flop_per_launch": flop_launch,
                            "achieved": ach, "frac": ach / PEAK_FP64_TFLOPS,
                            "traffic": measured_traffic(args.workload, A, name)}
+    step_s = main_run["seconds"] / main_run["steps"]
+    proj_flop = sum(fl.values())
     if kern:
-        ctx_kern = [k for k in kern if k not in SECOND_STREAM_KERNELS] or list(kern)
-        dom = max(ctx_kern, key=lambda k: kern[k][0])
+        dom = max(kern, key=lambda k: kern[k][0])
         d = by_kernel[dom.split(":", 1)[1]]
         roofline = {"bound": "mfma", "kernel": dom.split(":", 1)[1], "achieved": d["achieved"],
                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": d["frac"],
                     "traffic": d["traffic"], "flop_per_launch": d["flop_per_launch"],
-                    "avg_launch_ms": d["avg_launch_ms"]}
+                    "avg_launch_ms": d["avg_launch_ms"],
+                    "whole_step": {"flop_per_step": proj_flop, "ms_per_step": 1e3 * step_s,
+                                   "achieved": proj_flop / step_s / 1e12,
+                                   "frac": proj_flop / step_s / 1e12 / PEAK_FP64_TFLOPS,
+                                   "counted": "the projection's algorithmic flops only "
+                                              "(constraint machinery and canonical basis "
+                                              "not counted)"}}
 
-    # plain FP64 per-angle drop-in (denominator of the metric) and |dE| vs FP64
+    # plain FP64 per-angle drop-in (the metric's denominator), same iteration
     plain = None
     if args.plain_steps > 0:
-        pr = run_gpu(args, ham, plain=True, fp32=False, steps=args.plain_steps, warmup=0, timing=False)
-        plain = dict(value=pr["steps"] / pr["seconds"], E=pr["E"])
-        log("plain FP64: %.4f it/s, E = %.9f MeV" % (plain["value"], pr["E"]))
-    else:
-        pr = run_gpu(args, ham, plain=False, fp32=False, steps=1, warmup=0, timing=False)
-        plain = dict(value=None, E=pr["E"])
-    dE_keV = abs(main_run["E"] - plain["E"]) * 1e3
+        pr = iterate_gpu(args, ham, fix, plain=True, fp32=False, steps=args.plain_steps, warmup=0,
+                         timing=False)
+        plain = dict(value=pr["steps"] / pr["seconds"], unit="iterations/s",
+                     E_first=pr["E_first"],
+                     dE_vs_oracle64_keV=abs(pr["E_first"] - E64_ref) * 1e3)
+        log("plain FP64: %.4f it/s, first E = %.9f MeV" % (plain["value"], pr["E_first"]))
 
-    # the same iteration computing one angle of each (phi, -phi) pair (exact for
-    # real U, V: the other angle's quantities are complex conjugates)
     pairs = None
     if not args.conj_pairs and args.pair_steps > 0:
-        cp = run_gpu(args, ham, plain=(args.mode == "plain"), fp32=(args.tbme == "fp32"),
-                     steps=args.pair_steps, warmup=args.warmup, timing=False, conj_pairs=True)
+        cp = iterate_gpu(args, ham, fix, plain=False, fp32=fp32, steps=args.pair_steps,
+                         warmup=args.warmup, timing=False, conj_pairs=True)
         pairs = dict(value=cp["steps"] / cp["seconds"], unit="iterations/s",
                      ms_per_step=1e3 * cp["seconds"] / cp["steps"], angles_per_launch=cp["A"],
-                     dE_vs_all_angles_keV=abs(cp["E"] - main_run["E"]) * 1e3)
-        log("conjugate pairs: %.3f it/s, E = %.9f MeV" % (pairs["value"], cp["E"]))
+                     dE_vs_all_angles_keV=abs(cp["E_first"] - main_run["E_first"]) * 1e3)
+        log("conjugate pairs: %.3f it/s" % pairs["value"])
+
+    proj = None
+    if args.proj_steps > 0:
+        proj = projection_only(args, ham, fix, args.proj_steps, args.warmup)
+        log("projection alone: %.3f projections/s" % proj["value"])
 
     batched = None
     if args.batch_steps > 0:
@@ -487,21 +523,11 @@ def main():
                                  batched["max_abs_dE_fp32_vs_fp64_keV"]))
         except Exception as e:      # a failed side measurement must not hide the main one
             log("batched solves failed: %r" % (e,))
-    devit = None
-    if args.iterate_steps > 0:
-        try:
-            devit = device_iteration(args, ham)
-            log("device iteration: %.3f it/s (tg_iterate %.2f ms, device canonical basis %.2f ms, "
-                "%d states on the host route)" % (devit["value"], devit["iterate_ms"],
-                                                  devit["canonical_basis_ms"],
-                                                  devit["host_route_states"]))
-        except Exception as e:
-            log("device iteration failed: %r" % (e,))
 
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(args, ham, args.cpu_threads, main_run["UV"])
+            cpu = cpu_baseline(args, ham, fix, args.cpu_threads)
         except Exception as e:      # a failed baseline must not hide the measurement
             log("cpu baseline failed: %r" % (e,))
 
@@ -512,34 +538,42 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * main_run["seconds"] / main_run["steps"],
+        "ms_per_step": 1e3 * step_s,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "%s: HO basis of %d sp states, %dx%d gauge angles, %dp/%dn, seeded "
-                        "synthetic JT interaction (%d stored m-scheme TBMEs), TBMEs %s / FP64 "
-                        "arithmetic, state resident in HBM" % (
+            "workload": "%s: HO basis of %d sp states, %dx%d gauge angles, %dp/%dn, constraints "
+                        "<Z>, <N>, Q20 = 0.2, Jx = 0, seeded synthetic JT interaction (%d stored "
+                        "m-scheme TBMEs, TBMEs %s / FP64 arithmetic), state = the reference's "
+                        "first iteration; step = one full gradient iteration (tg_iterate: "
+                        "projection, multipliers, gradient, adjust_constraints; then the device "
+                        "canonical basis), state resident in HBM" % (
                             args.workload, n, M, M, w.Z, w.N, len(ham["trperm"]),
                             args.tbme.upper()),
-            "mode": args.mode + ("+graph" if args.graph else "") +
+            "mode": "batched" + ("+graph" if args.graph else "") +
                     ("+conj_pairs" if args.conj_pairs else ""),
             "parallelism": "single GPU, gauge angles batched",
         },
         "roofline": roofline,
         "roofline_by_kernel": by_kernel,
         "cpu_baseline": cpu,
-        "plain_fp64_gpu": None if plain["value"] is None else {"value": plain["value"],
-                                                               "unit": "iterations/s"},
-        "speedup_vs_plain_fp64_gpu": None if plain["value"] is None else ips / plain["value"],
-        "host_boundary": main_run["boundary"],
-        "dE_vs_fp64_keV": dE_keV,
-        "E_proj_MeV": main_run["E"],
+        "plain_fp64_gpu": plain,
+        "speedup_vs_plain_fp64_gpu": None if plain is None else ips / plain["value"],
+        "dE_vs_oracle64_keV": dE_keV,
+        "E_first_MeV": main_run["E_first"],
+        "E_oracle64_MeV": E64_ref,
+        "E_oracle32_MeV": E32_ref,
+        "hamiltonian_equals_reference": same_list,
+        "iteration_split_ms": {"tg_iterate": 1e3 * main_run["iterate_s"] / main_run["steps"],
+                               "canonical_basis": 1e3 * main_run["canonical_s"] / main_run["steps"],
+                               "adjust_steps": main_run["adjust_steps"],
+                               "host_route_states": main_run["host_route_states"]},
+        "projection_only": proj,
         "conj_pairs": pairs,
         "batched_solves": batched,
-        "device_iteration": devit,
     }
     print(json.dumps(out), flush=True)
 

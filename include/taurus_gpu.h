@@ -103,6 +103,17 @@ int tg_calculate_fields_diag(tg_ctx* ctx, const double* rhoLR, const double* kap
                              double* deltaRL);
 
 /*
+ * tg_inverse -- the ZGETRF + ZGETRI pair of calculate_thouless
+ * (src/module_projection.f90:702-705) and calculate_overlap (:860-883) for
+ * `count` complex matrices of order n at A + k*lda*n, replaced in place by their
+ * inverses (Gauss-Jordan with the partial pivoting of ZGETRF: first maximum of
+ * |Re|+|Im|).  det: complex[count], the determinants (the product of the pivots
+ * with the interchange signs, as calculate_overlap forms detR :866-877);
+ * info: int[count], 0 or the first zero pivot column (1-based, ZGETRF's info).
+ */
+int tg_inverse(int n, int count, double* A, int lda, double* det, int* info);
+
+/*
  * tg_zskpfa -- ZSKPFA('U','P',n,A,lda,pfaff,...) of src/subroutines_pfaffian.f:12-256
  * (Parlett-Reid, upper triangle read, INFO > 0 from a zero pivot column gives
  * pfaff = 0 and info = 0).  `count` matrices laid out at A + k*lda*n (complex).
@@ -249,6 +260,52 @@ int tg_set_constraints(tg_ctx* ctx, int constraint_dim, int constraint_pair,
  */
 int tg_iterate(tg_ctx* ctx, double gradient_eps, double* pnp_scalars, double* gradient_norm,
                double* lagrange_lambda1, int* adjust_iterations, int* converged);
+
+/*
+ * tg_set_gradient_type -- gradient_type of the device iteration (tg_iterate):
+ * 0 = the eta, mu of tg_set_gradient; 1 / 2 = the heavy-ball recipes of
+ * calculate_gradient (src/module_gradient.f90:97-120) from the H11 / h spectra
+ * that diagonalize_hsp_and_H11(0, ndim) (:158-453) computes every iteration,
+ * here on the device (h and Delta of calculate_fields_diag on rhoRR, kappaRR;
+ * calculate_H11_real, module_fields.f90:327-353; the lambda_m C11 and lambda_m
+ * constraint_HO terms; the eigenvalues of h and H11).  The eta, mu of
+ * tg_set_gradient are not used by types 1, 2 (as in the reference, they are
+ * the fall-back of a failed diagonalisation, which cannot happen here).
+ */
+int tg_set_gradient_type(tg_ctx* ctx, int gradient_type);
+
+/*
+ * tg_get_hsp_h11_eigenvalues -- after a tg_iterate with gradient_type 1 or 2:
+ * the spectra of h and H11 of state `state` (eigen_hsp, eigen_H11 of
+ * module_gradient.f90:35-36, real[ndim], ascending) and the eta, mu of its step.
+ * Any pointer may be NULL.
+ */
+int tg_get_hsp_h11_eigenvalues(tg_ctx* ctx, int state, double* eigen_hsp, double* eigen_H11,
+                               double* gradient_eta, double* gradient_mu);
+
+/*
+ * tg_diagonalize_hsp_and_H11 -- diagonalize_hsp_and_H11(0, ndim),
+ * src/module_gradient.f90:158-453 (opt = 0: what calculate_gradient needs), for
+ * the resident state (one state; tg_project_wavefunction or tg_set_state) with
+ * the constraint operators of tg_set_constraints (constraint_dim must match)
+ * and the multipliers lagrange_lambda1 real[constraint_dim]:
+ *   eigen_hsp, eigen_H11   real[ndim], ascending (the DSYEV of :297, :416)
+ *   info_hsp, info_H11     0 (the device solver has no failure mode)
+ *   hsp_RR, field_H11      (may be NULL) real ndim x ndim: the matrices handed to
+ *                          the eigenvalue solver, field_hspRR and field_H11 of
+ *                          :181-213 (constraint terms removed)
+ */
+int tg_diagonalize_hsp_and_H11(tg_ctx* ctx, int constraint_dim, const double* lagrange_lambda1,
+                               double* eigen_hsp, double* eigen_H11, int* info_hsp, int* info_H11,
+                               double* hsp_RR, double* field_H11);
+
+/*
+ * tg_sym_eigenvalues -- eigenvalues (ascending) of `count` real symmetric
+ * matrices of order n <= 512 at A + k*lda*n (upper triangles read): the
+ * eigenvalues DSYEV('V','U') returns at src/module_gradient.f90:297 and :416.
+ * w: real[count*n].  A is not modified.
+ */
+int tg_sym_eigenvalues(int n, int count, const double* A, int lda, double* w);
 
 /*
  * tg_get_state -- bogo_U0, bogo_V0 of the device states (column-major

@@ -18,9 +18,19 @@
 !   over  <infile> <outfile> calculate_overlap on given canonical data         !
 !   time  <outfile> <K>     reads the deck like iter (no initial constraint    !
 !                           adjustment), then K times: project_wavefunction    !
-!                           (0,...) at 1x2 and at 2x2 gauge angles and the     !
+!                           (0,...) at 1x2 and at 3x3 gauge angles and the     !
 !                           gradient step (calculate_lagrange_multipliers +    !
 !                           calculate_gradient); writes the seconds as text.   !
+!   timefull <outfile> <K>  as time, but K full iterations at the deck's own   !
+!                           gauge angles: project_wavefunction(0,...), the     !
+!                           gradient step and adjust_constraints, each timed.  !
+!   proj  <dumpfile> <K>    as iter for at most K iterations, dumping only     !
+!                           what stays small at large bases: the state of      !
+!                           iteration 1, the canonical-basis scalars, per-     !
+!                           angle scalars, the gauge sums, field_H20v, the     !
+!                           multipliers, the gradient and (gradient_type > 0)  !
+!                           the h and H11 matrices of diagonalize_hsp_and_H11  !
+!                           with their LAPACK eigenvalues; then histories.     !
 !                                                                              !
 ! Dump format (stream, little endian): repeated records                        !
 !   char(32) name | int32 kind | int32 rank | int64 dims(rank) | data          !
@@ -123,8 +133,12 @@ select case (trim(mode))
     call run_over(trim(f1), trim(f2))
   case ('time')
     call run_time(trim(f1), trim(f2))
+  case ('timefull')
+    call run_timefull(trim(f1), trim(f2))
+  case ('proj')
+    call run_proj(trim(f1), trim(f2))
   case default
-    stop 'ref_harness: mode must be iter, pfaff or over'
+    stop 'ref_harness: mode must be iter, proj, pfaff, over, time or timefull'
 end select
 
 contains
@@ -207,7 +221,7 @@ do iter = 1, iter_max
     if ( constraint_dim > 0 ) then
       call dump_r1('constraint_20', constraint_20, int(constraint_dim,i64)*ndim*ndim)
     endif
-    call dump_angles(ndim)
+    call dump_angles(ndim, .true.)
   endif
 
   call project_wavefunction(0,iter_print,proj_Mphip,proj_Mphin,HOsp_dim)
@@ -267,8 +281,9 @@ end subroutine run_iter
 ! Same body as the angle loop of project_wavefunction (iopt=0, iprint=0),      !
 ! built from the module's public routines, dumping each stage.                 !
 !------------------------------------------------------------------------------!
-subroutine dump_angles(ndim)
+subroutine dump_angles(ndim, full)
 integer, intent(in) :: ndim
+logical, intent(in) :: full
 integer :: i, j, nocc0, nemp0, nangle
 real(r64) :: ovac0
 real(r64), allocatable :: voveru0(:)
@@ -296,9 +311,11 @@ call dump_i0('nocc0', nocc0)
 call dump_i0('nemp0', nemp0)
 call dump_r0('ovac0', ovac0)
 call dump_r1('voveru0', voveru0, int(ndim-nocc0,i64))
-call dump_c2('zD0', bogo_zD0, ndim, ndim)
-call dump_c2('zU0c', bogo_zU0c, ndim, ndim)
-call dump_c2('zV0c', bogo_zV0c, ndim, ndim)
+if ( full ) then
+  call dump_c2('zD0', bogo_zD0, ndim, ndim)
+  call dump_c2('zU0c', bogo_zU0c, ndim, ndim)
+  call dump_c2('zV0c', bogo_zV0c, ndim, ndim)
+endif
 
 call assign_angles(proj_Mphip,proj_Mphin)
 call dump_i0('nangles', proj_myangles)
@@ -342,6 +359,10 @@ do nangle = 1, proj_myangles
   call dump_c0('rot_neut', rot_neut)
   call dump_c0('rot_prot2', rot_prot2)
   call dump_c0('rot_neut2', rot_neut2)
+  if ( .not. full ) then
+    call sum_gauge(0,proj_weip(nangle),proj_wein(nangle))
+    cycle
+  endif
   call dump_c2('rot_A', rot_A, ndim, ndim)
   call dump_c2('Utilde', zU0tilde, ndim, ndim)
   call dump_c2('Vtilde', zV0tilde, ndim, ndim)
@@ -470,5 +491,234 @@ do k = 1, kmax
 enddo
 close(ud)
 end subroutine run_time
+
+!------------------------------------------------------------------------------!
+! Iterations of taurus_vap.f90:77-113 with dumps that stay small at large n    !
+!------------------------------------------------------------------------------!
+subroutine run_proj(dumpfile, kstr)
+character(*), intent(in) :: dumpfile, kstr
+integer :: iter_max, iter_write, iter_print, iter, ndim, niter_done, kmax, m, k, l, i, j, &
+           info
+real(r64) :: q11_aux, q00_aux
+real(r64), allocatable :: e_hist(:), gn_hist(:), z_hist(:), n_hist(:), hmat(:,:), &
+                          evals(:), work(:)
+complex(r64), allocatable :: gRR(:,:), hRR(:,:), dRR(:,:)
+
+read(kstr,*) kmax
+call read_input(iter_max,iter_write,iter_print)
+call set_nucleus
+call set_basis
+call set_hamiltonian
+call set_wavefunctions
+call set_fields
+call set_particle_number
+call set_pairs
+call set_angular_momentum
+call set_multipoles
+call set_radius
+call set_projection
+call generate_wavefunction(HOsp_dim)
+call check_symmetries(HOsp_dim)
+call set_constraints
+call set_gradient
+if ( (iter_max > 0) .or. (seed_type /= 1) ) then
+  call adjust_constraints(gradient_Zi,HOsp_dim)
+endif
+iter_max = min(iter_max, kmax)
+
+ndim = HOsp_dim
+open(ud, file=dumpfile, status='replace', access='stream', form='unformatted')
+call dump_i0('ndim', ndim)
+call dump_i1('sp_2mj', HOsp_2mj, ndim)
+call dump_i1('sp_2mt', HOsp_2mt, ndim)
+call dump_i1('sp_l', HOsp_l, ndim)
+call dump_i1('sp_2j', HOsp_2j, ndim)
+call dump_i1('sp_tr', HOsp_tr, ndim)
+call dump_r0('valence_Z', valence_Z)
+call dump_r0('valence_N', valence_N)
+call dump_i0('Mphip', proj_Mphip)
+call dump_i0('Mphin', proj_Mphin)
+call dump_r0('facpi', proj_facpi)
+call dump_i0('gradient_type', gradient_type)
+call dump_r0('gradient_eta', gradient_eta)
+call dump_r0('gradient_mu', gradient_mu)
+call dump_i0('constraint_dim', constraint_dim)
+call dump_i0('constraint_pair', constraint_pair)
+call dump_r0('factor_delta', factor_delta)
+call dump_c0('H0', hamil_H0)
+if ( constraint_dim > 0 ) then
+  call dump_r1('constraint_HO', constraint_HO, int(constraint_dim,i64)*ndim*ndim)
+  call dump_r1('constraint_val', constraint_val, int(constraint_dim,i64))
+endif
+
+allocate(e_hist(max(iter_max,1)), gn_hist(max(iter_max,1)), z_hist(max(iter_max,1)), &
+         n_hist(max(iter_max,1)))
+allocate(hmat(ndim,ndim), evals(ndim), work(3*ndim), gRR(ndim,ndim), hRR(ndim,ndim), &
+         dRR(ndim,ndim))
+e_hist = zero
+gn_hist = zero
+z_hist = zero
+n_hist = zero
+niter_done = 0
+
+do iter = 1, iter_max
+  field_H20v = zero
+  call calculate_densities_real(bogo_U0,bogo_V0,dens_rhoRR,dens_kappaRR, &
+                                HOsp_dim)
+  call update_constraints_qpbasis
+  if ( iter == 1 ) then
+    call dump_r2('U0', bogo_U0, ndim, ndim)
+    call dump_r2('V0', bogo_V0, ndim, ndim)
+    if ( constraint_dim > 0 ) then
+      call dump_r1('constraint_20', constraint_20, int(constraint_dim,i64)*ndim*ndim)
+    endif
+    call dump_angles(ndim, .false.)
+  endif
+
+  call project_wavefunction(0,iter_print,proj_Mphip,proj_Mphin,HOsp_dim)
+  call calculate_lagrange_multipliers(field_H20v,HOsp_dim)
+
+  if ( iter == 1 ) then
+    call dump_c0('pnp_over', pnp_over)
+    call dump_c0('pnp_ener', pnp_ener)
+    call dump_c0('pnp_prot', pnp_prot)
+    call dump_c0('pnp_neut', pnp_neut)
+    call dump_c0('pnp_prot2', pnp_prot2)
+    call dump_c0('pnp_neut2', pnp_neut2)
+    call dump_c0('pnp_pari', pnp_pari)
+    call dump_i0('iter_print', iter_print)
+    call dump_r1('field_H20v', field_H20v, int(ndim,i64)*ndim)
+    if ( constraint_dim > 0 ) then
+      call dump_r1('lagrange_lambda1', lagrange_lambda1, int(constraint_dim,i64))
+    endif
+  endif
+
+  if ( gradient_type > 0 ) then
+    if ( iter == 1 ) then
+      ! the matrices diagonalize_hsp_and_H11(0) hands to dsyev (module_gradient.f90:
+      ! 177-213), built from the same public routines in the same order, and their
+      ! eigenvalues from the same LAPACK call
+      if ( max(proj_Mphip,proj_Mphip) == 1 ) then
+        field_hspRR = real(field_hspLR)
+        field_deltaRR = real(field_deltaRL)
+      else
+        call calculate_fields_diag(zone*dens_rhoRR,zone*dens_kappaRR,gRR,hRR, &
+                                   dRR,ndim=ndim)
+        field_hspRR = real(hRR)
+        field_deltaRR = real(dRR)
+      endif
+      call calculate_H11_real(ndim)
+      do m = 1, constraint_dim-constraint_pair
+        k = (m-1)*(ndim**2) + 1
+        call set_operator_qpbasis('f11',bogo_U0,bogo_V0,constraint_HO(k), &
+                                  constraint_11(k),ndim)
+      enddo
+      l = 0
+      do j = 1, ndim
+        do i = 1, ndim
+          l = l + 1
+          q11_aux = zero
+          q00_aux = zero
+          do m = 1, constraint_dim
+            k = (m-1)*(ndim**2)
+            q11_aux = q11_aux + lagrange_lambda1(m) * constraint_11(k+l)
+            q00_aux = q00_aux + lagrange_lambda1(m) * constraint_HO(k+l)
+          enddo
+          field_H11(i,j) = field_H11(i,j) - q11_aux
+          field_hspRR(i,j) = field_hspRR(i,j) - q00_aux
+        enddo
+      enddo
+      call dump_r2('hsp_RR', field_hspRR, ndim, ndim)
+      call dump_r2('H11', field_H11, ndim, ndim)
+      hmat = field_hspRR
+      call dsyev('v','u',ndim,hmat,ndim,evals,work,3*ndim-1,info)
+      call dump_r1('eigen_hsp', evals, int(ndim,i64))
+      call dump_i0('info_hsp', info)
+      hmat = field_H11
+      call dsyev('v','u',ndim,hmat,ndim,evals,work,3*ndim-1,info)
+      call dump_r1('eigen_H11', evals, int(ndim,i64))
+      call dump_i0('info_H11', info)
+    endif
+    call diagonalize_hsp_and_H11(0,HOsp_dim)
+  endif
+  call calculate_gradient(HOsp_dim)
+
+  if ( iter == 1 ) then
+    call dump_r1('gradient_Zi', gradient_Zi, int(ndim,i64)*ndim)
+    call dump_r0('gradient_norm', gradient_norm)
+  endif
+
+  e_hist(iter) = real(pnp_ener/pnp_over)
+  z_hist(iter) = real(pnp_prot/pnp_over)
+  n_hist(iter) = real(pnp_neut/pnp_over)
+  gn_hist(iter) = gradient_norm
+  niter_done = iter
+
+  if ( gradient_norm <= gradient_eps ) exit
+  call adjust_constraints(gradient_Zi,HOsp_dim)
+  gradient_Zim1 = gradient_Zi
+enddo
+
+call dump_i0('niter', niter_done)
+call dump_r1('hist_ener', e_hist, int(max(niter_done,1),i64))
+call dump_r1('hist_prot', z_hist, int(max(niter_done,1),i64))
+call dump_r1('hist_neut', n_hist, int(max(niter_done,1),i64))
+call dump_r1('hist_gradnorm', gn_hist, int(max(niter_done,1),i64))
+close(ud)
+
+end subroutine run_proj
+
+!------------------------------------------------------------------------------!
+! Wall-clock timing of full iterations at the deck's gauge angles              !
+!------------------------------------------------------------------------------!
+subroutine run_timefull(outfile, kstr)
+character(*), intent(in) :: outfile, kstr
+integer :: iter_max, iter_write, iter_print, k, kmax
+integer(i64) :: c0, c1, c2, c3, rate
+real(r64) :: tproj, tgrad, tadj
+
+read(kstr,*) kmax
+call read_input(iter_max,iter_write,iter_print)
+call set_nucleus
+call set_basis
+call set_hamiltonian
+call set_wavefunctions
+call set_fields
+call set_particle_number
+call set_pairs
+call set_angular_momentum
+call set_multipoles
+call set_radius
+call set_projection
+call generate_wavefunction(HOsp_dim)
+call check_symmetries(HOsp_dim)
+call set_constraints
+call set_gradient
+
+open(ud, file=outfile, status='replace', form='formatted')
+do k = 1, kmax
+  ! one pass of the loop body of taurus_vap.f90:77-113
+  call system_clock(c0, rate)
+  field_H20v = zero
+  call calculate_densities_real(bogo_U0,bogo_V0,dens_rhoRR,dens_kappaRR,HOsp_dim)
+  call update_constraints_qpbasis
+  call project_wavefunction(0,0,proj_Mphip,proj_Mphin,HOsp_dim)
+  call system_clock(c1)
+  call calculate_lagrange_multipliers(field_H20v,HOsp_dim)
+  if ( gradient_type > 0 ) call diagonalize_hsp_and_H11(0,HOsp_dim)
+  call calculate_gradient(HOsp_dim)
+  call system_clock(c2)
+  call adjust_constraints(gradient_Zi,HOsp_dim)
+  gradient_Zim1 = gradient_Zi
+  call system_clock(c3)
+  tproj = real(c1-c0,r64) / real(rate,r64)
+  tgrad = real(c2-c1,r64) / real(rate,r64)
+  tadj = real(c3-c2,r64) / real(rate,r64)
+  write(ud,'(i4,i6,3es16.8,es20.10)') k, proj_Mphip*proj_Mphin, tproj, tgrad, tadj, &
+                                      real(pnp_ener/pnp_over)
+  flush(ud)
+enddo
+close(ud)
+end subroutine run_timefull
 
 end program ref_harness

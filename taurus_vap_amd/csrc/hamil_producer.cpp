@@ -9,8 +9,8 @@
 //   * uncouple_JT (:828-885): sum over J and T of N_ab^-1 N_cd^-1 CG CG CG CG V;
 //   * reconstruct_2body_timerev (:970-1018): the int8 permutation/phase code of
 //     the time-reversed element.
-// Clebsch-Gordan coefficients come from the Racah single-sum formula evaluated
-// in long double with exact factorials, tabulated once for all (j1, j2, J, m1, m2).
+// Clebsch-Gordan coefficients are the Racah formula on log-gamma values,
+// tabulated once for all (j1, j2, J, m1, m2).
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
@@ -21,34 +21,49 @@
 
 namespace {
 
-long double fact(int k) {
-  long double f = 1.0L;
-  for (int i = 2; i <= k; ++i) f *= (long double)i;
-  return f;
+// <j1 m1 j2 m2 | j3 m3>, all arguments twice their values: the Racah formula
+// (j1,m1,j2,m2|j3,m3) = sqrt(2 j3 + 1) D(j1 j2 j3) [factorials of j +- m]^1/2
+// sum_l (-1)^l / [factorials], with the factorials as log-gamma values and the
+// sum as a ratio recurrence of its terms from the first non-vanishing one.
+// The operation order is that of ClebschGordan (src/module_mathmethods.f90:
+// 281-342), so the stored FP64 elements are bit-identical to the reference's.
+double cg_lgamma(int j1, int j2, int j3, int m1, int m2, int m3) {
+  const int n1 = 1 + (j1 + j2 - j3) / 2, n2 = 1 + (j2 + j3 - j1) / 2, n3 = 1 + (j3 + j1 - j2) / 2;
+  const int n4 = 1 + (j1 - m1) / 2, n5 = 1 + (j2 + m2) / 2, n6 = 1 + (j1 + m1) / 2;
+  const int n7 = 1 + (j2 - m2) / 2, n8 = 1 + (j3 + m3) / 2, n9 = 1 + (j3 - m3) / 2;
+  const int n10 = n1 + n2 + n3 - 1;
+  int nmin = n1;
+  for (int v : {n2, n3, n4, n5, n6, n7, n8, n9}) nmin = v < nmin ? v : nmin;
+  if (nmin < 1 || m1 + m2 != m3) return 0.0;
+  const double p = lgamma((double)n1) + lgamma((double)n2) + lgamma((double)n3) +
+                   lgamma((double)n4) + lgamma((double)n5) + lgamma((double)n6) +
+                   lgamma((double)n7) + lgamma((double)n8) + lgamma((double)n9) -
+                   lgamma((double)n10);
+  int k1 = n1, k2 = n4, k3 = n5, k4 = n4 - n3, k5 = n5 - n2;
+  int l1 = 0;
+  for (int v : {k4, k5}) l1 = v > l1 ? v : l1;
+  int l2 = k1;
+  for (int v : {k2, k3}) l2 = v < l2 ? v : l2;
+  double h = 1.0, hl = 1.0;
+  for (int l = l1 + 1; l <= l2; ++l) {
+    const double hlm1 = hl;
+    hl = hlm1 * (double)(l - k1) * (double)(l - k2) * (double)(l - k3) /
+         (double)((l - k4) * (l - k5) * l);
+    h = h + hl;
+  }
+  k1 -= l1;
+  k2 -= l1;
+  k3 -= l1;
+  k4 = l1 + 1 - k4;
+  k5 = l1 + 1 - k5;
+  const int k6 = l1 + 1;
+  const double q = lgamma((double)k1) + lgamma((double)k2) + lgamma((double)k3) +
+                   lgamma((double)k4) + lgamma((double)k5) + lgamma((double)k6);
+  const double sgn = (l1 & 1) ? -1.0 : 1.0;
+  return sqrt((double)j3 + 1.0) * sgn * exp(0.5 * p - q) * h;
 }
 
-// <j1 m1 j2 m2 | J M>, all arguments twice their values
-double cg_racah(int j1, int j2, int J, int m1, int m2) {
-  const int M = m1 + m2;
-  if (abs(m1) > j1 || abs(m2) > j2 || abs(M) > J) return 0.0;
-  if (((j1 + m1) & 1) || ((j2 + m2) & 1) || ((J + M) & 1)) return 0.0;
-  if (J < abs(j1 - j2) || J > j1 + j2 || ((j1 + j2 + J) & 1)) return 0.0;
-  const int a = (j1 + j2 - J) / 2, b = (j1 - j2 + J) / 2, c = (-j1 + j2 + J) / 2;
-  const int d = (j1 + j2 + J) / 2 + 1;
-  long double pre = (long double)(J + 1) * fact(a) * fact(b) * fact(c) / fact(d);
-  pre *= fact((j1 + m1) / 2) * fact((j1 - m1) / 2) * fact((j2 + m2) / 2) * fact((j2 - m2) / 2) *
-         fact((J + M) / 2) * fact((J - M) / 2);
-  long double sum = 0.0L;
-  for (int k = 0; k <= 200; ++k) {
-    const int e1 = a - k, e2 = (j1 - m1) / 2 - k, e3 = (j2 + m2) / 2 - k;
-    const int e4 = (J - j2 + m1) / 2 + k, e5 = (J - j1 - m2) / 2 + k;
-    if (e1 < 0 || e2 < 0 || e3 < 0) break;
-    if (e4 < 0 || e5 < 0) continue;
-    long double den = fact(k) * fact(e1) * fact(e2) * fact(e3) * fact(e4) * fact(e5);
-    sum += ((k & 1) ? -1.0L : 1.0L) / den;
-  }
-  return (double)(sqrtl(pre) * sum);
-}
+double cg_racah(int j1, int j2, int J, int m1, int m2) { return cg_lgamma(j1, j2, J, m1, m2, m1 + m2); }
 
 struct CGTable {
   int jmax2;   // largest 2j
@@ -78,8 +93,7 @@ struct CGTable {
 extern "C" {
 
 double th_clebsch_gordan(int j1, int j2, int j3, int m1, int m2, int m3) {
-  if (m1 + m2 != m3) return 0.0;
-  return cg_racah(j1, j2, j3, m1, m2);
+  return cg_lgamma(j1, j2, j3, m1, m2, m3);
 }
 
 int64_t th_mscheme(int n, const int32_t* sp_2j, const int32_t* sp_2mj, const int32_t* sp_2mt,

@@ -150,10 +150,11 @@ int gemm_tuned_variant(int kind, TgOp opa, TgOp opb, int M, int N, int K, int ba
 int batched_inverse(hipStream_t s, int n, cplx* A, int lda, long long sA, int batch,
                     cplx* det, int* info, int* perm);
 // Pfaffian of skew matrices (upper triangle read) by the Parlett-Reid LTL^T
-// reduction with the pivoting of ZSKTF2 (UPLO='U', MODE='P').  The matrix is
-// copied into work (n*n per matrix).  Output: logabs[b] = log|pf|, phase[b] =
+// reduction with the pivoting of ZSKTF2 (UPLO='U', MODE='P').  A is destroyed:
+// it and work (n*n per matrix) hold the leading block in turn.  Output:
+// logabs[b] = log|pf|, phase[b] =
 // pf/|pf| (complex), zero[b] = 1 when a pivot column vanished (pf = 0).
-int batched_pfaffian(hipStream_t s, int n, const cplx* A, int lda, long long sA, int batch,
+int batched_pfaffian(hipStream_t s, int n, cplx* A, int lda, long long sA, int batch,
                      cplx* work, double* logabs, cplx* phase, int* zero);
 
 // ---------------------------------------------------------------------------
@@ -248,9 +249,15 @@ struct tg_ctx {
   hipGraphExec_t graph_exec = nullptr;
   int iprint = 0;                                 // parity overlaps (rot_pari) wanted
   DevBuf d_psign;                                 // (-1)^l per single-particle state
-  // on-device gradient step (calculate_gradient, gradient_type 0)
+  // on-device gradient step (calculate_gradient)
   int grad_on = 0;
   double grad_eta = 0, grad_mu = 0;
+  // gradient_type of the device iteration: 0 = eta, mu as given; 1 / 2 = the
+  // heavy-ball recipes from the H11 / h spectra of diagonalize_hsp_and_H11
+  int grad_type = 0;
+  std::vector<double> etamu;                      // eta, mu per state of the last step
+  std::vector<double> eig_last;                   // S x (h, H11) spectra of the last step
+  DevBuf d_etamu;
   int grad_ncons = 0;
   std::vector<double> grad_lambda;
   DevBuf d_c20, d_Z;                              // constraint_20; Zi, Zim1, lambda, norm
@@ -264,6 +271,7 @@ struct tg_ctx {
   DevBuf d_cHO;                                   // constraint operators, internal order
   DevBuf d_it;                                    // iteration work (n*n real matrices)
   DevBuf d_obs;                                   // observables of the final projection
+  DevBuf d_a19;                                   // h, H11 and their eigenvalues
   std::vector<double> lambda_prev;                // multipliers of the previous iteration (ns x dim)
   // canonical basis on the device (tg_canon.hip): work buffers, rocBLAS handle
   DevBuf d_canon;
@@ -302,6 +310,18 @@ int tg_fetch_results(tg_ctx* c, int state, double* scal, double* pA, double* pH,
                      double* H20v, double* imag);
 // ensures the Z buffers for c->ns states (zeroed when they are (re)sized)
 int tg_ensure_z(tg_ctx* c);
+// diagonalize_hsp_and_H11(0) of every resident state (tg_a19.hip): spectra of
+// h and H11 (with the constraint terms of the device multipliers lam,
+// S x TG_MAX_CONS) to eig (S x 2 x n); hmat / h11 receive the device matrices.
+int tg_hsp_h11_device(tg_ctx* c, const double* lam, std::vector<double>& eig, double** hmat,
+                      double** h11);
+// eta, mu of calculate_gradient for gradient_type (module_gradient.f90:97-120)
+void gradient_recipe(int type, int n, const double* eig_hsp, const double* eig_h11, double eta0,
+                     double mu0, double* eta, double* mu);
+// eigenvalues (ascending) of count symmetric matrices A + b sA (upper triangle
+// read) to w + b ws; work: count n^2 doubles for orders above 112
+int batched_sym_eigvals(hipStream_t s, int n, const double* A, int lda, long long sA, int count,
+                        double* work, double* w, long long ws);
 // The resident U, V, D and c->st of c->ns states are set: the tail of
 // tg_set_states (Z buffers, gauge grid, state marked for the set-up upload).
 int tg_states_ready(tg_ctx* c);

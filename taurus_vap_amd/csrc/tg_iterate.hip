@@ -1,9 +1,10 @@
 // One gradient iteration of the reference driver on the device
-// (taurus_vap.f90:77-113 with gradient_type 0), for every resident state of a
-// context at once:
+// (taurus_vap.f90:77-113), for every resident state of a context at once:
 //   update_constraints_qpbasis          module_constraints.f90:381-397
 //   project_wavefunction(0, ...)        (tg_project.hip)
 //   calculate_lagrange_multipliers      module_constraints.f90:413-484
+//   diagonalize_hsp_and_H11(0)          module_gradient.f90:158-453 (gradient_type
+//                                       1, 2; tg_a19.hip)
 //   calculate_gradient                  module_gradient.f90:79-143
 //   adjust_constraints                  module_constraints.f90:270-373
 //     cholesky_factorization            module_constraints.f90:497-543
@@ -430,7 +431,17 @@ int iterate_post(tg_ctx* c, double gradient_eps, IterOut& out) {
     }
     TG_HIP(hipMemcpyAsync(lam, lamh.data(), sizeof(double) * lamh.size(), hipMemcpyHostToDevice, s));
   }
-  // calculate_gradient (gradient_type 0): Z = -eta (H20v - sum_j lambda_j C20_j) + mu Zim1
+  // gradient_type 1, 2: diagonalize_hsp_and_H11(0) with this iteration's
+  // multipliers, then eta, mu of the recipe per state (module_gradient.f90:97-120)
+  if (c->grad_type > 0) {
+    TG_TRY(tg_hsp_h11_device(c, lam, c->eig_last, nullptr, nullptr));
+    c->etamu.assign(2 * (size_t)S, 0.0);
+    for (int st = 0; st < S; ++st)
+      gradient_recipe(c->grad_type, n, c->eig_last.data() + 2 * (size_t)st * n,
+                      c->eig_last.data() + (2 * (size_t)st + 1) * n, c->grad_eta, c->grad_mu,
+                      &c->etamu[2 * st], &c->etamu[2 * st + 1]);
+  }
+  // calculate_gradient: Z = -eta (H20v - sum_j lambda_j C20_j) + mu Zim1
   TG_TRY(tg_gradient_step(c, c20, lam));
   std::vector<double> ph((size_t)S * ps);
   TG_HIP(hipMemcpyAsync(ph.data(), part, sizeof(double) * ph.size(), hipMemcpyDeviceToHost, s));
@@ -443,7 +454,7 @@ int iterate_post(tg_ctx* c, double gradient_eps, IterOut& out) {
   for (int st = 0; st < S; ++st) {
     double sum = 0;
     for (long long k = 0; k < nb; ++k) sum += ph[(size_t)st * ps + k];
-    out.gnorm[st] = sqrt(sum) / c->grad_eta;
+    out.gnorm[st] = sqrt(sum) / (c->grad_type > 0 ? c->etamu[2 * st] : c->grad_eta);
     out.conv[st] = out.gnorm[st] <= gradient_eps;
     if (!out.conv[st]) evolve.push_back(st);
   }
@@ -752,6 +763,103 @@ int tg_iterate(tg_ctx* c, double gradient_eps, double* pnp_scalars, double* grad
   TG_TRY(iterate_post(c, gradient_eps, o));
   copy_out(c, o, pnp_scalars, gradient_norm, lagrange_lambda1, adjust_iterations, converged);
   return TG_OK;
+}
+
+int tg_set_gradient_type(tg_ctx* c, int gradient_type) {
+  if (!c || gradient_type < 0 || gradient_type > 2) {
+    tg_set_error("tg_set_gradient_type: gradient_type must be 0, 1 or 2");
+    return TG_E_ARG;
+  }
+  c->grad_type = gradient_type;
+  c->etamu.clear();
+  return TG_OK;
+}
+
+int tg_diagonalize_hsp_and_H11(tg_ctx* c, int constraint_dim, const double* lagrange_lambda1,
+                               double* eigen_hsp, double* eigen_H11, int* info_hsp, int* info_H11,
+                               double* hsp_RR, double* field_H11) {
+  if (!c || !c->have_state || c->ns != 1 || constraint_dim != c->cons_dim ||
+      (constraint_dim > 0 && !lagrange_lambda1)) {
+    tg_set_error("tg_diagonalize_hsp_and_H11: one resident state and the constraints of "
+                 "tg_set_constraints are needed");
+    return TG_E_ARG;
+  }
+  const int n = c->n;
+  const size_t nn = (size_t)n * n;
+  TG_TRY(tg_ensure_z(c));
+  // the multipliers go to the device slot of the gradient step
+  double* lam = c->d_Z.as<double>() + 2 * nn;
+  std::vector<double> lamh(TG_MAX_CONS, 0.0);
+  for (int m = 0; m < constraint_dim; ++m) lamh[m] = lagrange_lambda1[m];
+  TG_HIP(hipMemcpyAsync(lam, lamh.data(), sizeof(double) * TG_MAX_CONS, hipMemcpyHostToDevice,
+                        c->stream));
+  double *hd = nullptr, *h11d = nullptr;
+  std::vector<double> eig;
+  TG_TRY(tg_hsp_h11_device(c, lam, eig, &hd, &h11d));
+  if (eigen_hsp) std::copy(eig.begin(), eig.begin() + n, eigen_hsp);
+  if (eigen_H11) std::copy(eig.begin() + n, eig.begin() + 2 * n, eigen_H11);
+  if (info_hsp) *info_hsp = 0;
+  if (info_H11) *info_H11 = 0;
+  // the matrices handed to the eigenvalue solver: h in the single-particle basis
+  // (rows and columns back to the reference order), H11 in the quasiparticle
+  // basis (the columns of U, V, which keep the reference order)
+  if (hsp_RR) {
+    std::vector<double> hm(nn);
+    TG_HIP(hipMemcpy(hm.data(), hd, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i)
+        hsp_RR[c->sp_perm[i] + (size_t)c->sp_perm[j] * n] = hm[i + (size_t)j * n];
+  }
+  if (field_H11) TG_HIP(hipMemcpy(field_H11, h11d, sizeof(double) * nn, hipMemcpyDeviceToHost));
+  return TG_OK;
+}
+
+int tg_get_hsp_h11_eigenvalues(tg_ctx* c, int state, double* eigen_hsp, double* eigen_H11,
+                               double* gradient_eta, double* gradient_mu) {
+  if (!c || state < 0 || state >= c->ns || c->eig_last.size() != 2 * (size_t)c->ns * c->n) {
+    tg_set_error("tg_get_hsp_h11_eigenvalues: no spectra (gradient_type 1 or 2 iteration)");
+    return TG_E_ARG;
+  }
+  const int n = c->n;
+  const double* e = c->eig_last.data() + 2 * (size_t)state * n;
+  if (eigen_hsp) std::copy(e, e + n, eigen_hsp);
+  if (eigen_H11) std::copy(e + n, e + 2 * n, eigen_H11);
+  if (gradient_eta) *gradient_eta = c->etamu[2 * state];
+  if (gradient_mu) *gradient_mu = c->etamu[2 * state + 1];
+  return TG_OK;
+}
+
+int tg_sym_eigenvalues(int n, int count, const double* A, int lda, double* w) {
+  if (n < 0 || count < 0 || lda < std::max(1, n) || n > 512 || (count > 0 && (!A || !w))) {
+    tg_set_error("tg_sym_eigenvalues: invalid argument (n <= 512)");
+    return TG_E_ARG;
+  }
+  if (count == 0 || n == 0) return TG_OK;
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess || cnt < 1) {
+    tg_set_error("no HIP device available");
+    return TG_E_NODEVICE;
+  }
+  const size_t m = (size_t)lda * n;
+  double *dA = nullptr, *dW = nullptr, *dw = nullptr;
+  int r = TG_OK;
+  if (hipMalloc(&dA, sizeof(double) * m * count) != hipSuccess ||
+      hipMalloc(&dW, sizeof(double) * (size_t)n * n * count) != hipSuccess ||
+      hipMalloc(&dw, sizeof(double) * (size_t)n * count) != hipSuccess) {
+    tg_set_error("tg_sym_eigenvalues: hipMalloc failed");
+    r = TG_E_NOMEM;
+  }
+  if (r == TG_OK && hipMemcpy(dA, A, sizeof(double) * m * count, hipMemcpyHostToDevice) != hipSuccess)
+    r = TG_E_HIP;
+  if (r == TG_OK) r = batched_sym_eigvals(0, n, dA, lda, (long long)m, count, dW, dw, n);
+  if (r == TG_OK && (hipDeviceSynchronize() != hipSuccess ||
+                     hipMemcpy(w, dw, sizeof(double) * (size_t)n * count, hipMemcpyDeviceToHost) !=
+                         hipSuccess))
+    r = TG_E_HIP;
+  if (dA) (void)hipFree(dA);
+  if (dW) (void)hipFree(dW);
+  if (dw) (void)hipFree(dw);
+  return r;
 }
 
 int tg_get_state(tg_ctx* c, double* U, double* V) {

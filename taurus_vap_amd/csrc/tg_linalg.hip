@@ -604,6 +604,232 @@ pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long
   }
 }
 
+// ---------------------------------------------------------------------------
+// Pfaffian on the matrix cores (n <= NMAX <= TH), one workgroup per matrix.
+//
+// Parlett-Reid with the pivoting of ZSKTF2 (UPLO = 'U', MODE = 'P': pivot =
+// first maximum of |Re|+|Im| over rows 0..k-1 of column k, interchanged with
+// row/column k-1), restated for a matrix that never fits a CU:
+//   * only the upper triangle of the leading block is stored and updated,
+//     (4/3) n^3 flops in all;
+//   * pair steps run in panels of PF_PS.  Within a panel nothing is written
+//     to the matrix: thread t stands for stored row t and carries its current
+//     row label (a symmetric interchange kp <-> k-1 swaps two labels) and its
+//     pending rank-2 vectors X_t[s] = c_t / p, Y_t[s] (column k / pivot and
+//     column k-1 of step s), in registers and in LDS.  A column a step needs is
+//     the stored column of the row that carries the label, plus the pending
+//     terms X_t Y_k - Y_t X_k;
+//   * after the panel the leading block [0, m) receives the whole rank-2*PF_PS
+//     update in 16 x 16 tiles of v_mfma_f64_16x16x4f64 (complex as four real
+//     products), read through the label permutation from one buffer and
+//     written in label order to the other (ping-pong between the input, which
+//     is destroyed, and the work matrix), so the interchanges cost no pass of
+//     their own.
+// Pf = prod of the pivots W(k-1,k) x (-1)^(number of interchanges), kept as
+// log|pf| and a phase.
+// ---------------------------------------------------------------------------
+constexpr int PFM_PS = 8;   // pair steps per panel: rank-16 tile updates
+
+typedef double __attribute__((ext_vector_type(4))) pf_d4;
+
+template <int TH, int NMAX>
+__global__ void __launch_bounds__(TH)
+pfaffian_mfma_kernel(int n, cplx* __restrict__ A, int lda, long long sA, cplx* __restrict__ work,
+                     double* __restrict__ logabs, cplx* __restrict__ phase, int* __restrict__ zero) {
+  constexpr int NW = TH / 64;
+  constexpr int LDP = NMAX + 16;        // plane stride (spreads the four k rows over the banks)
+  __shared__ double Xr[PFM_PS * LDP], Xi[PFM_PS * LDP], Yr[PFM_PS * LDP], Yi[PFM_PS * LDP];
+  __shared__ int rowslot[NMAX];         // stored row carrying each label
+  __shared__ double wabs[NW], wcr[NW], wci[NW];
+  __shared__ int wlab[NW], wslot[NW];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const bool own = t < n;
+  cplx* bufA = A + (long long)blockIdx.x * sA;
+  cplx* bufW = work + (long long)blockIdx.x * n * n;
+  cplx* R = bufA;          // stored leading block (read)
+  int ldr = lda;
+  cplx* Wt = bufW;         // updated leading block (written)
+  int ldw = n;
+
+  double la = 0.0;
+  cplx ph = cmk(1.0, 0.0);
+  int z = 0;
+  int k_top = n - 1;
+  cplx X[PFM_PS], Y[PFM_PS];
+  while (k_top >= 1) {
+    int lab = t;
+    if (own) rowslot[t] = t;
+#pragma unroll
+    for (int s2 = 0; s2 < PFM_PS; ++s2) { X[s2] = cmk(0.0, 0.0); Y[s2] = cmk(0.0, 0.0); }
+    __syncthreads();
+    int ns = 0, m = k_top + 1;
+    for (int s = 0; s < PFM_PS; ++s) {
+      const int k = k_top - 2 * s;
+      if (k < 1) break;
+      const int kk = k - 1;
+      // ---- column k (rows labelled < k): stored column of row rowslot[k] + pending
+      const int sk = rowslot[k];
+      const bool act = own && lab < k;
+      cplx c = cmk(0.0, 0.0);
+      if (act) {
+        if (t < sk) {
+          c = R[t + (long long)sk * ldr];
+        } else {
+          const cplx v = R[sk + (long long)t * ldr];
+          c = cmk(-v.x, -v.y);
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < PFM_PS; ++s2)
+          if (s2 < s) {
+            cfma(c, X[s2], cmk(Yr[s2 * LDP + sk], Yi[s2 * LDP + sk]));
+            cfms(c, Y[s2], cmk(Xr[s2 * LDP + sk], Xi[s2 * LDP + sk]));
+          }
+      }
+      // ---- pivot: first maximum of |Re|+|Im| in label order (IZAMAX)
+      double v = act ? cabs1(c) : -1.0;
+      int vl = act ? lab : n;
+      wave_argmax(v, vl);
+      if ((vl == n && lane == 0) || (act && lab == vl)) {
+        wabs[wid] = v;
+        wlab[wid] = vl;
+        wslot[wid] = t;
+        wcr[wid] = c.x;
+        wci[wid] = c.y;
+      }
+      __syncthreads();
+      double mx = wabs[0];
+      int kp = wlab[0], w = wslot[0];
+      double pr = wcr[0], pi = wci[0];
+#pragma unroll
+      for (int q = 1; q < NW; ++q)
+        if (wabs[q] > mx || (wabs[q] == mx && wlab[q] < kp)) {
+          mx = wabs[q]; kp = wlab[q]; w = wslot[q]; pr = wcr[q]; pi = wci[q];
+        }
+      // zero (or non-finite) pivot column: ZSKTF2 INFO > 0, ZSKPFA returns 0
+      if (!(mx > 0.0) || kp >= k) { z = 1; break; }
+      // ---- symmetric interchange kp <-> kk: two labels change hands
+      if (kp != kk) {
+        if (lab == kk) { lab = kp; rowslot[kp] = t; }
+        else if (t == w) { lab = kk; rowslot[kk] = t; }
+        ph = cmk(-ph.x, -ph.y);
+      }
+      const cplx p = cmk(pr, pi);
+      const double ap = sqrt(p.x * p.x + p.y * p.y);
+      la += log(ap);
+      ph = cmul(ph, cmk(p.x / ap, p.y / ap));
+      const cplx pinv = cinv(p);
+      // ---- column kk after the interchange = column of stored row w
+      cplx x = cmk(0.0, 0.0), y = cmk(0.0, 0.0);
+      if (own && lab < kk) {
+        cplx c2;
+        if (t < w) {
+          c2 = R[t + (long long)w * ldr];
+        } else {
+          const cplx u = R[w + (long long)t * ldr];
+          c2 = cmk(-u.x, -u.y);
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < PFM_PS; ++s2)
+          if (s2 < s) {
+            cfma(c2, X[s2], cmk(Yr[s2 * LDP + w], Yi[s2 * LDP + w]));
+            cfms(c2, Y[s2], cmk(Xr[s2 * LDP + w], Xi[s2 * LDP + w]));
+          }
+        x = cmul(c, pinv);
+        y = c2;
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < PFM_PS; ++s2)
+        if (s2 == s) { X[s2] = x; Y[s2] = y; }
+      if (own) {
+        Xr[s * LDP + t] = x.x;
+        Xi[s * LDP + t] = x.y;
+        Yr[s * LDP + t] = y.x;
+        Yi[s * LDP + t] = y.y;
+      }
+      ns = s + 1;
+      m = kk;
+      __syncthreads();
+    }
+    if (z) break;
+    // ---- rank-2*ns update of the upper triangle of the leading block [0, m):
+    //   W(i,j) = stored(slot i, slot j) + sum_s X_i[s] Y_j[s] - Y_i[s] X_j[s]
+    // as D = B^T A^T per 16 x 16 tile, A(i,q) = [X_i | Y_i], B(q,j) = [Y_j ; -X_j],
+    // so that lane l holds rows I0 + l%16 of columns J0 + l/16 + 4r (coalesced).
+    if (m >= 2) {
+      const int nt = (m + 15) >> 4;
+      const int ntiles = nt * (nt + 1) / 2;
+      const int li = lane & 15, lq = lane >> 4;
+      for (int tau = wid; tau < ntiles; tau += NW) {
+        int bj = (int)((sqrt(8.0 * tau + 1.0) - 1.0) * 0.5);
+        while ((bj + 1) * (bj + 2) / 2 <= tau) ++bj;
+        while (bj * (bj + 1) / 2 > tau) --bj;
+        const int bi = tau - bj * (bj + 1) / 2;
+        const int I0 = bi * 16, J0 = bj * 16;
+        const int i = I0 + li, ja = J0 + li;
+        const bool iok = i < m, jok = ja < m;
+        const int ri = iok ? rowslot[i] : 0;
+        const int rja = jok ? rowslot[ja] : 0;
+        // stored values of this lane's four elements (issued before the products)
+        cplx cv[4];
+        bool wr[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int j = J0 + lq + 4 * r;
+          wr[r] = iok && j < m && i < j;
+          cv[r] = cmk(0.0, 0.0);
+          if (wr[r]) {
+            const int rj = rowslot[j];
+            if (ri < rj) {
+              cv[r] = R[ri + (long long)rj * ldr];
+            } else {
+              const cplx u = R[rj + (long long)ri * ldr];
+              cv[r] = cmk(-u.x, -u.y);
+            }
+          }
+        }
+        pf_d4 dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int cstep = 0; cstep < PFM_PS / 2; ++cstep) {
+          const int q = lq + 4 * cstep;          // 0 .. 2*PFM_PS-1
+          const bool xq = q < PFM_PS;
+          const int qs = xq ? q : q - PFM_PS;
+          const bool live = qs < ns;
+          // A operand of the product: B(q, ja) = Y_ja[q] or -X_ja[q - PS]
+          double ar = 0.0, ai = 0.0, br = 0.0, bi2 = 0.0;
+          if (live && jok) {
+            if (xq) { ar = Yr[qs * LDP + rja]; ai = Yi[qs * LDP + rja]; }
+            else    { ar = -Xr[qs * LDP + rja]; ai = -Xi[qs * LDP + rja]; }
+          }
+          // B operand: A(i, q) = X_i[q] or Y_i[q - PS]
+          if (live && iok) {
+            if (xq) { br = Xr[qs * LDP + ri]; bi2 = Xi[qs * LDP + ri]; }
+            else    { br = Yr[qs * LDP + ri]; bi2 = Yi[qs * LDP + ri]; }
+          }
+          dr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, dr, 0, 0, 0);
+          dr = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi2, dr, 0, 0, 0);
+          di = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi2, di, 0, 0, 0);
+          di = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, di, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (wr[r]) {
+            const int j = J0 + lq + 4 * r;
+            Wt[i + (long long)j * ldw] = cmk(cv[r].x + dr[r], cv[r].y + di[r]);
+          }
+      }
+    }
+    __syncthreads();
+    cplx* tp = R; R = Wt; Wt = tp;
+    const int tl = ldr; ldr = ldw; ldw = tl;
+    k_top = m - 1;
+  }
+  if (t == 0) {
+    logabs[blockIdx.x] = z ? -INFINITY : la;
+    phase[blockIdx.x] = z ? cmk(0.0, 0.0) : ph;
+    zero[blockIdx.x] = z;
+  }
+}
+
 }  // namespace
 
 int batched_inverse(hipStream_t s, int n, cplx* A, int lda, long long sA, int batch, cplx* det,
@@ -644,12 +870,35 @@ int batched_inverse(hipStream_t s, int n, cplx* A, int lda, long long sA, int ba
   return TG_OK;
 }
 
-int batched_pfaffian(hipStream_t s, int n, const cplx* A, int lda, long long sA, int batch,
+int batched_pfaffian(hipStream_t s, int n, cplx* A, int lda, long long sA, int batch,
                      cplx* work, double* logabs, cplx* phase, int* zero) {
   if (batch <= 0) return TG_OK;
   if (n % 2 != 0 || n == 0) {
     tg_set_error("batched_pfaffian: n must be even and positive here");
     return TG_E_ARG;
+  }
+  static const bool old_pf = [] {
+    const char* e = std::getenv("TG_PF");
+    return e && std::atoi(e) == 1;
+  }();
+  if (!old_pf && n <= 512) {
+    // matrix-core Pfaffian, one thread per row (at least four waves)
+#define TG_PF2_LAUNCH(TH, NMAX)                                                                \
+  hipLaunchKernelGGL((pfaffian_mfma_kernel<TH, NMAX>), dim3(batch), dim3(TH), 0, s, n, A, lda, sA, \
+                     work, logabs, phase, zero)
+    if (n <= 64)
+      TG_PF2_LAUNCH(256, 64);
+    else if (n <= 128)
+      TG_PF2_LAUNCH(256, 128);
+    else if (n <= 256)
+      TG_PF2_LAUNCH(256, 256);
+    else if (n <= 384)
+      TG_PF2_LAUNCH(384, 384);
+    else
+      TG_PF2_LAUNCH(512, 512);
+#undef TG_PF2_LAUNCH
+    TG_HIP(hipGetLastError());
+    return TG_OK;
   }
   if (n <= PF_THREADS) {
     // one thread per row in the smallest workgroup that holds the rows

@@ -718,16 +718,19 @@ int timing_collect(tg_ctx* c) {
 
 namespace {
 
-// calculate_gradient with fixed eta, mu (gradient_type 0, module_gradient.f90:
-// 122-141): Z = -eta (H20v - sum_j lambda_j C20_j) + mu Zm1; part[b] = sum Z^2.
+// calculate_gradient (module_gradient.f90:122-141): Z = -eta (H20v - sum_j
+// lambda_j C20_j) + mu Zm1; part[b] = sum Z^2.
 __global__ void __launch_bounds__(NT)
-gradient_kernel(long long nn, double eta, double mu, int ncons, const double* __restrict__ lam0,
-                long long ls, const double* __restrict__ c200, long long cs,
-                const double* __restrict__ H20v0, long long hs, const double* __restrict__ Zm10,
-                double* __restrict__ Z0, double* __restrict__ part0, long long ps) {
+gradient_kernel(long long nn, double eta0, double mu0, const double* __restrict__ etamu, int ncons,
+                const double* __restrict__ lam0, long long ls, const double* __restrict__ c200,
+                long long cs, const double* __restrict__ H20v0, long long hs,
+                const double* __restrict__ Zm10, double* __restrict__ Z0,
+                double* __restrict__ part0, long long ps) {
   // state blockIdx.y: lambda at + s * ls, constraint_20 at + s * cs, field_H20v at
-  // + s * hs, Z / Zm1 at + s * nn, partial sums at + s * ps
+  // + s * hs, Z / Zm1 at + s * nn, partial sums at + s * ps; eta, mu per state
+  // from etamu (gradient_type 1, 2) or the common eta0, mu0
   const long long st = blockIdx.y;
+  const double eta = etamu ? etamu[2 * st] : eta0, mu = etamu ? etamu[2 * st + 1] : mu0;
   const double* lam = lam0 + st * ls;
   const double* c20 = c200 + st * cs;
   const double* H20v = H20v0 + st * hs;
@@ -1271,7 +1274,8 @@ int project_run(tg_ctx* c) {
     const long long ls = c->c20_shared ? 0 : TG_MAX_CONS;
     const long long cs = c->c20_shared ? 0 : (long long)c->grad_ncons * nn;
     hipLaunchKernelGGL(gradient_kernel, dim3(nblk(nn), S), dim3(NT), 0, s, nn, c->grad_eta,
-                       c->grad_mu, c->grad_ncons, lam, ls, c->d_c20.as<double>(), cs, H20v,
+                       c->grad_mu, (const double*)nullptr, c->grad_ncons, lam, ls,
+                       c->d_c20.as<double>(), cs, H20v,
                        2 * P.outs, Zprev, Zcur, part, (long long)tg_part_len(n));
     TG_HIP(hipGetLastError());
     hipLaunchKernelGGL(copy_real_kernel, dim3(nblk(ks)), dim3(NT), 0, s, ks, Zcur, Zprev);
@@ -1550,8 +1554,16 @@ int tg_gradient_step(tg_ctx* c, const double* c20, const double* lam) {
   double* Zb = c->d_Z.as<double>();
   double* part = Zb + 2 * S * nn + (size_t)S * TG_MAX_CONS;
   const double* H20v = reinterpret_cast<const double*>(c->d_out.as<cplx>() + 3 * nn);
+  // per-state eta, mu of the gradient recipes (gradient_type 1, 2), else the given ones
+  const double* em = nullptr;
+  if (c->grad_type > 0 && c->etamu.size() == 2 * (size_t)S) {
+    TG_TRY(c->d_etamu.ensure(sizeof(double) * 2 * S));
+    TG_HIP(hipMemcpyAsync(c->d_etamu.p, c->etamu.data(), sizeof(double) * 2 * S,
+                          hipMemcpyHostToDevice, c->stream));
+    em = c->d_etamu.as<double>();
+  }
   hipLaunchKernelGGL(gradient_kernel, dim3(nblk(nn), S), dim3(NT), 0, c->stream, nn, c->grad_eta,
-                     c->grad_mu, c->grad_ncons, lam, (long long)TG_MAX_CONS, c20,
+                     c->grad_mu, em, c->grad_ncons, lam, (long long)TG_MAX_CONS, c20,
                      (long long)c->grad_ncons * nn, H20v, 2 * tg_out_stride(n), Zb + S * nn, Zb,
                      part, tg_part_len(n));
   TG_HIP(hipGetLastError());
@@ -1748,6 +1760,39 @@ int tg_calculate_fields_diag(tg_ctx* c, const double* rhoLR, const double* kappa
   permute_out(c, h.data() + 2 * nn, reinterpret_cast<cplx*>(deltaLR));
   if (deltaRL) permute_out(c, h.data() + 3 * nn, reinterpret_cast<cplx*>(deltaRL));
   return TG_OK;
+}
+
+int tg_inverse(int n, int count, double* A, int lda, double* det, int* info) {
+  if (n < 0 || count < 0 || lda < std::max(1, n) || (count > 0 && (!A || !det || !info))) {
+    tg_set_error("tg_inverse: invalid argument");
+    return TG_E_ARG;
+  }
+  if (count == 0 || n == 0) return TG_OK;
+  TG_TRY(get_device_ready());
+  const size_t m = (size_t)lda * n;
+  cplx *dA = nullptr, *dDet = nullptr;
+  int *dInfo = nullptr, *dPerm = nullptr;
+  int r = TG_OK;
+  if (hipMalloc(&dA, sizeof(cplx) * m * count) != hipSuccess ||
+      hipMalloc(&dDet, sizeof(cplx) * count) != hipSuccess ||
+      hipMalloc(&dInfo, sizeof(int) * count) != hipSuccess ||
+      hipMalloc(&dPerm, sizeof(int) * (size_t)n * count) != hipSuccess) {
+    tg_set_error("tg_inverse: hipMalloc failed");
+    r = TG_E_NOMEM;
+  }
+  if (r == TG_OK && hipMemcpy(dA, A, sizeof(cplx) * m * count, hipMemcpyHostToDevice) != hipSuccess)
+    r = TG_E_HIP;
+  if (r == TG_OK) r = batched_inverse(0, n, dA, lda, (long long)m, count, dDet, dInfo, dPerm);
+  if (r == TG_OK && (hipDeviceSynchronize() != hipSuccess ||
+                     hipMemcpy(A, dA, sizeof(cplx) * m * count, hipMemcpyDeviceToHost) != hipSuccess ||
+                     hipMemcpy(det, dDet, sizeof(cplx) * count, hipMemcpyDeviceToHost) != hipSuccess ||
+                     hipMemcpy(info, dInfo, sizeof(int) * count, hipMemcpyDeviceToHost) != hipSuccess))
+    r = TG_E_HIP;
+  if (dA) (void)hipFree(dA);
+  if (dDet) (void)hipFree(dDet);
+  if (dInfo) (void)hipFree(dInfo);
+  if (dPerm) (void)hipFree(dPerm);
+  return r;
 }
 
 int tg_zskpfa(int n, int count, const double* A, int lda, double* pfaff, int* info) {
@@ -2089,38 +2134,22 @@ int tg_get_gradient(tg_ctx* c, double* gradient_Zi, double* gradient_norm) {
   return tg_get_gradient_state(c, 0, gradient_Zi, gradient_norm);
 }
 
-int tg_project_observables(tg_ctx* c, int nobo, const double* obo_ops, int nobos,
-                           const double* obos_ops, int nshell, const int* sp_shell,
-                           double* per_angle, double* pnp_rho) {
-  if (!c || !c->have_state || c->plain || c->ns != 1 || nobo < 0 || nobos < 0 || nshell < 0 ||
-      (nobo > 0 && !obo_ops) || (nobos > 0 && !obos_ops) || (nshell > 0 && !sp_shell) ||
-      !per_angle) {
-    tg_set_error("tg_project_observables: invalid argument (one resident state, batched mode)");
-    return TG_E_ARG;
-  }
+}  // extern "C"
+
+namespace {
+// Observables of the angles of the current grid (c->A of them, all resident in
+// the work buffers after project_device): records a0 .. a0 + A - 1 of per_angle,
+// and the rho term of sum_gauge added to rho_acc (complex n^2, internal order).
+int observables_grid(tg_ctx* c, int nobo, const double* obo_ops, int nobos,
+                     const double* obos_ops, int nshell, const int* sp_shell, int a0,
+                     double* per_angle, cplx* rho_acc) {
   const int n = c->n, h = n / 2;
   const long long nn = (long long)n * n;
-  for (int i = 0; i < n && nshell > 0; ++i)
-    if (sp_shell[i] < 0 || sp_shell[i] >= nshell) {
-      tg_set_error("tg_project_observables: shell index out of range");
-      return TG_E_ARG;
-    }
-  // every angle of the grid (no pairing of conjugate angles) and the parity
-  // overlaps (iopt + iprint > 0, :279-284)
-  if (c->A != c->Mphip * c->Mphin) assign_angles(c, false);
-  c->iprint = 1;
-  c->state_dirty = 1;
-  const int grad_was = c->grad_on;
-  c->grad_on = 0;
-  const int r = project_device(c);
-  c->grad_on = grad_was;
-  TG_TRY(r);
   const RunPlan P = make_plan(c);
   const int A = P.A;
   hipStream_t s = c->stream;
   if (P.chunk != P.B) {
-    // the observables read every angle's densities and fields after the loop
-    tg_set_error("tg_project_observables: the angles do not fit into device memory at once");
+    tg_set_error("tg_project_observables: the angles of a block do not fit into device memory");
     return TG_E_NOMEM;
   }
   // device layout: ops (nobo n^2) | obos (2 nobos n^2) | ecomp (12 A) | occ (2 nshell A)
@@ -2198,7 +2227,7 @@ int tg_project_observables(tg_ctx* c, int nobo, const double* obo_ops, int nobos
   const cplx* o_obo = o_occ + 2 * (size_t)nshell * A;
   const cplx* o_obs = o_obo + 2 * (size_t)nobo * A;
   for (int a = 0; a < A; ++a) {
-    cplx* q = rec + R * a;
+    cplx* q = rec + R * (a0 + a);
     const cplx* t = tr.data() + 9 * (size_t)a;
     q[0] = ov[a];
     q[1] = pa[a];
@@ -2215,11 +2244,83 @@ int tg_project_observables(tg_ctx* c, int nobo, const double* obo_ops, int nobos
       for (int m = 0; m < 4; ++m)
         q[19 + 2 * nshell + 2 * nobo + 4 * k + m] = o_obs[(size_t)4 * k * A + 4 * a + m];
   }
-  if (pnp_rho) {
-    std::vector<cplx> hr(nn);
-    for (long long i = 0; i < nn; ++i) hr[i] = out[nc_out - nn + i];
-    permute_out(c, hr.data(), reinterpret_cast<cplx*>(pnp_rho));
+  for (long long i = 0; i < nn; ++i) rho_acc[i] = cadd(rho_acc[i], out[nc_out - nn + i]);
+  return TG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tg_project_observables(tg_ctx* c, int nobo, const double* obo_ops, int nobos,
+                           const double* obos_ops, int nshell, const int* sp_shell,
+                           double* per_angle, double* pnp_rho) {
+  if (!c || !c->have_state || c->ns != 1 || nobo < 0 || nobos < 0 || nshell < 0 ||
+      (nobo > 0 && !obo_ops) || (nobos > 0 && !obos_ops) || (nshell > 0 && !sp_shell) ||
+      !per_angle) {
+    tg_set_error("tg_project_observables: invalid argument (one resident state)");
+    return TG_E_ARG;
   }
+  const int n = c->n, h = n / 2;
+  const long long nn = (long long)n * n;
+  for (int i = 0; i < n && nshell > 0; ++i)
+    if (sp_shell[i] < 0 || sp_shell[i] >= nshell) {
+      tg_set_error("tg_project_observables: shell index out of range");
+      return TG_E_ARG;
+    }
+  // every angle of the grid (no pairing of conjugate angles) with the parity
+  // overlaps (iopt + iprint > 0, :279-284), in blocks of angles: all at once when
+  // they fit, one angle at a time in the plain per-angle mode, blocks of the
+  // memory-short chunk size otherwise.  The caller's angle set-up, iprint and
+  // gradient step are restored afterwards.
+  const std::vector<double> sv_phip = c->phip, sv_phin = c->phin, sv_mult = c->mult;
+  const std::vector<cplx> sv_weight = c->weight;
+  const int sv_A = c->A, sv_iprint = c->iprint, sv_grad = c->grad_on, sv_graph = c->use_graph;
+  auto restore = [&]() {
+    c->phip = sv_phip;
+    c->phin = sv_phin;
+    c->mult = sv_mult;
+    c->weight = sv_weight;
+    c->A = sv_A;
+    c->iprint = sv_iprint;
+    c->grad_on = sv_grad;
+    c->use_graph = sv_graph;
+    c->state_dirty = 1;
+  };
+  assign_angles(c, false);
+  const std::vector<double> all_phip = c->phip, all_phin = c->phin, all_mult = c->mult;
+  const std::vector<cplx> all_weight = c->weight;
+  const int Aall = c->A;
+  c->iprint = 1;
+  c->grad_on = 0;
+  c->use_graph = 0;
+  std::vector<cplx> rho_acc((size_t)nn, cmk(0.0, 0.0));
+  int blk = c->plain ? 1 : Aall;
+  int r = TG_OK;
+  for (int a0 = 0; a0 < Aall && r == TG_OK;) {
+    const int na = std::min(blk, Aall - a0);
+    c->phip.assign(all_phip.begin() + a0, all_phip.begin() + a0 + na);
+    c->phin.assign(all_phin.begin() + a0, all_phin.begin() + a0 + na);
+    c->mult.assign(all_mult.begin() + a0, all_mult.begin() + a0 + na);
+    c->weight.assign(all_weight.begin() + a0, all_weight.begin() + a0 + na);
+    c->A = na;
+    c->state_dirty = 1;
+    r = project_device(c);
+    if (r != TG_OK) break;
+    const RunPlan P = make_plan(c);
+    if (P.chunk != P.B) {
+      // memory-short: the device holds P.chunk angles' work matrices at once
+      if (P.chunk >= na) { r = TG_E_NOMEM; break; }
+      blk = P.chunk;
+      continue;
+    }
+    r = observables_grid(c, nobo, obo_ops, nobos, obos_ops, nshell, sp_shell, a0, per_angle,
+                         rho_acc.data());
+    a0 += na;
+  }
+  restore();
+  TG_TRY(r);
+  if (pnp_rho) permute_out(c, rho_acc.data(), reinterpret_cast<cplx*>(pnp_rho));
   return TG_OK;
 }
 
@@ -2244,7 +2345,9 @@ int tg_get_gradient_state(tg_ctx* c, int state, double* gradient_Zi, double* gra
                    sizeof(double) * nb, hipMemcpyDeviceToHost));
   double sum = 0;
   for (double v : part) sum += v;
-  if (gradient_norm) *gradient_norm = sqrt(sum) / c->grad_eta;
+  const double eta = (c->grad_type > 0 && c->etamu.size() == 2 * (size_t)S) ? c->etamu[2 * state]
+                                                                            : c->grad_eta;
+  if (gradient_norm) *gradient_norm = sqrt(sum) / eta;
   return TG_OK;
 }
 

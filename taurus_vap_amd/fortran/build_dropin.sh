@@ -4,9 +4,8 @@
 # Compiles the reference sources (read from REFSRC, default /root/reference/src)
 # together with taurus_gpu.f90 and links libtaurus_gpu.so.  The reference's
 # module_projection.f90 gets the two-line change a maintainer makes to adopt the
-# library: its project_wavefunction is renamed project_wavefunction_cpu (kept for
-# the iopt = 1 printing path) and project_wavefunction_gpu.inc is included in the
-# module.  The edited copy lives in a scratch directory; outputs go to
+# library: its project_wavefunction is removed and project_wavefunction_gpu.inc is
+# included in the module in its place.  The edited copy lives in a scratch directory; outputs go to
 # taurus_vap_amd/fortran/_build/ (taurus_vap_gpu32.exe: stock rH2 = real32,
 # taurus_vap_gpu64.exe: rH2 = real64).
 set -euo pipefail
@@ -41,13 +40,22 @@ build_variant () {
   if [ "$kind" = real64 ]; then
     sed -i 's/rH2 = real32/rH2 = real64/' "$d/module_constants.f90"
   fi
-  # adopt the library in MODULE Projection
-  sed -i -e 's/^subroutine project_wavefunction(iopt,iprint,Mphip,Mphin,ndim)/subroutine project_wavefunction_cpu(iopt,iprint,Mphip,Mphin,ndim)/' \
-         -e 's/^end subroutine project_wavefunction *$/end subroutine project_wavefunction_cpu/' \
+  # adopt the library in MODULE Projection: the reference's project_wavefunction
+  # is removed and the drop-in included in its place
+  sed -i -e '/^subroutine project_wavefunction(iopt,iprint,Mphip,Mphin,ndim)/,/^end subroutine project_wavefunction *$/d' \
          -e "s/^END MODULE Projection/include 'project_wavefunction_gpu.inc'\n\nEND MODULE Projection/" \
          "$d/module_projection.f90"
-  [ "$(grep -c 'project_wavefunction_cpu' "$d/module_projection.f90")" = 2 ] || { echo "patch failed" >&2; exit 4; }
+  [ "$(grep -c '^subroutine project_wavefunction(' "$d/module_projection.f90")" = 0 ] || { echo "patch failed" >&2; exit 4; }
   grep -q "include 'project_wavefunction_gpu.inc'" "$d/module_projection.f90" || { echo "patch failed" >&2; exit 4; }
+  # and in MODULE Gradient: diagonalize_hsp_and_H11(0) on the device, the
+  # reference's routine kept under another name for opt = 1 (eigenbasis files)
+  cp "$HERE/diagonalize_hsp_and_H11_gpu.inc" "$d/"
+  sed -i -e 's/^subroutine diagonalize_hsp_and_H11(opt,ndim)/subroutine diagonalize_hsp_and_H11_files(opt,ndim)/' \
+         -e 's/^end subroutine diagonalize_hsp_and_H11 *$/end subroutine diagonalize_hsp_and_H11_files/' \
+         -e "s/^END MODULE Gradient/include 'diagonalize_hsp_and_H11_gpu.inc'\n\nEND MODULE Gradient/" \
+         "$d/module_gradient.f90"
+  [ "$(grep -c 'diagonalize_hsp_and_H11_files' "$d/module_gradient.f90")" = 2 ] || { echo "patch failed" >&2; exit 4; }
+  grep -q "include 'diagonalize_hsp_and_H11_gpu.inc'" "$d/module_gradient.f90" || { echo "patch failed" >&2; exit 4; }
   ( cd "$d"
     for m in $PRE; do $FC $OPT -c $m.f90; done
     $FC $OPT -c taurus_gpu.f90

@@ -178,6 +178,19 @@ interface
     real(c_double), intent(out) :: ovacc(*), voveru(*)
   end function
 
+  ! diagonalize_hsp_and_H11(0, ndim) of the resident state
+  integer(c_int) function tg_diagonalize_hsp_and_H11(ctx, constraint_dim, lagrange_lambda1, &
+                                                     eigen_hsp, eigen_H11, info_hsp, info_H11, &
+                                                     hsp_RR, field_H11) &
+                                                     bind(C, name='tg_diagonalize_hsp_and_H11')
+    import
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: constraint_dim
+    real(c_double), intent(in) :: lagrange_lambda1(*)
+    real(c_double), intent(out) :: eigen_hsp(*), eigen_H11(*), hsp_RR(*), field_H11(*)
+    integer(c_int), intent(out) :: info_hsp, info_H11
+  end function
+
   ! per-angle observables of project_wavefunction(1, ...) (final printing)
   integer(c_int) function tg_project_observables(ctx, nobo, obo_ops, nobos, obos_ops, &
                                                  nshell, sp_shell, per_angle, pnp_rho) &

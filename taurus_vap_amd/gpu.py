@@ -39,7 +39,9 @@ EXPORTS = [
     "tg_iterate", "tg_get_state", "tg_set_states", "tg_get_results_state",
     "tg_get_gradient_state", "tg_group_create", "tg_group_destroy", "tg_group_project",
     "tg_group_iterate", "tg_group_iterate_results", "tg_project_observables",
-    "tg_set_batch_limit", "tg_canonical_basis",
+    "tg_set_batch_limit", "tg_canonical_basis", "tg_set_gradient_type",
+    "tg_get_hsp_h11_eigenvalues", "tg_diagonalize_hsp_and_H11", "tg_sym_eigenvalues",
+    "tg_inverse",
 ]
 
 _lib = None
@@ -67,6 +69,7 @@ def load_library():
     lib.tg_calculate_fields.argtypes = [P] * 8
     lib.tg_calculate_fields_diag.argtypes = [P] * 7
     lib.tg_zskpfa.argtypes = [I, I, P, I, P, P]
+    lib.tg_inverse.argtypes = [I, I, P, I, P, P]
     lib.tg_calculate_overlap.argtypes = [I, I, I, D, P, P, I, I, D, P, P, P, P]
     lib.tg_project_wavefunction.argtypes = [P, I, P, P, P, I, I, D, P, I, I, D, D, D, D, P, P, P, P, P, P]
     lib.tg_set_state.argtypes = [P, P, P, P, I, I, D, P, I, I, D, D, D, D, I]
@@ -93,6 +96,10 @@ def load_library():
     lib.tg_project_observables.argtypes = [P, I, P, I, P, I, P, P, P]
     lib.tg_set_batch_limit.argtypes = [P, I]
     lib.tg_canonical_basis.argtypes = [P, D, P, P, P, P]
+    lib.tg_set_gradient_type.argtypes = [P, I]
+    lib.tg_get_hsp_h11_eigenvalues.argtypes = [P, I, P, P, P, P]
+    lib.tg_diagonalize_hsp_and_H11.argtypes = [P, I, P, P, P, P, P, P, P]
+    lib.tg_sym_eigenvalues.argtypes = [I, I, P, I, P]
     _lib = lib
     return lib
 
@@ -324,6 +331,37 @@ class Context:
         out = self._iterate_dicts(*bufs)
         return out[0] if self.nstates == 1 else out
 
+    def set_gradient_type(self, gradient_type):
+        """gradient_type of the device iteration: 0 (eta, mu as given), 1 / 2 (the
+        recipes from the H11 / h spectra, diagonalize_hsp_and_H11 on the device)."""
+        _check(load_library().tg_set_gradient_type(self._h, int(gradient_type)),
+               "tg_set_gradient_type")
+
+    def hsp_h11_eigenvalues(self, state=0):
+        """(eigen_hsp, eigen_H11, eta, mu) of the last gradient_type 1 / 2 iteration."""
+        eh = np.zeros(self.n)
+        e11 = np.zeros(self.n)
+        em = np.zeros(2)
+        _check(load_library().tg_get_hsp_h11_eigenvalues(self._h, int(state), _p(eh), _p(e11),
+                                                         _p(em[:1]), _p(em[1:])),
+               "tg_get_hsp_h11_eigenvalues")
+        return eh, e11, float(em[0]), float(em[1])
+
+    def diagonalize_hsp_and_H11(self, lagrange_lambda1=None):
+        """diagonalize_hsp_and_H11(0) of the resident state: (eigen_hsp, eigen_H11,
+        hsp_RR, field_H11) with the constraints of set_constraints."""
+        dim = getattr(self, "constraint_dim", 0)
+        lam = np.ascontiguousarray(np.asarray(lagrange_lambda1 if dim else [0.0], dtype=np.float64))
+        n = self.n
+        eh, e11 = np.zeros(n), np.zeros(n)
+        hm = np.zeros((n, n), order="F")
+        h11 = np.zeros((n, n), order="F")
+        info = np.zeros(2, np.int32)
+        _check(load_library().tg_diagonalize_hsp_and_H11(self._h, dim, _p(lam), _p(eh), _p(e11),
+                                                         _p(info[:1]), _p(info[1:]), _p(hm),
+                                                         _p(h11)), "tg_diagonalize_hsp_and_H11")
+        return eh, e11, hm, h11
+
     def get_state(self):
         """(U, V) of the device state, or lists of them for several states."""
         n, S = self.n, self.nstates
@@ -438,6 +476,37 @@ def zskpfa(A):
     info = np.zeros(cnt, np.int32)
     _check(load_library().tg_zskpfa(n, cnt, _p(buf), max(1, n), _p(pf), _p(info)), "tg_zskpfa")
     return (pf[0], int(info[0])) if single else (pf, info)
+
+
+def inverse(A):
+    """tg_inverse on one complex matrix or a stack (count, n, n): (inverses, det, info)."""
+    A = np.asarray(A, dtype=np.complex128)
+    single = A.ndim == 2
+    if single:
+        A = A[None]
+    cnt, n, _ = A.shape
+    buf = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))
+    det = np.zeros(cnt, np.complex128)
+    info = np.zeros(cnt, np.int32)
+    _check(load_library().tg_inverse(n, cnt, _p(buf), max(1, n), _p(det), _p(info)), "tg_inverse")
+    inv = np.transpose(buf, (0, 2, 1))
+    return (inv[0], det[0], int(info[0])) if single else (inv, det, info)
+
+
+def sym_eigenvalues(A):
+    """Eigenvalues (ascending) of one real symmetric matrix or a stack (count, n, n),
+    upper triangles read (tg_sym_eigenvalues)."""
+    A = np.asarray(A, dtype=np.float64)
+    single = A.ndim == 2
+    if single:
+        A = A[None]
+    cnt, n, _ = A.shape
+    buf = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))
+    w = np.zeros(cnt * n)
+    _check(load_library().tg_sym_eigenvalues(n, cnt, _p(buf), max(1, n), _p(w)),
+           "tg_sym_eigenvalues")
+    w = w.reshape(cnt, n)
+    return w[0] if single else w
 
 
 def calculate_overlap(nocc0, nemp0, ovac0, voveru0, D0, nocc1, nemp1, ovac1, voveru1, D1, ROT):

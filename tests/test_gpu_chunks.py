@@ -98,17 +98,42 @@ def test_chunked_states_hold_whole_states():
         assert np.max(np.abs(Z - Zref)) <= 1e-10
 
 
-def test_observables_refuse_chunks():
-    from taurus_vap_amd import gpu
+def test_observables_in_blocks_match_all_at_once():
+    """tg_project_observables when the angles do not run all at once: blocks of the
+    memory-short chunk size (batch limit 5) and one angle at a time (the plain
+    per-angle mode) give the per-angle records and the rho sum of the all-at-once
+    run; a projection after it (conjugate pairs, gradient step) still gives its
+    own results: the caller's angle set-up, iprint and gradient are restored."""
     g = load_golden("sd64")
+    n = int(g["ndim"])
     cb = golden_canon(g)
-    ctx = _ctx(g)
-    try:
-        ctx.set_batch_limit(5)
-        ctx.set_state(g["U0"], g["V0"], cb["D"].real, cb["nocc"], cb["nemp"], cb["ovac"],
-                      cb["voveru"], int(g["Mphip"]), int(g["Mphin"]), float(g["facpi"]),
-                      float(g["valence_Z"]), float(g["valence_N"]), float(g["factor_delta"]))
-        with pytest.raises(gpu.TaurusGPUError, match="-3"):
-            ctx.project_observables()
-    finally:
-        ctx.close()
+    rng = np.random.default_rng(4)
+    ops = [rng.normal(size=(n, n)) for _ in range(2)]
+    pairs = [(ops[0], ops[0] @ ops[0])]
+    shell = np.arange(n) % 3
+    out = {}
+    for mode in ("all", "blocks", "plain"):
+        ctx = _ctx(g, plain=(mode == "plain"), conj_pairs=(mode == "all"))
+        try:
+            if mode == "blocks":
+                ctx.set_batch_limit(5)
+            ctx.set_gradient(0.01, 0.2)
+            ctx.set_state(g["U0"], g["V0"], cb["D"].real, cb["nocc"], cb["nemp"], cb["ovac"],
+                          cb["voveru"], int(g["Mphip"]), int(g["Mphin"]), float(g["facpi"]),
+                          float(g["valence_Z"]), float(g["valence_N"]), float(g["factor_delta"]))
+            ctx.project_resident()
+            before = ctx.get_results()
+            print("observables", mode, flush=True)
+            out[mode] = ctx.project_observables(ops, pairs, shell, 3)
+            ctx.project_resident()
+            after = ctx.get_results()
+            for k in ("over", "ener", "prot", "neut"):
+                assert abs(after[k] - before[k]) <= 1e-13 * max(1.0, abs(before[k])), (mode, k)
+            assert np.max(np.abs(after["H20v"] - before["H20v"])) <= 1e-13, mode
+        finally:
+            ctx.close()
+    rec, rho = out["all"]
+    for mode in ("blocks", "plain"):
+        r2, rho2 = out[mode]
+        assert np.max(np.abs(r2 - rec)) <= 1e-11 * max(1.0, np.max(np.abs(rec))), mode
+        assert np.max(np.abs(rho2 - rho)) <= 1e-11 * max(1.0, np.max(np.abs(rho))), mode

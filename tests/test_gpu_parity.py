@@ -268,3 +268,31 @@ def test_adjoint_gradient_emax4_finite_difference():
             assert abs(fd - pred) < 1e-5 * abs(pred) + 2e-3, (fd, pred)
     finally:
         ctx.close()
+
+
+def test_batched_inverse_large_orders():
+    """tg_inverse (the ZGETRF + ZGETRI pair of calculate_thouless) at the orders of
+    the e_max = 4 / 6 inverses and up to 512 (each workgroup size of the blocked
+    Gauss-Jordan kernel): A A^-1 = 1, the LAPACK inverse and determinant, and a
+    singular matrix's info."""
+    from taurus_vap_amd import gpu
+    rng = np.random.default_rng(21)
+    for n in (24, 64, 128, 129, 140, 300, 336, 384, 512):
+        # entries of variance 1/n keep the determinant within the FP64 range
+        A = (rng.normal(size=(3, n, n)) + 1j * rng.normal(size=(3, n, n))) / np.sqrt(2.0 * n)
+        print("inverse n=%d" % n, flush=True)
+        inv, det, info = gpu.inverse(A)
+        assert np.all(info == 0)
+        for k in range(3):
+            ref = np.linalg.inv(A[k])
+            err = np.max(np.abs(inv[k] - ref)) / np.max(np.abs(ref))
+            assert err < 1e-11 * n, (n, k, err)
+            assert np.max(np.abs(A[k] @ inv[k] - np.eye(n))) < 1e-11 * n
+            sign, logdet = np.linalg.slogdet(A[k])
+            assert abs(np.log(abs(det[k])) - logdet) < 1e-10 * n, (n, k)
+            assert abs(det[k] / abs(det[k]) - sign) < 1e-10 * n, (n, k)
+    # a zero column: info = first zero pivot column (1-based), det = 0
+    A = rng.normal(size=(200, 200)) + 0j
+    A[:, 57] = 0
+    _, det, info = gpu.inverse(A)
+    assert info == 58 and det == 0
