@@ -22,6 +22,7 @@
 #ifndef TAURUS_GPU_H
 #define TAURUS_GPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -120,6 +121,20 @@ int tg_inverse(int n, int count, double* A, int lda, double* det, int* info);
  * pfaff: complex[count]; info: int[count].  A is not modified.
  */
 int tg_zskpfa(int n, int count, const double* A, int lda, double* pfaff, int* info);
+
+/*
+ * tg_zskpfa_noalloc -- tg_zskpfa without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Every device buffer of the call (the copy of A, the work matrices, the
+ * per-matrix results) comes from one static device array of
+ * TG_NOALLOC_SCRATCH_BYTES per device, held until the process ends; a call that
+ * needs more (count * 16 * (lda*n + n*n) bytes plus a few per matrix) returns
+ * TG_E_ARG.  Calls on one device share that array: calls that overlap on one
+ * device race.  Results are those of tg_zskpfa.
+ */
+#define TG_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int tg_zskpfa_noalloc(int n, int count, const double* A, int lda, double* pfaff, int* info);
 
 /*
  * tg_calculate_overlap -- calculate_overlap(nocc0,nemp0,ovac0,voveru0,D0,nocc1,

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtaurus_gpu.so")
 SOURCES = ["tg_gemm.hip", "tg_mfma.hip", "tg_linalg.hip", "tg_fields.hip", "tg_project.hip",
-           "tg_iterate.hip", "tg_canon.hip", "tg_a19.hip"]
+           "tg_iterate.hip", "tg_canon.hip", "tg_a19.hip", "tg_noalloc.hip"]
 ARCH = os.environ.get("TAURUS_GPU_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

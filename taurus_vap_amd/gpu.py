@@ -41,7 +41,7 @@ EXPORTS = [
     "tg_group_iterate", "tg_group_iterate_results", "tg_project_observables",
     "tg_set_batch_limit", "tg_canonical_basis", "tg_set_gradient_type",
     "tg_get_hsp_h11_eigenvalues", "tg_diagonalize_hsp_and_H11", "tg_sym_eigenvalues",
-    "tg_inverse",
+    "tg_zskpfa_noalloc", "tg_inverse",
 ]
 
 _lib = None
@@ -69,6 +69,7 @@ def load_library():
     lib.tg_calculate_fields.argtypes = [P] * 8
     lib.tg_calculate_fields_diag.argtypes = [P] * 7
     lib.tg_zskpfa.argtypes = [I, I, P, I, P, P]
+    lib.tg_zskpfa_noalloc.argtypes = [I, I, P, I, P, P]
     lib.tg_inverse.argtypes = [I, I, P, I, P, P]
     lib.tg_calculate_overlap.argtypes = [I, I, I, D, P, P, I, I, D, P, P, P, P]
     lib.tg_project_wavefunction.argtypes = [P, I, P, P, P, I, I, D, P, I, I, D, D, D, D, P, P, P, P, P, P]
@@ -475,6 +476,22 @@ def zskpfa(A):
     pf = np.zeros(cnt, np.complex128)
     info = np.zeros(cnt, np.int32)
     _check(load_library().tg_zskpfa(n, cnt, _p(buf), max(1, n), _p(pf), _p(info)), "tg_zskpfa")
+    return (pf[0], int(info[0])) if single else (pf, info)
+
+
+def zskpfa_noalloc(A):
+    """zskpfa through tg_zskpfa_noalloc (no device allocation: the library's static
+    per-device scratch array; limited to TG_NOALLOC_SCRATCH_BYTES per call)."""
+    A = np.asarray(A, dtype=np.complex128)
+    single = A.ndim == 2
+    if single:
+        A = A[None]
+    cnt, n, _ = A.shape
+    buf = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))
+    pf = np.zeros(cnt, np.complex128)
+    info = np.zeros(cnt, np.int32)
+    _check(load_library().tg_zskpfa_noalloc(n, cnt, _p(buf), max(1, n), _p(pf), _p(info)),
+           "tg_zskpfa_noalloc")
     return (pf[0], int(info[0])) if single else (pf, info)
 
 
