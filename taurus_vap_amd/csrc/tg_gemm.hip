@@ -392,6 +392,9 @@ int gemm_dd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, double alpha
             double beta, double* C, int ldc, long long sC, int batch, int divA, int divB) {
   if (opa == OP_C) opa = OP_T;
   if (opb == OP_C) opb = OP_T;
+  if (sel().mfma)
+    return mfma_gemm_dd(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, beta != 0.0,
+                        C, ldc, sC, batch, divA, divB);
   return dispatch<double, double, double>(s, opa, opb, M, N, K, alpha, A, lda, sA, B, ldb, sB,
                                           beta, beta != 0.0, C, ldc, sC, batch, divA, divB);
 }

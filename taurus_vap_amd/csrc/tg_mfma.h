@@ -27,3 +27,7 @@ int mfma_gemm_zd(int variant, hipStream_t s, TgOp opa, TgOp opb, int M, int N, i
 int mfma_gemm_grouped(int variant, hipStream_t s, int M, const cplx* X, cplx* C, const void* W,
                       int w_is_fp32, const GroupDesc* groups_dev, const int* tile_group_dev,
                       const int* tile_n0_dev, int ntiles_total);
+int mfma_gemm_dd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, double alpha,
+                 const double* A, int lda, long long sA, const double* B, int ldb, long long sB,
+                 double beta, int use_beta, double* C, int ldc, long long sC, int batch,
+                 int divA = 1, int divB = 1);

@@ -237,6 +237,72 @@ mfma_batched_kernel(int M, int N, int K, cplx alpha, const TA* __restrict__ A, i
          C + b * sC, ldc, tm * BM, tn * BN, sm);
 }
 
+// Real FP64 GEMM on the matrix cores for the real products of the constraint
+// machinery and the canonical basis (n x n, batched over states):
+// C[b] = alpha op(A[b]) op(B[b]) + beta C[b].  A 32 x 32 tile per workgroup of
+// four waves, one 16 x 16 MFMA block per wave, K staged 16 deep in LDS.  Each
+// wave computes the transposed block (D = op(B)^T op(A)^T), so that lane l
+// holds rows l % 16 of four columns: the stores are unit-stride.
+template <int OPA, int OPB>
+__global__ void __launch_bounds__(256)
+mfma_dd_kernel(int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
+               long long sA, const double* __restrict__ B, int ldb, long long sB, double beta,
+               int use_beta, double* __restrict__ C, int ldc, long long sC, int tiles_m, int divA,
+               int divB) {
+  constexpr int BT = 32, BK = 16, LD = BT + 1;
+  __shared__ double As[BK][LD], Bs[BK][LD];     // As[k][i] = op(A)(i,k), Bs[k][j] = op(B)(k,j)
+  const int b = blockIdx.y;
+  const double* Ab = A + (long long)(b / divA) * sA;
+  const double* Bb = B + (long long)(b / divB) * sB;
+  double* Cb = C + (long long)b * sC;
+  const int m0 = (blockIdx.x % tiles_m) * BT, n0 = (blockIdx.x / tiles_m) * BT;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wi = (w & 1) * 16, wj = (w >> 1) * 16;
+  const int lr = lane & 15, lk = lane >> 4;
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    // 512 elements of each operand tile, two per thread, contiguous index fastest
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int e = t + 256 * r;
+      int i, k;
+      if (OPA == OP_N) { i = e % BT; k = e / BT; } else { k = e % BK; i = e / BK; }
+      const int gi = m0 + i, gk = k0 + k;
+      double v = 0.0;
+      if (gi < M && gk < K) v = (OPA == OP_N) ? Ab[gi + (long long)gk * lda] : Ab[gk + (long long)gi * lda];
+      As[k][i] = v;
+      int j, k2;
+      if (OPB == OP_N) { k2 = e % BK; j = e / BK; } else { j = e % BT; k2 = e / BT; }
+      const int gj = n0 + j, gk2 = k0 + k2;
+      double u = 0.0;
+      if (gj < N && gk2 < K) u = (OPB == OP_N) ? Bb[gk2 + (long long)gj * ldb] : Bb[gj + (long long)gk2 * ldb];
+      Bs[k2][j] = u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const double av = Bs[kk + lk][wj + lr];    // D rows: columns of C
+      const double bv = As[kk + lk][wi + lr];    // D columns: rows of C
+      acc = mfma(av, bv, acc);
+    }
+    __syncthreads();
+  }
+  // lane l, register r: C(m0 + wi + lr, n0 + wj + lk + 4 r)
+  const int gi = m0 + wi + lr;
+  if (gi < M) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gj = n0 + wj + lk + 4 * r;
+      if (gj < N) {
+        double* cp = Cb + gi + (long long)gj * ldc;
+        double v = alpha * acc[r];
+        if (use_beta) v = fma(beta, *cp, v);
+        *cp = v;
+      }
+    }
+  }
+}
+
 // Grouped contraction: C_g (M x s_g) = X_g (M x s_g) W_g (s_g x s_g), W real (float/double).
 template <typename TW, int BM, int BN, int WM, int WN, int OCC = 1>
 __global__ void __launch_bounds__(64 * WM * WN * OCC)
@@ -407,6 +473,25 @@ int mfma_gemm_grouped(int variant, hipStream_t s, int M, const cplx* X, cplx* C,
     tg_set_error("mfma grouped gemm: bad tile variant");
     return TG_E_ARG;
   }
+  TG_HIP(hipGetLastError());
+  return TG_OK;
+}
+
+int mfma_gemm_dd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, double alpha,
+                 const double* A, int lda, long long sA, const double* B, int ldb, long long sB,
+                 double beta, int use_beta, double* C, int ldc, long long sC, int batch, int divA,
+                 int divB) {
+  if (M <= 0 || N <= 0 || batch <= 0) return TG_OK;
+  const int tiles_m = (M + 31) / 32, tiles_n = (N + 31) / 32;
+  const dim3 grid(tiles_m * tiles_n, batch);
+#define TG_DD(OA, OB)                                                                              \
+  hipLaunchKernelGGL((mfma_dd_kernel<OA, OB>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, \
+                     B, ldb, sB, beta, use_beta, C, ldc, sC, tiles_m, divA, divB)
+  if (opa == OP_N && opb == OP_N) TG_DD(OP_N, OP_N);
+  else if (opa == OP_N) TG_DD(OP_N, OP_T);
+  else if (opb == OP_N) TG_DD(OP_T, OP_N);
+  else TG_DD(OP_T, OP_T);
+#undef TG_DD
   TG_HIP(hipGetLastError());
   return TG_OK;
 }

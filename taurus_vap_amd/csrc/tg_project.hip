@@ -186,6 +186,15 @@ skew_part_kernel(int n, const cplx* __restrict__ G, cplx* __restrict__ H) {
 // nb = 128 on, one band below.
 inline int q_band_rows(int nb) { return nb >= 128 ? (((nb + 2) / 3 + 15) / 16) * 16 : nb; }
 
+// Tile variant of the reduced-overlap row bands: the two-wave 64x32 tile (8) that
+// fits beside the 384-thread inverse for 256 < n <= 384, else tuned (-1);
+// TG_Q_TILE=k forces variant k (-1: tuned).
+int q_tile_variant(int n) {
+  const char* e = std::getenv("TG_Q_TILE");   // read per call: tests set it at run time
+  if (e) return std::atoi(e);
+  return (n > 256 && n <= 384) ? 8 : -1;
+}
+
 // Per-angle traces (calculate_particle_number :78-143, calculate_expectval_energy
 // module_hamiltonian.f90:1098-1126).  out[a*9 + k]:
 // 0 prot, 1 neut, 2 Tr(rho_p^2), 3 Tr(kRL_p^T kLR_p), 4 Tr(rho_n^2),
@@ -987,7 +996,10 @@ int project_upload(tg_ctx* c) {
     TG_TRY(gemm_tune_dz(s, OP_N, OP_N, n, n, n, U, n, 0, Xa, n, nn, Ya, n, nn, chunk));
     TG_TRY(gemm_tune_dz(s, OP_N, OP_T, n, n, n, U, n, 0, Xa, n, nn, Ya, n, nn, chunk));
     TG_TRY(gemm_tune_zd(s, OP_N, OP_T, n, n, n, Xa, n, nn, U, n, 0, Ya, n, nn, chunk));
-    if (nb0 > 0) {
+    // the reduced-overlap bands (not when their tile is forced: the variant
+    // beside the inverse at 256 < n <= 384, or TG_Q_TILE; nbloc follows the state,
+    // so a tuned band shape may be new in every iteration)
+    if (nb0 > 0 && q_tile_variant(n) < 0) {
       const int nb = nb0, rb = q_band_rows(nb);
       const long long mm = (long long)nb * nb;
       for (int r0 = 0; r0 < nb; r0 += rb)
@@ -1105,8 +1117,7 @@ int project_run(tg_ctx* c) {
           // leaves two SIMDs of each of its CUs half free, which the two-wave 64x32
           // tile (256 VGPRs, 26 KB LDS) fits; other sizes take the tuned tile.
           // TG_Q_TILE=k forces variant k (-1: tuned).
-          const char* qe = std::getenv("TG_Q_TILE");
-          const int qtile = qe ? std::atoi(qe) : ((n > 256 && n <= 384) ? 8 : -1);
+          const int qtile = q_tile_variant(n);
           for (int r0 = 0; r0 < nb; r0 += rb)
             TG_TRY(gemm_zz(s2, OP_T, OP_N, std::min(rb, nb - r0), nb - r0, nb, one,
                            Q + (size_t)r0 * nb, nb, mm, JQ + (size_t)r0 * nb, nb, mm, zero,
