@@ -1,12 +1,16 @@
 // m-scheme two-body matrix elements in the reference's storage convention.
 //
-// Restates, for JT-scheme (ANTOINE-type) interactions, the producer side of
-// the TBME list consumed by the contraction:
-//   * the m-scheme loop of set_hamiltonian_2body (src/module_hamiltonian.f90:
-//     330-378): one representative per symmetry class, a<b, c>=a, d>c (d<=b when
-//     c==a), M = m_a+m_b >= 0, parity / T_z / M conservation, cut |V| > 1e-16,
-//     stored in loop order;
-//   * uncouple_JT (:828-885): sum over J and T of N_ab^-1 N_cd^-1 CG CG CG CG V;
+// Restates the producer side of the TBME list consumed by the contraction:
+//   * the symmetry reconstruction of set_hamiltonian_2body (src/module_hamiltonian.
+//     f90:284-319): every stored coupled element, in the reference's loop order,
+//     copied to its seven index permutations with the exchange phases, the
+//     isospin channel of a J-scheme element following the permutation (find_iso,
+//     :911-959);
+//   * the m-scheme loop (:330-378): one representative per symmetry class, a<b,
+//     c>=a, d>c (d<=b when c==a), M = m_a+m_b >= 0, parity / T_z / M
+//     conservation, cut |V| > 1e-16, stored in loop order;
+//   * uncouple_JT (:828-885) for JT-scheme (ANTOINE, types 1-2) interactions and
+//     uncouple_J (:747-814) for J-scheme (types 3-4) ones;
 //   * reconstruct_2body_timerev (:970-1018): the int8 permutation/phase code of
 //     the time-reversed element.
 // Clebsch-Gordan coefficients are the Racah formula on log-gamma values,
@@ -88,32 +92,40 @@ struct CGTable {
   }
 };
 
-}  // namespace
 
-extern "C" {
-
-double th_clebsch_gordan(int j1, int j2, int j3, int m1, int m2, int m3) {
-  return cg_lgamma(j1, j2, j3, m1, m2, m3);
+// Isospin channel of a J-scheme element after an index permutation of the
+// symmetry reconstruction.  Channels: 0 = pp pp, 1 = pn pn, 2 = pn np,
+// 3 = np pn, 4 = np np, 5 = nn nn.  Permutations 1..7 = abdc, bacd, badc, cdab,
+// cdba, dcab, dcba: position k of the new element holds old position P[perm][k];
+// the new channel follows from which of the moved particles are protons.
+int iso_after_perm(int t, int perm) {
+  if (t == 0 || t == 5) return t;
+  static const int P[8][4] = {{0, 1, 2, 3}, {0, 1, 3, 2}, {1, 0, 2, 3}, {1, 0, 3, 2},
+                              {2, 3, 0, 1}, {2, 3, 1, 0}, {3, 2, 0, 1}, {3, 2, 1, 0}};
+  const int bra_np = (t == 3 || t == 4), ket_np = (t == 2 || t == 4);
+  const bool prot[4] = {!bra_np, (bool)bra_np, !ket_np, (bool)ket_np};
+  const int nbra = prot[P[perm][0]] ? 0 : 1, nket = prot[P[perm][2]] ? 0 : 1;
+  return 1 + 2 * nbra + nket;
 }
 
-int64_t th_mscheme(int n, const int32_t* sp_2j, const int32_t* sp_2mj, const int32_t* sp_2mt,
-                   const int32_t* sp_l, const int32_t* sp_sh, int nsh, int jmax,
-                   const double* Vcpd, double scaling_2b, int64_t capacity, int16_t* abcd,
-                   double* H2) {
-  if (n <= 0 || nsh <= 0 || jmax < 0 || (abcd && (!Vcpd || !H2))) return -1;
-  int jm2 = 0;
-  for (int i = 0; i < n; ++i) jm2 = sp_2j[i] > jm2 ? sp_2j[i] : jm2;
-  CGTable cg;
-  cg.build(jm2);
-  // isospin CG <1/2 t1 1/2 t2 | T MT>
-  auto cgt = [](int T, int t1, int t2) { return cg_racah(1, 1, 2 * T, t1, t2); };
+// Channel of the m-scheme element (ta, tb, tc, td) (2 t_z, protons -1) in the
+// J-scheme ordering; -1 when charge is not conserved.
+int jscheme_channel(int ta, int tb, int tc, int td) {
+  if (ta == -1 && tb == -1 && tc == -1 && td == -1) return 0;
+  if (ta == -1 && tb == 1 && tc == -1 && td == 1) return 1;
+  if (ta == -1 && tb == 1 && tc == 1 && td == -1) return 2;
+  if (ta == 1 && tb == -1 && tc == -1 && td == 1) return 3;
+  if (ta == 1 && tb == -1 && tc == 1 && td == -1) return 4;
+  if (ta == 1 && tb == 1 && tc == 1 && td == 1) return 5;
+  return -1;
+}
+
+// The m-scheme loop of set_hamiltonian_2body; `uncouple(a, b, c, d)` gives the
+// m-scheme element of 0-based single-particle states.
+template <class F>
+int64_t mscheme_loop(int n, const int32_t* sp_2mj, const int32_t* sp_2mt, const int32_t* sp_l,
+                     double scaling_2b, int64_t capacity, int16_t* abcd, double* H2, F uncouple) {
   const double Vcut = 1.0e-16;
-  auto V = [&](int mmt, int J, int a, int b, int c, int d) {
-    // hamil_H2cpd(0:3, 0:jmax, nsh, nsh, nsh, nsh), shells 1-based
-    return Vcpd[(size_t)mmt + 4 * ((size_t)J + (size_t)(jmax + 1) *
-                                   ((size_t)(a - 1) + nsh * ((size_t)(b - 1) + nsh *
-                                   ((size_t)(c - 1) + nsh * (size_t)(d - 1)))))];
-  };
   int64_t cnt = 0;
   for (int a = 0; a < n; ++a) {
     const int la = sp_l[a], ma = sp_2mj[a], ta = sp_2mt[a];
@@ -132,30 +144,7 @@ int64_t th_mscheme(int n, const int32_t* sp_2j, const int32_t* sp_2mj, const int
             ++cnt;
             continue;
           }
-          // uncouple_JT
-          const int ja = sp_2j[a], jb = sp_2j[b], jc = sp_2j[c], jd = sp_2j[d];
-          const int sha = sp_sh[a], shb = sp_sh[b], shc = sp_sh[c], shd = sp_sh[d];
-          const int MJ = ma + mb, MT = ta + tb;
-          const double dab = (sha == shb) ? 1.0 : 0.0, dcd = (shc == shd) ? 1.0 : 0.0;
-          double sum = 0.0;
-          const int J0 = abs(MJ) / 2, J1 = ((ja + jb) / 2 < (jc + jd) / 2) ? (ja + jb) / 2 : (jc + jd) / 2;
-          for (int J = J0; J <= J1; ++J) {
-            const double cg1 = cg.at(ja, jb, J, ma, mb);
-            const double cg2 = cg.at(jc, jd, J, mc, md);
-            const int T0 = abs(MT) / 2;
-            for (int T = T0; T <= 1; ++T) {
-              const int mmt = MT / 2 + 2 * T;
-              const double v = V(mmt, J, sha, shb, shc, shd);
-              if (fabs(v) < 1.0e-12) continue;
-              const double ph = ((J + T) & 1) ? -1.0 : 1.0;
-              const double nab = sqrt(1.0 - dab * ph) / (1.0 + dab);
-              const double ncd = sqrt(1.0 - dcd * ph) / (1.0 + dcd);
-              if (nab <= 1.0e-12 || ncd <= 1.0e-12) continue;
-              const double ninv = 1.0 / (nab * ncd);
-              sum += ninv * cg1 * cg2 * cgt(T, ta, tb) * cgt(T, tc, td) * v;
-            }
-          }
-          sum *= scaling_2b;
+          const double sum = uncouple(a, b, c, d) * scaling_2b;
           if (fabs(sum) > Vcut) {
             if (cnt >= capacity) return -2;
             abcd[4 * cnt] = (int16_t)(a + 1);
@@ -170,6 +159,146 @@ int64_t th_mscheme(int n, const int32_t* sp_2j, const int32_t* sp_2mj, const int
     }
   }
   return cnt;
+}
+
+}  // namespace
+
+extern "C" {
+
+double th_clebsch_gordan(int j1, int j2, int j3, int m1, int m2, int m3) {
+  return cg_lgamma(j1, j2, j3, m1, m2, m3);
+}
+
+int64_t th_mscheme(int n, const int32_t* sp_2j, const int32_t* sp_2mj, const int32_t* sp_2mt,
+                   const int32_t* sp_l, const int32_t* sp_sh, int nsh, int jmax,
+                   const double* Vcpd, double scaling_2b, int64_t capacity, int16_t* abcd,
+                   double* H2) {
+  if (n <= 0 || nsh <= 0 || jmax < 0 || (abcd && (!Vcpd || !H2))) return -1;
+  int jm2 = 0;
+  for (int i = 0; i < n; ++i) jm2 = sp_2j[i] > jm2 ? sp_2j[i] : jm2;
+  CGTable cg;
+  if (abcd) cg.build(jm2);
+  // isospin CG <1/2 t1 1/2 t2 | T MT>
+  auto cgt = [](int T, int t1, int t2) { return cg_racah(1, 1, 2 * T, t1, t2); };
+  auto V = [&](int mmt, int J, int a, int b, int c, int d) {
+    // hamil_H2cpd(0:3, 0:jmax, nsh, nsh, nsh, nsh), shells 1-based
+    return Vcpd[(size_t)mmt + 4 * ((size_t)J + (size_t)(jmax + 1) *
+                                   ((size_t)(a - 1) + nsh * ((size_t)(b - 1) + nsh *
+                                   ((size_t)(c - 1) + nsh * (size_t)(d - 1)))))];
+  };
+  // uncouple_JT
+  auto uncouple = [&](int a, int b, int c, int d) {
+    const int ja = sp_2j[a], jb = sp_2j[b], jc = sp_2j[c], jd = sp_2j[d];
+    const int ma = sp_2mj[a], mb = sp_2mj[b], mc = sp_2mj[c], md = sp_2mj[d];
+    const int ta = sp_2mt[a], tb = sp_2mt[b], tc = sp_2mt[c], td = sp_2mt[d];
+    const int sha = sp_sh[a], shb = sp_sh[b], shc = sp_sh[c], shd = sp_sh[d];
+    const int MJ = ma + mb, MT = ta + tb;
+    const double dab = (sha == shb) ? 1.0 : 0.0, dcd = (shc == shd) ? 1.0 : 0.0;
+    double sum = 0.0;
+    const int J0 = abs(MJ) / 2, J1 = ((ja + jb) / 2 < (jc + jd) / 2) ? (ja + jb) / 2 : (jc + jd) / 2;
+    for (int J = J0; J <= J1; ++J) {
+      const double cg1 = cg.at(ja, jb, J, ma, mb);
+      const double cg2 = cg.at(jc, jd, J, mc, md);
+      for (int T = abs(MT) / 2; T <= 1; ++T) {
+        const int mmt = MT / 2 + 2 * T;
+        const double v = V(mmt, J, sha, shb, shc, shd);
+        if (fabs(v) < 1.0e-12) continue;
+        const double ph = ((J + T) & 1) ? -1.0 : 1.0;
+        const double nab = sqrt(1.0 - dab * ph) / (1.0 + dab);
+        const double ncd = sqrt(1.0 - dcd * ph) / (1.0 + dcd);
+        if (nab <= 1.0e-12 || ncd <= 1.0e-12) continue;
+        const double ninv = 1.0 / (nab * ncd);
+        sum += ninv * cg1 * cg2 * cgt(T, ta, tb) * cgt(T, tc, td) * v;
+      }
+    }
+    return sum;
+  };
+  return mscheme_loop(n, sp_2mj, sp_2mt, sp_l, scaling_2b, capacity, abcd, H2, uncouple);
+}
+
+int64_t th_mscheme_j(int n, const int32_t* sp_2j, const int32_t* sp_2mj, const int32_t* sp_2mt,
+                     const int32_t* sp_l, const int32_t* sp_sh, int nsh, int jmax,
+                     const double* Vcpd, double scaling_2b, int64_t capacity, int16_t* abcd,
+                     double* H2) {
+  if (n <= 0 || nsh <= 0 || jmax < 0 || (abcd && (!Vcpd || !H2))) return -1;
+  int jm2 = 0;
+  for (int i = 0; i < n; ++i) jm2 = sp_2j[i] > jm2 ? sp_2j[i] : jm2;
+  CGTable cg;
+  if (abcd) cg.build(jm2);
+  auto V = [&](int t, int J, int a, int b, int c, int d) {
+    // hamil_H2cpd(0:5, 0:jmax, nsh, nsh, nsh, nsh), shells 1-based
+    return Vcpd[(size_t)t + 6 * ((size_t)J + (size_t)(jmax + 1) *
+                                 ((size_t)(a - 1) + nsh * ((size_t)(b - 1) + nsh *
+                                 ((size_t)(c - 1) + nsh * (size_t)(d - 1)))))];
+  };
+  bool bad = false;
+  // uncouple_J
+  auto uncouple = [&](int a, int b, int c, int d) {
+    const int ja = sp_2j[a], jb = sp_2j[b], jc = sp_2j[c], jd = sp_2j[d];
+    const int ma = sp_2mj[a], mb = sp_2mj[b], mc = sp_2mj[c], md = sp_2mj[d];
+    const int sha = sp_sh[a], shb = sp_sh[b], shc = sp_sh[c], shd = sp_sh[d];
+    const int T = jscheme_channel(sp_2mt[a], sp_2mt[b], sp_2mt[c], sp_2mt[d]);
+    if (T < 0) {
+      bad = true;
+      return 0.0;
+    }
+    double dab = 0.0, dcd = 0.0;
+    if (T == 0 || T == 5) {
+      dab = (sha == shb) ? 1.0 : 0.0;
+      dcd = (shc == shd) ? 1.0 : 0.0;
+    }
+    double sum = 0.0;
+    const int J0 = abs(ma + mb) / 2, J1 = ((ja + jb) / 2 < (jc + jd) / 2) ? (ja + jb) / 2 : (jc + jd) / 2;
+    for (int J = J0; J <= J1; ++J) {
+      const double v = V(T, J, sha, shb, shc, shd);
+      if (fabs(v) < 1.0e-12) continue;
+      const double ph = (J & 1) ? 1.0 : -1.0;   // (-1)^(J+1)
+      const double nab = sqrt(1.0 - dab * ph) / (1.0 + dab);
+      const double ncd = sqrt(1.0 - dcd * ph) / (1.0 + dcd);
+      if (nab <= 1.0e-12 || ncd <= 1.0e-12) continue;
+      const double ninv = 1.0 / (nab * ncd);
+      sum += ninv * cg.at(ja, jb, J, ma, mb) * cg.at(jc, jd, J, mc, md) * v;
+    }
+    return sum;
+  };
+  const int64_t cnt = mscheme_loop(n, sp_2mj, sp_2mt, sp_l, scaling_2b, capacity, abcd, H2, uncouple);
+  return bad ? -1 : cnt;
+}
+
+int th_reconstruct_2body(int htype, int nsh, int jmax, const int32_t* sh_2j, double* Vcpd) {
+  if (nsh <= 0 || jmax < 0 || !sh_2j || !Vcpd || htype < 1 || htype > 4) return -1;
+  const bool jt = htype <= 2;
+  const int nch = jt ? 4 : 6;
+  auto at = [&](int t, int J, int a, int b, int c, int d) -> double& {
+    return Vcpd[(size_t)t + nch * ((size_t)J + (size_t)(jmax + 1) *
+                                  ((size_t)a + nsh * ((size_t)b + nsh * ((size_t)c + nsh * (size_t)d))))];
+  };
+  // every element, in the reference's loop order (a outermost, the channel
+  // innermost), is copied to its seven images; an image written here is read
+  // again when the loop reaches it
+  for (int a = 0; a < nsh; ++a)
+    for (int b = 0; b < nsh; ++b)
+      for (int c = 0; c < nsh; ++c)
+        for (int d = 0; d < nsh; ++d)
+          for (int J = 0; J <= jmax; ++J)
+            for (int t = 0; t < nch; ++t) {
+              const double v = at(t, J, a, b, c, d);
+              if (!(fabs(v) > 1.0e-12)) continue;
+              // (-1)^(ja + jb + J + x): x = 0 for the T = 0 channel of a JT
+              // interaction, else 1
+              const int x = (jt && t == 0) ? 0 : 1;
+              const double pab = (((sh_2j[a] + sh_2j[b]) / 2 + J + x) & 1) ? -1.0 : 1.0;
+              const double pcd = (((sh_2j[c] + sh_2j[d]) / 2 + J + x) & 1) ? -1.0 : 1.0;
+              auto ch = [&](int perm) { return jt ? t : iso_after_perm(t, perm); };
+              at(ch(1), J, a, b, d, c) = v * pcd;
+              at(ch(2), J, b, a, c, d) = v * pab;
+              at(ch(3), J, b, a, d, c) = v * pab * pcd;
+              at(ch(4), J, c, d, a, b) = v;
+              at(ch(5), J, c, d, b, a) = v * pab;
+              at(ch(6), J, d, c, a, b) = v * pcd;
+              at(ch(7), J, d, c, b, a) = v * pab * pcd;
+            }
+  return 0;
 }
 
 int th_timerev(int n, const int32_t* sp_2j, const int32_t* sp_tr, int64_t ntb,

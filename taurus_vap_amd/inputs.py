@@ -7,6 +7,10 @@ and the reference CPU build:
   ``read_hamiltonian_1body_antoine`` / ``read_hamiltonian_2body_antoine``
   (src/module_hamiltonian.f90:150-193, 409-456) and ``set_basis``
   (src/module_basis.f90:73-189).
+* J-scheme Hamiltonians (hamil_type 3 and 4): ``<name>.sho``, ``.01b``, ``.2b`` and
+  ``.com``, read by ``read_hamiltonian_1body_general`` and
+  ``read_hamiltonian_2body_Jscheme`` / ``_com`` (src/module_hamiltonian.f90:
+  200-239, 463-525).
 * The fixed-format STDIN deck read by ``read_input``
   (src/module_initialization.f90:86-155).
 * ``initial_wf.txt`` in the text layout of ``read_wavefunction``
@@ -174,6 +178,107 @@ def write_antoine(path: str, name: str, shells, spe, tbme, core=(8, 8), opt_hw=N
 
 
 # --------------------------------------------------------------------------
+# J-scheme interactions (hamil_type 3 and 4)
+# --------------------------------------------------------------------------
+
+def jscheme_name(sh: tuple[int, int, int]) -> int:
+    """Shell label of the J-scheme files: 10000 n + 100 l + 2j (set_basis,
+    src/module_basis.f90:164-175)."""
+    n, l, j2 = sh
+    return 10000 * n + 100 * l + j2
+
+
+def synth_j_tbme(shells, seed: int, sigma: float = 1.0, decay: float | None = None) -> list[tuple]:
+    """Random J-scheme matrix elements for one representative per symmetry class
+    (a<=b, c<=d, (a,b)<=(c,d), 1-based shells), parity conserving, independently
+    drawn in the six channels of the J-scheme files (0 pp pp, 1 pn pn, 2 pn np,
+    3 np pn, 4 np np, 5 nn nn); the pp and nn channels are zero for J odd when
+    a==b or c==d.  Where a degenerate class (a==b, c==d or (a,b)==(c,d)) maps one
+    channel onto another, the symmetry reconstruction of the reference decides
+    which value stays (src/module_hamiltonian.f90:284-319).
+    Returns (a, b, c, d, J, [v_0 .. v_5])."""
+    rng = np.random.default_rng(seed)
+    ns = len(shells)
+    pairs = [(a, b) for a in range(ns) for b in range(a, ns)]
+    out = []
+    for i1, (a, b) in enumerate(pairs):
+        for (c, d) in pairs[i1:]:
+            la, lb, lc, ld = (shells[x][1] for x in (a, b, c, d))
+            if (la + lb) % 2 != (lc + ld) % 2:
+                continue
+            ja, jb, jc, jd = (shells[x][2] for x in (a, b, c, d))
+            jmin = max(abs(ja - jb), abs(jc - jd)) // 2
+            jmax = min(ja + jb, jc + jd) // 2
+            if jmin > jmax:
+                continue
+            fac = 1.0
+            if decay is not None:
+                Ns = sum(2 * shells[x][0] + shells[x][1] for x in (a, b, c, d))
+                fac = math.exp(-Ns / decay)
+            for J in range(jmin, jmax + 1):
+                v = [float("%.10f" % (rng.normal(0.0, sigma) * fac)) for _ in range(6)]
+                if (a == b or c == d) and J % 2 == 1:
+                    v[0] = v[5] = 0.0
+                out.append((a + 1, b + 1, c + 1, d + 1, J, v))
+    return out
+
+
+def write_jscheme(stem: str, name: str, shells, htype: int, core=(0, 0), opt_hw: int = 2,
+                  hw: float = 20.0, H0: float = 0.0, h1=None, tbme=(), com_tbme=None):
+    """``<stem>.sho`` and ``<stem>.2b`` of a J-scheme interaction, with ``<stem>.01b``
+    for type 3 and ``<stem>.com`` when com_tbme is given (set_basis,
+    src/module_basis.f90:79-124; read_hamiltonian_1body_general, _2body_Jscheme,
+    _2body_com, src/module_hamiltonian.f90:200-239, 463-525).
+    h1: {(a, b): (e_p, e_n)} over 1-based shells; tbme / com_tbme as synth_j_tbme."""
+    with open(stem + ".sho", "w") as f:
+        f.write(" %s\n %d\n" % (name, htype))
+        f.write(" %d " % len(shells) + " ".join(str(jscheme_name(s)) for s in shells) + "\n")
+        f.write(" %d %d\n" % core)
+        f.write(" %d %.6f\n" % (opt_hw, hw))
+    if htype == 3:
+        with open(stem + ".01b", "w") as f:
+            f.write(" %s\n %.10f\n" % (name, H0))
+            for (a, b), (ep, en) in (h1 or {}).items():
+                f.write(" %d %d %.10f %.10f\n" % (jscheme_name(shells[a - 1]),
+                                                 jscheme_name(shells[b - 1]), ep, en))
+
+    def blocks(path, rows_in):
+        by_quad: dict = {}
+        for (a, b, c, d, J, vals) in rows_in:
+            by_quad.setdefault((a, b, c, d), []).append((J, vals))
+        with open(path, "w") as f:
+            f.write(" %s\n" % name)
+            for (a, b, c, d), rows in by_quad.items():
+                names = [jscheme_name(shells[x - 1]) for x in (a, b, c, d)]
+                f.write(" 0 5 %d %d %d %d %d %d\n" % (*names, rows[0][0], rows[-1][0]))
+                for J, vals in rows:
+                    f.write(" " + " ".join("%.10f" % v for v in vals) + "\n")
+
+    blocks(stem + ".2b", tbme)
+    if com_tbme is not None:
+        blocks(stem + ".com", com_tbme)
+
+
+def write_jscheme_synthetic(stem: str, htype: int, com: int, seed: int = 11, emax: int = 2):
+    """The seeded J-scheme interaction of the J-scheme tests on the e_max space:
+    random couplings of synth_j_tbme (sigma 1, decay 4), for type 3 a .01b file with
+    H0 = 1.5, hw (N + 3/2) diagonals shifted per shell and one 0s1/2 - 1s1/2
+    coupling; hw = 20 (type 3) or 18 (type 4); with com a .com file of width 0.1."""
+    shells = ho_shells(emax)
+    tb = synth_j_tbme(shells, seed=seed, sigma=1.0, decay=4.0)
+    com_tb = synth_j_tbme(shells, seed=seed + 1, sigma=0.1) if com else None
+    h1 = {(i + 1, i + 1): (20.0 * (2 * sh[0] + sh[1] + 1.5) + 0.1 * i,
+                           20.0 * (2 * sh[0] + sh[1] + 1.5) - 0.05 * i)
+          for i, sh in enumerate(shells)}
+    s1 = [i + 1 for i, sh in enumerate(shells) if sh[1] == 0 and sh[2] == 1]
+    if len(s1) >= 2:
+        h1[(s1[0], s1[1])] = (0.7, -0.4)
+    write_jscheme(stem, "synthetic J-scheme type %d seed %d" % (htype, seed), shells, htype,
+                  hw=18.0 if htype == 4 else 20.0, H0=1.5, h1=h1, tbme=tb, com_tbme=com_tb)
+    return shells
+
+
+# --------------------------------------------------------------------------
 # STDIN deck (fixed format, src/module_initialization.f90:93-155)
 # --------------------------------------------------------------------------
 
@@ -195,7 +300,7 @@ def _l30(s: str) -> str:
 
 
 def write_deck(path: str, hamil: str, Z: float, N: float, Mphip: int, Mphin: int, *,
-               hamil_read: int = 0, seed_type: int = 1, seed_text: int = 1,
+               hamil_read: int = 0, hamil_com: int = 0, seed_type: int = 1, seed_text: int = 1,
                seed_symm: int = 0, iter_max: int = 100, iter_print: int = 0,
                gradient_type: int = 0, eta: float = 0.01, mu: float = 0.0,
                grad_eps: float = 1e-4, enforce_NZ: int = 1, opt_betalm: int = 1,
@@ -205,7 +310,7 @@ def write_deck(path: str, hamil: str, Z: float, N: float, Mphip: int, Mphin: int
     constraints = constraints or {}
     L = []
     L += ["Interaction", "-----------", _l30("Master name hamil. files") + hamil,
-          _l30("Center-of-mass correction") + "0", _l30("Read reduced hamiltonian") + str(hamil_read),
+          _l30("Center-of-mass correction") + str(hamil_com), _l30("Read reduced hamiltonian") + str(hamil_read),
           _l30("No. of MPI proc per H team") + "    0", "",
           "Particle Number", "---------------",
           _l30("Number of active protons") + "%7.2f" % Z, _l30("Number of active neutrons") + "%7.2f" % N,
@@ -284,12 +389,14 @@ def seed_wavefunction(basis: Basis, Z: int, N: int, seed: int, sigma: float = 0.
     return thouless_evolve(U0, V0, Zt)
 
 
-def write_wavefunction_text(path: str, shells, U, V, label: int = 0):
-    n = U.shape[0]
+def write_wavefunction_text(path: str, shells, U, V, label: int = 0, jscheme: bool = False):
+    """Shells are labelled as the interaction labels them: the ANTOINE
+    1000 n + 100 l + 2j, or 10000 n + 100 l + 2j for a J-scheme interaction."""
+    name = jscheme_name if jscheme else antoine_name
     with open(path, "w") as f:
         f.write("%12d\n" % len(shells))
         for s in shells:
-            f.write("%12d\n" % antoine_name(s))
+            f.write("%12d\n" % name(s))
         f.write("%22d\n" % label)
         for M in (U, V):
             for v in np.asarray(M, dtype=np.float64).flatten(order="F"):

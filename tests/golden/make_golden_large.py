@@ -9,7 +9,11 @@ which dumps only what stays small at large bases:
 * emax4_11.npz -- e_max = 4 (140 states), 11x11 angles, 8p/8n, 2 iterations, FP64;
 * a19_<case>.npz -- small decks with gradient_type 1 or 2: the h and H11 matrices
                   diagonalize_hsp_and_H11 hands to dsyev, their eigenvalues and the
-                  recipe steps (src/module_gradient.f90:79-453).
+                  recipe steps (src/module_gradient.f90:79-453);
+* jscheme_<case>.npz -- J-scheme interactions (hamil_type 3 with a .01b file,
+                  hamil_type 4 with the centre-of-mass correction), e_max = 2: the
+                  reference's reduced list, H0 and H1, and the projection of the
+                  first iterations.
 
 In every case the reference builds its own Hamiltonian from the .sho file
 (hamil_read = 0) and writes its reduced file; the fixture keeps the SHA-256 of
@@ -169,6 +173,32 @@ def small_a19_case(wname: str, M: int, kmax: int, gtype: int, eta: float = 0.0,
         return out
 
 
+def jscheme_case(htype: int, com: int, Z: int, N: int, M: int, kmax: int,
+                 seed: int = 11) -> dict:
+    """Seeded J-scheme interaction on the e_max = 2 space (40 states), written by
+    inputs.write_jscheme_synthetic; the reference builds it (hamil_read = 0) and runs kmax
+    iterations at M x M gauge angles."""
+    name = "jscheme%d" % htype
+    with tempfile.TemporaryDirectory() as td:
+        shells = I.write_jscheme_synthetic(os.path.join(td, name), htype, com, seed)
+        I.write_deck(os.path.join(td, "input.txt"), name, Z, N, M, M, hamil_read=0, hamil_com=com,
+                     iter_max=kmax, grad_eps=1e-12)
+        basis = I.Basis(shells)
+        U, V = I.seed_wavefunction(basis, Z, N, seed=seed + 2, sigma=0.1)
+        I.write_wavefunction_text(os.path.join(td, "initial_wf.txt"), shells, U, V, jscheme=True)
+        d = run_proj(td, 64, kmax, 1)
+        n = int(one(d, "ndim"))
+        h = P.read_red(os.path.join(td, name + ".red"), n, 8)
+        sha, ntb = red_payload_sha256(os.path.join(td, name + ".red"), n, 8)
+        out = pack(d)
+        out.update(red_sha256_64=np.asarray(sha), ntb=np.asarray(ntb), htype=np.asarray(htype),
+                   hamil_com=np.asarray(com), seed=np.asarray(seed),
+                   abcd=(h["abcd"] + 1).astype(np.int16), H2=h["H2"],
+                   trperm=h["trperm"].astype(np.int8), H0=np.asarray(h["H0"]), H1=h["H1"],
+                   prec=np.asarray(64))
+        return out
+
+
 CASES = {
     # configs[4]: e_max = 6, 15x15, Q20 + Jx, FP64 and FP32 TBMEs, 2 iterations
     "emax6": lambda: workload_case(I.WORKLOADS["emax6"], 2, precs=(64, 32), threads=8),
@@ -181,6 +211,10 @@ CASES = {
     "a19_pf_g2": lambda: small_a19_case("pf", 9, 3, 2),
     # one gauge angle: the h/Delta of the M = 1 projection feed H11 directly
     "a19_sd_m1_g1": lambda: small_a19_case("sd", 1, 3, 1),
+    # J-scheme interactions: type 3 (.01b one-body file), 3p/4n, and type 4 (kinetic
+    # one-body part) with the centre-of-mass correction (.com), 4p/4n; 5x5 angles
+    "jscheme_t3": lambda: jscheme_case(3, 0, 3, 4, 5, 2),
+    "jscheme_t4_com": lambda: jscheme_case(4, 1, 4, 4, 5, 2),
 }
 
 
