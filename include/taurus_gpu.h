@@ -323,6 +323,16 @@ int tg_diagonalize_hsp_and_H11(tg_ctx* ctx, int constraint_dim, const double* la
 int tg_sym_eigenvalues(int n, int count, const double* A, int lda, double* w);
 
 /*
+ * tg_sym_eigensystem -- eigenvalues (ascending) and orthonormal eigenvectors of
+ * count real symmetric n x n matrices (upper triangles read; host buffers: A
+ * count x lda x n, w count x n, Z count x n x n, column-major), as DSYEV('V','U')
+ * returns them for the canonical basis (construct_canonical_basis,
+ * src/module_wavefunctions.f90:806): Householder tridiagonalization, divide and
+ * conquer, back-transformation, all on the device.  n <= 1024.
+ */
+int tg_sym_eigensystem(int n, int count, const double* A, int lda, double* w, double* Z);
+
+/*
  * tg_get_state -- bogo_U0, bogo_V0 of the device states (column-major
  * real[ndim*ndim] each, state k at + k*ndim*ndim, rows in the sp order given to
  * tg_init).

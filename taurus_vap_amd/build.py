@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtaurus_gpu.so")
 SOURCES = ["tg_gemm.hip", "tg_mfma.hip", "tg_linalg.hip", "tg_fields.hip", "tg_project.hip",
-           "tg_iterate.hip", "tg_canon.hip", "tg_a19.hip", "tg_noalloc.hip"]
+           "tg_iterate.hip", "tg_canon.hip", "tg_a19.hip", "tg_noalloc.hip", "tg_eig.hip"]
 ARCH = os.environ.get("TAURUS_GPU_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -43,9 +43,8 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write("---- %s\n%s\n" % (src, out))
         raise RuntimeError("hipcc failed for " + ", ".join(s for s, _ in failed))
     tmp = LIB + ".tmp"
-    # rocSOLVER (dsyevd of the device canonical basis) and its rocBLAS
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=%s" % ARCH, *objs, "-o", tmp,
-           "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+    # the HIP runtime is the only library linked
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=%s" % ARCH, *objs, "-o", tmp]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     for o in objs:

@@ -3,8 +3,10 @@
 //
 // For every state s of the context (real U, V in the internal single-particle
 // order):
-//   rho = V V^T, D = eigenvectors of rho (ascending eigenvalues; rocSOLVER
-//   dsyevd, batched over the states), rhoc = D^T rho D,
+//   rho = V V^T, D = eigenvectors of rho (ascending eigenvalues, upper triangle
+//   read as DSYEV('V','U') of :806; batched_sym_eigensystem: Householder
+//   tridiagonalization, divide and conquer, back-transformation, all on the
+//   device), rhoc = D^T rho D,
 //   v2_i = |rhoc_ii|: nocc counts v2 >= 1 - eps, nemp v2 <= eps (the
 //   reference's seed_allemp = 0), ovac = sum log sqrt|1 - v2| over the rest,
 //   kappac = D^T (V U^T) D, and the degenerate-subspace analysis of kappac
@@ -20,9 +22,6 @@
 // dense problems solved on the host, as the dim x dim multiplier systems of the
 // device iteration are.
 #include "tg_internal.h"
-
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -84,17 +83,6 @@ __global__ void block_rotate_kernel(int n, int k0, int m, double* __restrict__ D
     for (int q = 0; q < m; ++q) acc = fma(d[q], T[q + c * m], acc);
     D[r + (long long)(k0 + c) * n] = acc;
   }
-}
-
-rocblas_handle handle_of(tg_ctx* c) {
-  if (!c->rb) {
-    rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
-    c->rb = h;
-  }
-  rocblas_handle h = static_cast<rocblas_handle>(c->rb);
-  if (rocblas_set_stream(h, c->stream) != rocblas_status_success) return nullptr;
-  return h;
 }
 
 // Subspaces of the reference's analysis (:851-873), 0-based, on the columns
@@ -210,11 +198,6 @@ bool skew_canonical(int m, const std::vector<double>& K, std::vector<double>& T)
 
 }  // namespace
 
-void tg_canon_release(tg_ctx* c) {
-  if (c->rb) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(c->rb));
-  c->rb = nullptr;
-}
-
 extern "C" int tg_canonical_basis(tg_ctx* c, double eps, int* nocc_out, int* nemp_out,
                                   double* ovac_out, double* voveru_out) {
   if (!c || !c->d_U.p || !c->d_V.p || c->ns < 1 || c->Mphip < 1 || !(eps > 0.0 && eps < 0.5)) {
@@ -223,15 +206,17 @@ extern "C" int tg_canonical_basis(tg_ctx* c, double eps, int* nocc_out, int* nem
   }
   const int n = c->n, S = c->ns;
   const long long nn = (long long)n * n;
-  // work: R (rho, then rho D), E (eigenvectors D), T1, T2, K (kappac), W, F, diag,
-  // kd, then ints lohi, reach, info, then the block transformations
+  // work: R (rho, then rho D), E (eigenvectors D), T1, T2, K (kappac), W, diag,
+  // kd, then ints lohi, reach, info, then the block transformations, then the
+  // eigen-solver's workspace
   const size_t mats = (size_t)5 * S * nn;
-  const size_t vecs = (size_t)3 * S * n;
+  const size_t vecs = (size_t)2 * S * n;
   const size_t ints = 2 * (size_t)S + (size_t)S * n + (size_t)S;
   const size_t base = sizeof(double) * (mats + vecs + 2 * (size_t)S * n) +
                       sizeof(int) * (ints + 2);   // pad: the block area stays 8-byte aligned
   const size_t tmax = (size_t)n * MAX_BLOCK;     // block transformations of one state
-  TG_TRY(c->d_canon.ensure(base + 8 + sizeof(double) * tmax));
+  const size_t eoff = ((base + 8 + sizeof(double) * tmax + 255) / 256) * 256;
+  TG_TRY(c->d_canon.ensure(eoff + sym_eigensystem_work_bytes(n, S)));
   TG_TRY(c->d_D.ensure(sizeof(double) * nn * S));
   double* R = c->d_canon.as<double>();
   double* E = R + S * nn;
@@ -239,8 +224,7 @@ extern "C" int tg_canonical_basis(tg_ctx* c, double eps, int* nocc_out, int* nem
   double* T2 = T1 + S * nn;
   double* K = T2 + S * nn;
   double* W = K + S * nn;             // eigenvalues
-  double* F = W + (size_t)S * n;      // dsyevd off-diagonal workspace
-  double* dg = F + (size_t)S * n;     // diag(D^T rho D)
+  double* dg = W + (size_t)S * n;     // diag(D^T rho D)
   double* kd = dg + (size_t)S * n;    // partner entries, 2 per column
   int* lohi = reinterpret_cast<int*>(kd + 2 * (size_t)S * n);
   int* reach = lohi + 2 * S;
@@ -251,19 +235,10 @@ extern "C" int tg_canonical_basis(tg_ctx* c, double eps, int* nocc_out, int* nem
   const double* V = c->d_V.as<double>();
   hipStream_t s = c->stream;
 
-  // rho = V V^T; D from dsyevd on a copy (upper triangle, as dsyev('V','U'))
+  // rho = V V^T; D its eigenvectors (upper triangle read, as dsyev('V','U'))
   TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, V, n, nn, V, n, nn, 0.0, R, n, nn, S));
-  TG_HIP(hipMemcpyAsync(E, R, sizeof(double) * nn * S, hipMemcpyDeviceToDevice, s));
-  rocblas_handle h = handle_of(c);
-  if (!h) {
-    tg_set_error("tg_canonical_basis: rocBLAS handle");
-    return TG_E_HIP;
-  }
-  if (rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n, E, n, nn,
-                                       W, n, F, n, info, S) != rocblas_status_success) {
-    tg_set_error("tg_canonical_basis: rocsolver_dsyevd failed");
-    return TG_E_HIP;
-  }
+  TG_TRY(batched_sym_eigensystem(s, n, R, n, nn, S, static_cast<char*>(c->d_canon.p) + eoff, W, n, E,
+                                 nn, info));
   // rhoc diagonal, kappac = D^T (V U^T) D
   TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, R, n, nn, E, n, nn, 0.0, T1, n, nn, S));
   hipLaunchKernelGGL(diag_kernel, dim3(nblk(n), S), dim3(NT), 0, s, n, nn, E, T1, dg);
@@ -279,7 +254,7 @@ extern "C" int tg_canonical_basis(tg_ctx* c, double eps, int* nocc_out, int* nem
   TG_HIP(hipStreamSynchronize(s));
   for (int k = 0; k < S; ++k)
     if (hinfo[k] != 0) {
-      tg_set_error("tg_canonical_basis: dsyevd did not converge (state " + std::to_string(k) + ")");
+      tg_set_error("tg_canonical_basis: eigen-solver did not converge (state " + std::to_string(k) + ")");
       return TG_E_SINGULAR;
     }
   std::vector<int> nocc(S, 0), nemp(S, 0), hlohi(2 * S);

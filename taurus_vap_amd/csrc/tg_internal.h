@@ -273,9 +273,8 @@ struct tg_ctx {
   DevBuf d_obs;                                   // observables of the final projection
   DevBuf d_a19;                                   // h, H11 and their eigenvalues
   std::vector<double> lambda_prev;                // multipliers of the previous iteration (ns x dim)
-  // canonical basis on the device (tg_canon.hip): work buffers, rocBLAS handle
+  // canonical basis on the device (tg_canon.hip): work buffers
   DevBuf d_canon;
-  void* rb = nullptr;
   // timing: event pairs recorded without synchronisation, summed at collection
   struct Pending { std::string key; hipEvent_t e0, e1; };
   int timing = 0;
@@ -320,13 +319,19 @@ void gradient_recipe(int type, int n, const double* eig_hsp, const double* eig_h
                      double mu0, double* eta, double* mu);
 // eigenvalues (ascending) of count symmetric matrices A + b sA (upper triangle
 // read) to w + b ws; work: count n^2 doubles for orders above 112
+// Eigenvalues (ascending, stride ws) and eigenvectors (n x n, stride sE) of count
+// symmetric matrices (upper triangles read), n <= 1024; work of
+// sym_eigensystem_work_bytes(n, count) bytes; info[k] != 0 when a secular root of
+// matrix k did not converge (tg_eig.hip).
+size_t sym_eigensystem_work_bytes(int n, int count);
+int batched_sym_eigensystem(hipStream_t s, int n, const double* A, int lda, long long sA, int count,
+                            void* work, double* w, long long ws, double* E, long long sE, int* info);
 int batched_sym_eigvals(hipStream_t s, int n, const double* A, int lda, long long sA, int count,
                         double* work, double* w, long long ws);
 // The resident U, V, D and c->st of c->ns states are set: the tail of
 // tg_set_states (Z buffers, gauge grid, state marked for the set-up upload).
 int tg_states_ready(tg_ctx* c);
 // releases the rocBLAS handle of the device canonical basis
-void tg_canon_release(tg_ctx* c);
 
 int contraction_build(tg_ctx* c, int64_t ntb, const int16_t* abcd, const void* H2,
                       int h2_bytes, const int8_t* trperm);

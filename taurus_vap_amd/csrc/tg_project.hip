@@ -1688,7 +1688,6 @@ int tg_finalize(tg_ctx* c) {
                     &c->d_blk, &c->d_scal, &c->d_out, &c->d_ovl, &c->d_int, &c->d_pfw, &c->d_c20,
                     &c->d_Z, &c->d_psign, &c->d_cHO, &c->d_it, &c->d_obs, &c->d_canon})
     b->release();
-  tg_canon_release(c);
   for (Channel* ch : {&c->ph, &c->pp})
     for (DevBuf* b : {&ch->d_inv, &ch->d_groups, &ch->d_W, &ch->d_tile_group, &ch->d_tile_n0})
       b->release();

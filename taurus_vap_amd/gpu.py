@@ -41,7 +41,7 @@ EXPORTS = [
     "tg_group_iterate", "tg_group_iterate_results", "tg_project_observables",
     "tg_set_batch_limit", "tg_canonical_basis", "tg_set_gradient_type",
     "tg_get_hsp_h11_eigenvalues", "tg_diagonalize_hsp_and_H11", "tg_sym_eigenvalues",
-    "tg_zskpfa_noalloc", "tg_inverse",
+    "tg_zskpfa_noalloc", "tg_inverse", "tg_sym_eigensystem",
 ]
 
 _lib = None
@@ -101,6 +101,7 @@ def load_library():
     lib.tg_get_hsp_h11_eigenvalues.argtypes = [P, I, P, P, P, P]
     lib.tg_diagonalize_hsp_and_H11.argtypes = [P, I, P, P, P, P, P, P, P]
     lib.tg_sym_eigenvalues.argtypes = [I, I, P, I, P]
+    lib.tg_sym_eigensystem.argtypes = [I, I, P, I, P, P]
     _lib = lib
     return lib
 
@@ -524,6 +525,25 @@ def sym_eigenvalues(A):
            "tg_sym_eigenvalues")
     w = w.reshape(cnt, n)
     return w[0] if single else w
+
+
+def sym_eigensystem(A):
+    """Eigenvalues (ascending) and orthonormal eigenvectors (columns) of one real
+    symmetric matrix or a stack (count, n, n), upper triangles read
+    (tg_sym_eigensystem).  Returns (w, Z)."""
+    A = np.asarray(A, dtype=np.float64)
+    single = A.ndim == 2
+    if single:
+        A = A[None]
+    cnt, n, _ = A.shape
+    buf = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))
+    w = np.zeros(cnt * n)
+    Z = np.zeros((cnt, n, n))
+    _check(load_library().tg_sym_eigensystem(n, cnt, _p(buf), max(1, n), _p(w), _p(Z)),
+           "tg_sym_eigensystem")
+    w = w.reshape(cnt, n)
+    Z = np.transpose(Z, (0, 2, 1))
+    return (w[0], Z[0]) if single else (w, Z)
 
 
 def calculate_overlap(nocc0, nemp0, ovac0, voveru0, D0, nocc1, nemp1, ovac1, voveru1, D1, ROT):
