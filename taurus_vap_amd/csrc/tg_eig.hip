@@ -29,8 +29,7 @@
 //          for the wide ones), their columns in ascending order of eigenvalue.
 //   3. The back-transformation by the reflectors in blocks of 32 as
 //      I - V T V^T (T upper triangular from V^T V and tau), each workgroup
-//      carrying 32 (or 16) columns of the eigenvector matrix, in LDS, through
-//      every block.
+//      carrying 8 columns of the eigenvector matrix, in LDS, through every block.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -83,22 +82,37 @@ __device__ inline double block_max(double v, double* red) {
   return s;
 }
 
+// 1/x from the hardware reciprocal and two Newton steps (the secular sums only
+// need the terms to a few ulps; the quotients that fix the eigenvectors divide)
+__device__ inline double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // 1. Householder tridiagonalization (lower), one workgroup per matrix.
 // A: n x n working copy (ld n), Vm: reflectors by column (v_i(i+1) = 1, zero
-// above), Wm: the panel's w vectors (n x EG_NB).  d, e (n - 1), tau (n - 2).
-// Dynamic LDS: 3 n + 2 EG_NB + TH / 64 doubles.
-template <int TH>
+// above) for the back-transformation.  d, e (n - 1), tau (n - 2).
+// The panel of PB columns keeps its reflectors and w vectors in LDS (Vp, Wp,
+// n x PB each), so that only the product with the trailing matrix reads L2; that
+// product runs by rows (A22 symmetric), each thread one row over a slice of the
+// columns, with the slices summed through LDS.
+// Dynamic LDS: (3 + 2 PB + TD_NS) n + 2 PB + TH / 64 doubles.
+constexpr int TD_TH = 512;
+constexpr int TD_NS = 4;        // column slices of the row products
+
+template <int TH, int PB>
 __global__ void __launch_bounds__(TH)
 tridiag_kernel(int n, const double* __restrict__ Ain, int lda, long long sA, double* __restrict__ A,
-               double* __restrict__ Vm, double* __restrict__ Wm, double* __restrict__ d,
-               double* __restrict__ e, double* __restrict__ tau) {
+               double* __restrict__ Vm, double* __restrict__ d, double* __restrict__ e,
+               double* __restrict__ tau) {
   const int b = blockIdx.x;
   const long long nn = (long long)n * n;
   const double* src = Ain + (long long)b * sA;
   A += b * nn;
   Vm += b * nn;
-  Wm += (long long)b * n * EG_NB;
   d += (long long)b * n;
   e += (long long)b * n;
   tau += (long long)b * n;
@@ -108,32 +122,31 @@ tridiag_kernel(int n, const double* __restrict__ Ain, int lda, long long sA, dou
   double* col = sh;
   double* v = col + n;
   double* p = v + n;
-  double* cf = p + n;             // 2 EG_NB coefficients
-  double* red = cf + 2 * EG_NB;
+  double* Vp = p + n;                       // Vp[r + jj n] = V(r, k0 + jj)
+  double* Wp = Vp + (size_t)PB * n;         // Wp[r + jj n] = W(r, jj)
+  double* part = Wp + (size_t)PB * n;       // part[sl n + r]
+  double* cf = part + (size_t)TD_NS * n;    // 2 PB
+  double* red = cf + 2 * PB;
   // symmetric working copy from the upper triangle
-  for (long long idx = t; idx < nn; idx += TH) {
-    const int i = (int)(idx % n), j = (int)(idx / n);
-    A[idx] = (i <= j) ? src[i + (long long)j * lda] : src[j + (long long)i * lda];
-    Vm[idx] = 0.0;
-  }
+  for (int j = 0; j < n; ++j)
+    for (int i = t; i < n; i += TH) {
+      const long long o = i + (long long)j * n;
+      A[o] = (i <= j) ? src[i + (long long)j * lda] : src[j + (long long)i * lda];
+      Vm[o] = 0.0;
+    }
   __syncthreads();
   const int nref = n - 2;
-  for (int k0 = 0; k0 < nref; k0 += EG_NB) {
-    const int nbp = min(EG_NB, nref - k0);
+  for (int k0 = 0; k0 < nref; k0 += PB) {
+    const int nbp = min(PB, nref - k0);
     for (int j = 0; j < nbp; ++j) {
       const int i = k0 + j;
       const int m = n - i;   // rows i..n-1
       // column i with the panel's earlier reflectors applied
-      if (t < j) {
-        cf[t] = Wm[i + (long long)t * n];
-        cf[EG_NB + t] = Vm[i + (long long)(k0 + t) * n];
-      }
-      __syncthreads();
       for (int r = t; r < m; r += TH) {
         const int row = i + r;
         double a = A[row + (long long)i * n];
         for (int jj = 0; jj < j; ++jj)
-          a -= Vm[row + (long long)(k0 + jj) * n] * cf[jj] + Wm[row + (long long)jj * n] * cf[EG_NB + jj];
+          a -= Vp[row + jj * n] * Wp[i + jj * n] + Wp[row + jj * n] * Vp[i + jj * n];
         col[r] = a;
       }
       __syncthreads();
@@ -156,21 +169,36 @@ tridiag_kernel(int n, const double* __restrict__ Ain, int lda, long long sA, dou
       for (int r = 1 + t; r < m; r += TH) {
         const double vr = (r == 1) ? 1.0 : col[r] * scale;
         v[r - 1] = vr;
+        Vp[(i + r) + j * n] = vr;
         Vm[(i + r) + (long long)i * n] = vr;
       }
       __syncthreads();
-      // p = A22 v (A22 symmetric: column c dotted with v), t1 = W^T v, t2 = V^T v
+      // p = A22 v by rows over TD_NS column slices; t1 = W^T v, t2 = V^T v (LDS)
       const int mv = m - 1;
-      for (int c = wid; c < mv; c += NW) {
-        const double* ac = A + (i + 1) + (long long)(i + 1 + c) * n;
-        double acc = 0.0;
-        for (int r = lane; r < mv; r += 64) acc = fma(ac[r], v[r], acc);
-        acc = wave_sum(acc);
-        if (lane == 0) p[c] = acc;
+      const int ns = min(TD_NS, max(1, TH / mv));
+      for (int tt = t; tt < ns * mv; tt += TH) {
+        const int sl = tt / mv, r = tt - sl * mv;
+        const double* ar = A + (i + 1 + r) + (long long)(i + 1) * n;
+        // sixteen loads in flight per lane: the row product is bound by L2 latency
+        constexpr int U = 16;
+        double acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[u] = 0.0;
+        int c = sl;
+        for (; c + (U - 1) * ns < mv; c += U * ns) {
+          double x[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) x[u] = ar[(long long)(c + u * ns) * n];
+#pragma unroll
+          for (int u = 0; u < U; ++u) acc[u] = fma(x[u], v[c + u * ns], acc[u]);
+        }
+        for (; c < mv; c += ns) acc[0] = fma(ar[(long long)c * n], v[c], acc[0]);
+#pragma unroll
+        for (int u = 1; u < U; ++u) acc[0] += acc[u];
+        part[sl * n + r] = acc[0];
       }
       for (int q = wid; q < 2 * j; q += NW) {
-        const double* x = (q < j) ? Wm + (i + 1) + (long long)q * n
-                                  : Vm + (i + 1) + (long long)(k0 + q - j) * n;
+        const double* x = (q < j) ? Wp + (i + 1) + q * n : Vp + (i + 1) + (q - j) * n;
         double acc = 0.0;
         for (int r = lane; r < mv; r += 64) acc = fma(x[r], v[r], acc);
         acc = wave_sum(acc);
@@ -178,10 +206,10 @@ tridiag_kernel(int n, const double* __restrict__ Ain, int lda, long long sA, dou
       }
       __syncthreads();
       for (int r = t; r < mv; r += TH) {
-        double a = p[r];
+        double a = part[r];
+        for (int sl = 1; sl < ns; ++sl) a += part[sl * n + r];
         for (int jj = 0; jj < j; ++jj)
-          a -= Vm[(i + 1 + r) + (long long)(k0 + jj) * n] * cf[jj] +
-               Wm[(i + 1 + r) + (long long)jj * n] * cf[j + jj];
+          a -= Vp[(i + 1 + r) + jj * n] * cf[jj] + Wp[(i + 1 + r) + jj * n] * cf[j + jj];
         p[r] = tv * a;
       }
       __syncthreads();
@@ -189,18 +217,34 @@ tridiag_kernel(int n, const double* __restrict__ Ain, int lda, long long sA, dou
       for (int r = t; r < mv; r += TH) dot = fma(p[r], v[r], dot);
       dot = block_sum<TH>(dot, red);
       const double al = -0.5 * tv * dot;
-      for (int r = t; r < mv; r += TH) Wm[(i + 1 + r) + (long long)j * n] = fma(al, v[r], p[r]);
+      for (int r = t; r < mv; r += TH) Wp[(i + 1 + r) + j * n] = fma(al, v[r], p[r]);
       __syncthreads();
     }
-    // trailing update A(s0:, s0:) -= V W^T + W V^T
+    // trailing update A(s0:, s0:) -= V W^T + W V^T: each thread one row (its V, W
+    // rows in registers) over a share of the columns
     const int s0 = k0 + nbp, ms = n - s0;
-    for (long long idx = t; idx < (long long)ms * ms; idx += TH) {
-      const int r = s0 + (int)(idx % ms), c = s0 + (int)(idx / ms);
-      double a = A[r + (long long)c * n];
-      for (int jj = 0; jj < nbp; ++jj)
-        a -= Vm[r + (long long)(k0 + jj) * n] * Wm[c + (long long)jj * n] +
-             Wm[r + (long long)jj * n] * Vm[c + (long long)(k0 + jj) * n];
-      A[r + (long long)c * n] = a;
+    if (ms > 0) {
+      const int ng = max(1, TH / ms);
+      for (int tt = t; tt < ng * ms; tt += TH) {
+        const int r = s0 + tt % ms, cg = tt / ms;
+        double vr[PB], wr[PB];
+#pragma unroll
+        for (int jj = 0; jj < PB; ++jj) {
+          vr[jj] = (jj < nbp) ? Vp[r + jj * n] : 0.0;
+          wr[jj] = (jj < nbp) ? Wp[r + jj * n] : 0.0;
+        }
+        for (int c = s0 + cg; c < n; c += ng) {
+          double* ap = A + r + (long long)c * n;
+          double a = *ap;
+#pragma unroll
+          for (int jj = 0; jj < PB; ++jj) {
+            const double wc = (jj < nbp) ? Wp[c + jj * n] : 0.0;
+            const double vc = (jj < nbp) ? Vp[c + jj * n] : 0.0;
+            a = fma(-vr[jj], wc, fma(-wr[jj], vc, a));
+          }
+          *ap = a;
+        }
+      }
     }
     __syncthreads();
   }
@@ -295,8 +339,9 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
         continue;
       }
       if (prev >= 0) {
-        const double tt = hypot(zs[prev], zs[p]);
-        const double c = zs[p] / tt, s = -zs[prev] / tt;
+        const double tt = sqrt(fma(zs[prev], zs[prev], zs[p] * zs[p]));
+        const double it = 1.0 / tt;
+        const double c = zs[p] * it, s = -zs[prev] * it;
         if (fabs(c * s * (ds[p] - ds[prev])) <= tol) {
           ra[nrot] = prev;
           rb[nrot] = p;
@@ -339,7 +384,11 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
     zz[j] = zs[pos[j]];
   }
   __syncthreads();
-  for (int j = t; j < k; j += TH) pos[nd[j]] = j;   // position -> root index
+  for (int j = t; j < k; j += TH) {
+    pos[nd[j]] = j;                   // position -> root index
+    zh[j] = rho * zz[j] * zz[j];      // rho z^2 (zh holds z-hat after the roots)
+  }
+  __syncthreads();
   // secular roots: l_j = dd[og_j] + tz_j
   for (int j = t; j < k; j += TH) {
     int o;
@@ -347,7 +396,7 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
     if (j < k - 1) {
       const double mid = 0.5 * (dd[j + 1] - dd[j]);
       double fm = 1.0;
-      for (int i = 0; i < k; ++i) fm += rho * zz[i] * zz[i] / ((dd[i] - dd[j]) - mid);
+      for (int i = 0; i < k; ++i) fm += zh[i] / ((dd[i] - dd[j]) - mid);
       if (fm > 0.0) {
         o = j;
         a = 0.0;
@@ -369,16 +418,19 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
     bool ok = false;
     for (int it = 0; it < 100 && !ok; ++it) {
       double psi = 0.0, phi = 0.0, dpsi = 0.0, dphi = 0.0;
-      for (int i = 0; i < k; ++i) {
-        const double den = (dd[i] - dd[o]) - tau;
-        const double term = rho * zz[i] * zz[i] / den;
-        if (jh < 0 || i < jh) {
-          psi += term;
-          dpsi += term / den;
-        } else {
-          phi += term;
-          dphi += term / den;
-        }
+      const int isplit = (jh < 0) ? k : jh;
+      const double dob = dd[o];
+      for (int i = 0; i < isplit; ++i) {
+        const double rinv = recip((dd[i] - dob) - tau);
+        const double term = zh[i] * rinv;
+        psi += term;
+        dpsi = fma(term, rinv, dpsi);
+      }
+      for (int i = isplit; i < k; ++i) {
+        const double rinv = recip((dd[i] - dob) - tau);
+        const double term = zh[i] * rinv;
+        phi += term;
+        dphi = fma(term, rinv, dphi);
       }
       const double f = 1.0 + psi + phi;
       const double err = 4.0 * DBL_EPSILON * (1.0 + fabs(psi) + fabs(phi) + fabs(tau) * (dpsi + dphi));
@@ -449,23 +501,21 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
     }
     fp[p] = r;
   }
-  for (long long idx = t; idx < (long long)m * m; idx += TH)
-    Uf[(r0 + (int)(idx % m)) + (long long)(r0 + (int)(idx / m)) * n] = 0.0;
+  for (int c = 0; c < m; ++c)
+    for (int row = t; row < m; row += TH) Uf[(r0 + row) + (long long)(r0 + c) * n] = 0.0;
   __syncthreads();
   // the merge's eigenvector matrix: column fp[p] is e_p (deflated p) or the
   // secular eigenvector of root j (p = nd[j]) over the non-deflated rows
-  for (int p = t; p < m; p += TH) {
-    double* uc = Uf + r0 + (long long)(r0 + fp[p]) * n;
-    if (flag[p]) {
-      uc[p] = 1.0;
-    } else {
-      const int j = pos[p];
-      const double dj = dd[og[j]], tj = tz[j], sj = cn[j];
-      for (int i = 0; i < k; ++i) uc[nd[i]] = zh[i] / ((dd[i] - dj) - tj) * sj;
-    }
+  for (int p = t; p < m; p += TH)
+    if (flag[p]) Uf[(r0 + p) + (long long)(r0 + fp[p]) * n] = 1.0;
+  for (int i = t; i < k; i += TH) {   // row nd[i]: consecutive threads, consecutive rows
+    double* ur = Uf + (r0 + nd[i]);
+    const double zi = zh[i], di = dd[i];
+    for (int j = 0; j < k; ++j)
+      ur[(long long)(r0 + fp[nd[j]]) * n] = zi / ((di - dd[og[j]]) - tz[j]) * cn[j];
   }
   // the children's eigenvectors gathered in sorted order, then the deflation
-  // rotations, row by row
+  // rotations in their order, row by row
   for (int row = t; row < m; row += TH) {
     const double* zr = Z + (r0 + row);
     double* qr = Qr + (r0 + row);
@@ -479,8 +529,8 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
   }
   __syncthreads();
   if (m <= EG_SMALL) {
-    for (long long idx = t; idx < (long long)m * m; idx += TH) {
-      const int row = (int)(idx % m), c = (int)(idx / m);
+    for (int idx = t; idx < m * m; idx += TH) {
+      const int c = idx / m, row = idx - c * m;
       double acc = 0.0;
       for (int q = 0; q < m; ++q)
         acc = fma(Qr[(r0 + row) + (long long)(r0 + q) * n], Uf[(r0 + q) + (long long)(r0 + c) * n], acc);
@@ -503,20 +553,31 @@ wy_t_kernel(int n, const double* __restrict__ Vm, const double* __restrict__ tau
   tau += (long long)st * n;
   Tm += ((long long)st * npan + pnl) * EG_NB * EG_NB;
   const int nref = n - 2, k0 = pnl * EG_NB, nbp = min(EG_NB, nref - k0);
-  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-  constexpr int NW = TH / 64;
-  __shared__ double G[EG_NB][EG_NB + 1], T[EG_NB][EG_NB + 1];
-  // G(a, c) = v_a . v_c for a < c (v_c vanishes above row k0 + c + 1)
-  for (int q = wid; q < nbp * nbp; q += NW) {
-    const int a = q % nbp, c = q / nbp;
-    if (a >= c) continue;
-    const double* va = Vm + (long long)(k0 + a) * n;
-    const double* vc = Vm + (long long)(k0 + c) * n;
-    double acc = 0.0;
-    for (int r = k0 + c + 1 + lane; r < n; r += 64) acc = fma(va[r], vc[r], acc);
-    acc = wave_sum(acc);
-    if (lane == 0) G[a][c] = acc;
+  const int t = threadIdx.x;
+  constexpr int RC = 64, OPT = EG_NB * EG_NB / TH;
+  __shared__ double G[EG_NB][EG_NB + 1], T[EG_NB][EG_NB + 1], Vc[RC][EG_NB + 1];
+  // G = V^T V over rows k0+1 .. n-1, the rows passing through LDS in chunks;
+  // thread t holds G(a, c) for c = t % 32, a = t / 32 + 8 u
+  double acc[OPT];
+#pragma unroll
+  for (int u = 0; u < OPT; ++u) acc[u] = 0.0;
+  const int cc = t % EG_NB, ab = t / EG_NB;
+  for (int r0 = k0 + 1; r0 < n; r0 += RC) {
+    const int rows = min(RC, n - r0);
+    __syncthreads();
+    for (int q = t; q < RC * EG_NB; q += TH) {
+      const int rr = q % RC, a = q / RC;
+      Vc[rr][a] = (rr < rows && a < nbp) ? Vm[(r0 + rr) + (long long)(k0 + a) * n] : 0.0;
+    }
+    __syncthreads();
+    for (int rr = 0; rr < rows; ++rr) {
+      const double vc = Vc[rr][cc];
+#pragma unroll
+      for (int u = 0; u < OPT; ++u) acc[u] = fma(Vc[rr][ab + (TH / EG_NB) * u], vc, acc[u]);
+    }
   }
+#pragma unroll
+  for (int u = 0; u < OPT; ++u) G[ab + (TH / EG_NB) * u][cc] = acc[u];
   for (int q = t; q < EG_NB * EG_NB; q += TH) T[q % EG_NB][q / EG_NB] = 0.0;
   __syncthreads();
   if (t < nbp) T[t][t] = tau[k0 + t];
@@ -533,7 +594,11 @@ wy_t_kernel(int n, const double* __restrict__ Vm, const double* __restrict__ tau
 }
 
 // E(:, c0:c0+CB) = (I - V_0 T_0 V_0^T) ... (I - V_P T_P V_P^T) Z(:, c0:c0+CB).
-// Dynamic LDS: n (CB + 1) + EG_NB CB + EG_NB (EG_NB + 1) doubles.
+// V of the block passes through LDS in chunks of WY_RC rows, twice per block
+// (for X = V^T Zb, then for Zb -= V (T X)).
+// Dynamic LDS: n (CB + 1) + EG_NB CB + EG_NB (EG_NB + 1) + WY_RC (EG_NB + 1) doubles.
+constexpr int WY_RC = 64;
+
 template <int TH, int CB>
 __global__ void __launch_bounds__(TH)
 wy_apply_kernel(int n, int npan, const double* __restrict__ Vm, const double* __restrict__ Tm,
@@ -545,56 +610,84 @@ wy_apply_kernel(int n, int npan, const double* __restrict__ Vm, const double* __
   E += st * sE;
   Tm += (long long)st * npan * EG_NB * EG_NB;
   const int t = threadIdx.x;
+  constexpr int LDZ = CB + 1, LDV = EG_NB + 1;
+  constexpr int NG = TH / CB;               // thread groups per column
+  constexpr int OPT = EG_NB / NG > 0 ? EG_NB / NG : 1;   // X entries per thread
   extern __shared__ double sh[];
-  double* Zb = sh;                               // Zb[r * (CB + 1) + c]
-  double* X = Zb + (size_t)n * (CB + 1);         // X[a * CB + c]
-  double* T = X + EG_NB * CB;                    // T[a * (EG_NB + 1) + b]
-  constexpr int LDZ = CB + 1;
-  for (long long idx = t; idx < (long long)n * CB; idx += TH) {
-    const int r = (int)(idx % n), c = (int)(idx / n);
-    Zb[r * LDZ + c] = (c < nc) ? Z[r + (long long)(c0 + c) * n] : 0.0;
-  }
+  double* Zb = sh;                               // Zb[r * LDZ + c]
+  double* X = Zb + (size_t)n * LDZ;              // X[a * CB + c]
+  double* T = X + EG_NB * CB;                    // T[a * LDV + b]
+  double* Vc = T + EG_NB * LDV;                  // Vc[rr * LDV + a]
+  for (int cc = 0; cc < CB; ++cc)
+    for (int r = t; r < n; r += TH) Zb[r * LDZ + cc] = (cc < nc) ? Z[r + (long long)(c0 + cc) * n] : 0.0;
   const int nref = n - 2;
   const int c = t % CB, g = t / CB;
-  constexpr int NG = TH / CB;
   for (int pnl = npan - 1; pnl >= 0; --pnl) {
     const int k0 = pnl * EG_NB, nbp = min(EG_NB, nref - k0);
     for (int q = t; q < EG_NB * EG_NB; q += TH)
-      T[(q % EG_NB) * (EG_NB + 1) + q / EG_NB] = Tm[(long long)pnl * EG_NB * EG_NB + q];
-    __syncthreads();
-    // X = V^T Zb
-    for (int a = g; a < nbp; a += NG) {
-      const double* va = Vm + (long long)(k0 + a) * n;
-      double acc = 0.0;
-      for (int r = k0 + a + 1; r < n; ++r) acc = fma(va[r], Zb[r * LDZ + c], acc);
-      X[a * CB + c] = acc;
+      T[(q % EG_NB) * LDV + q / EG_NB] = Tm[(long long)pnl * EG_NB * EG_NB + q];
+    double acc[OPT];
+#pragma unroll
+    for (int u = 0; u < OPT; ++u) acc[u] = 0.0;
+    // X = V^T Zb over rows k0+1 .. n-1 (v_a vanishes above row k0 + a + 1)
+    for (int r0 = k0 + 1; r0 < n; r0 += WY_RC) {
+      const int rows = min(WY_RC, n - r0);
+      __syncthreads();
+      for (int q = t; q < WY_RC * EG_NB; q += TH) {
+        const int rr = q % WY_RC, a = q / WY_RC;
+        Vc[rr * LDV + a] = (rr < rows && a < nbp) ? Vm[(r0 + rr) + (long long)(k0 + a) * n] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < OPT; ++u) {
+        const int a = g + NG * u;
+        if (a < EG_NB) {
+          double x = acc[u];
+          for (int rr = 0; rr < rows; ++rr) x = fma(Vc[rr * LDV + a], Zb[(r0 + rr) * LDZ + c], x);
+          acc[u] = x;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < OPT; ++u) {
+      const int a = g + NG * u;
+      if (a < EG_NB) X[a * CB + c] = acc[u];
     }
     __syncthreads();
-    // Y = T X (in place, row a from rows >= a: top-down reads rows not yet written)
-    double y[EG_NB / NG + 1];
-    int ny = 0;
-    for (int a = g; a < nbp; a += NG) {
-      double acc = 0.0;
-      for (int l = a; l < nbp; ++l) acc = fma(T[a * (EG_NB + 1) + l], X[l * CB + c], acc);
-      y[ny++] = acc;
+    // Y = T X (T upper triangular), kept in registers, then back into X
+#pragma unroll
+    for (int u = 0; u < OPT; ++u) {
+      const int a = g + NG * u;
+      double y = 0.0;
+      if (a < nbp)
+        for (int l = a; l < nbp; ++l) y = fma(T[a * LDV + l], X[l * CB + c], y);
+      acc[u] = y;
     }
     __syncthreads();
-    ny = 0;
-    for (int a = g; a < nbp; a += NG) X[a * CB + c] = y[ny++];
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < OPT; ++u) {
+      const int a = g + NG * u;
+      if (a < EG_NB) X[a * CB + c] = acc[u];
+    }
     // Zb -= V Y
-    for (int r = k0 + 1 + g; r < n; r += NG) {
-      double acc = Zb[r * LDZ + c];
-      const int amax = min(nbp, r - k0);   // v_a(r) = 0 for r <= k0 + a
-      for (int a = 0; a < amax; ++a) acc = fma(-Vm[r + (long long)(k0 + a) * n], X[a * CB + c], acc);
-      Zb[r * LDZ + c] = acc;
+    for (int r0 = k0 + 1; r0 < n; r0 += WY_RC) {
+      const int rows = min(WY_RC, n - r0);
+      __syncthreads();
+      for (int q = t; q < WY_RC * EG_NB; q += TH) {
+        const int rr = q % WY_RC, a = q / WY_RC;
+        Vc[rr * LDV + a] = (rr < rows && a < nbp) ? Vm[(r0 + rr) + (long long)(k0 + a) * n] : 0.0;
+      }
+      __syncthreads();
+      for (int rr = g; rr < rows; rr += NG) {
+        double z = Zb[(r0 + rr) * LDZ + c];
+        for (int a = 0; a < nbp; ++a) z = fma(-Vc[rr * LDV + a], X[a * CB + c], z);
+        Zb[(r0 + rr) * LDZ + c] = z;
+      }
     }
     __syncthreads();
   }
-  for (long long idx = t; idx < (long long)n * nc; idx += TH) {
-    const int r = (int)(idx % n), cc = (int)(idx / n);
-    E[r + (long long)(c0 + cc) * n] = Zb[r * LDZ + cc];
-  }
+  for (int cc = 0; cc < nc; ++cc)
+    for (int r = t; r < n; r += TH) E[r + (long long)(c0 + cc) * n] = Zb[r * LDZ + cc];
 }
 
 // merges grouped by height (children first): (r0, n1, n2) triples
@@ -634,7 +727,7 @@ const MergePlan& merge_plan(int n) {
 size_t sym_eigensystem_work_bytes(int n, int count) {
   const size_t nn = (size_t)n * n;
   const int npan = n > 2 ? (n - 2 + EG_NB - 1) / EG_NB : 0;
-  const size_t doubles = (size_t)count * (5 * nn + (size_t)n * EG_NB + 4 * (size_t)n +
+  const size_t doubles = (size_t)count * (5 * nn + 4 * (size_t)n +
                                           (size_t)std::max(1, npan) * EG_NB * EG_NB);
   const size_t ints = (size_t)3 * n + (size_t)count;
   return sizeof(double) * doubles + sizeof(int) * ints + 256;
@@ -654,8 +747,7 @@ int batched_sym_eigensystem(hipStream_t s, int n, const double* A, int lda, long
   double* Z = Vm + count * nn;
   double* Qr = Z + count * nn;
   double* Uf = Qr + count * nn;
-  double* Wm = Uf + count * nn;
-  double* d = Wm + (size_t)count * n * EG_NB;
+  double* d = Uf + count * nn;
   double* e = d + (size_t)count * n;
   double* tau = e + (size_t)count * n;
   double* wi = tau + (size_t)count * n;
@@ -665,9 +757,19 @@ int batched_sym_eigensystem(hipStream_t s, int n, const double* A, int lda, long
   const MergePlan& P = merge_plan(n);
   if (!P.desc.empty())
     TG_HIP(hipMemcpyAsync(idesc, P.desc.data(), sizeof(int) * P.desc.size(), hipMemcpyHostToDevice, s));
-  const size_t lds1 = sizeof(double) * (3 * (size_t)n + 2 * EG_NB + EG_TH / 64);
-  hipLaunchKernelGGL(tridiag_kernel<EG_TH>, dim3(count), dim3(EG_TH), lds1, s, n, A, lda, sA, Aw, Vm,
-                     Wm, d, e, tau);
+  {
+    const int pb = n <= 384 ? 16 : (n <= 640 ? 8 : 4);
+    const size_t lds1 = sizeof(double) * ((3 + 2 * (size_t)pb + TD_NS) * n + 2 * pb + TD_TH / 64);
+    if (pb == 16)
+      hipLaunchKernelGGL((tridiag_kernel<TD_TH, 16>), dim3(count), dim3(TD_TH), lds1, s, n, A, lda, sA,
+                         Aw, Vm, d, e, tau);
+    else if (pb == 8)
+      hipLaunchKernelGGL((tridiag_kernel<TD_TH, 8>), dim3(count), dim3(TD_TH), lds1, s, n, A, lda, sA,
+                         Aw, Vm, d, e, tau);
+    else
+      hipLaunchKernelGGL((tridiag_kernel<TD_TH, 4>), dim3(count), dim3(TD_TH), lds1, s, n, A, lda, sA,
+                         Aw, Vm, d, e, tau);
+  }
   TG_HIP(hipGetLastError());
   const unsigned gi = (unsigned)std::min<long long>((nn + EG_TH - 1) / EG_TH, 1024);
   hipLaunchKernelGGL(dc_init_kernel, dim3(gi, count), dim3(EG_TH), 0, s, n, d, e, wi, Z, dinfo);
@@ -690,17 +792,13 @@ int batched_sym_eigensystem(hipStream_t s, int n, const double* A, int lda, long
     hipLaunchKernelGGL(wy_t_kernel<EG_TH>, dim3(npan, count), dim3(EG_TH), 0, s, n, Vm, tau, Tm);
     TG_HIP(hipGetLastError());
   }
-  if (n <= 512) {
-    constexpr int CB = 32;
-    const size_t lds = sizeof(double) * ((size_t)n * (CB + 1) + EG_NB * CB + EG_NB * (EG_NB + 1));
-    hipLaunchKernelGGL((wy_apply_kernel<EG_TH, CB>), dim3((n + CB - 1) / CB, count), dim3(EG_TH), lds, s,
-                       n, npan, Vm, Tm, Z, E, sE);
-  } else {
-    constexpr int CB = 16;
-    const size_t lds = sizeof(double) * ((size_t)n * (CB + 1) + EG_NB * CB + EG_NB * (EG_NB + 1));
-    hipLaunchKernelGGL((wy_apply_kernel<EG_TH, CB>), dim3((n + CB - 1) / CB, count), dim3(EG_TH), lds, s,
-                       n, npan, Vm, Tm, Z, E, sE);
-  }
+  auto apply_lds = [&](int cb) {
+    return sizeof(double) * ((size_t)n * (cb + 1) + EG_NB * cb + EG_NB * (EG_NB + 1) +
+                             WY_RC * (EG_NB + 1));
+  };
+  // eight columns per workgroup: n / 8 workgroups carry the blocks in parallel
+  hipLaunchKernelGGL((wy_apply_kernel<EG_TH, 8>), dim3((n + 7) / 8, count), dim3(EG_TH), apply_lds(8), s,
+                     n, npan, Vm, Tm, Z, E, sE);
   TG_HIP(hipGetLastError());
   TG_HIP(hipMemcpy2DAsync(w, sizeof(double) * ws, wi, sizeof(double) * n, sizeof(double) * n, count,
                           hipMemcpyDeviceToDevice, s));

@@ -58,14 +58,23 @@ skew_half_kernel(int n, const double* __restrict__ T10, const double* __restrict
   }
 }
 
-// out[p] = sum_k X_p[k] Y_p[k] for the pointer pairs of the list (one block per pair)
+// out[p * gridDim.y + q] = sum_k X_p[k] Y_p[k] over slice q of the gridDim.y slices
+// of [0, len), for the pointer pairs of the list (block (p, q))
 __global__ void __launch_bounds__(NT)
 dots_kernel(long long len, const double* const* __restrict__ xs, const double* const* __restrict__ ys,
             double* __restrict__ out) {
   const double* x = xs[blockIdx.x];
   const double* y = ys[blockIdx.x];
-  double s = 0.0;
-  for (long long k = threadIdx.x; k < len; k += NT) s = fma(x[k], y[k], s);
+  const long long chunk = (len + gridDim.y - 1) / gridDim.y;
+  const long long k0 = chunk * blockIdx.y, k1 = min(len, k0 + chunk);
+  double s = 0.0, s1 = 0.0;
+  long long k = k0 + threadIdx.x;
+  for (; k + NT < k1; k += 2 * NT) {
+    s = fma(x[k], y[k], s);
+    s1 = fma(x[k + NT], y[k + NT], s1);
+  }
+  if (k < k1) s = fma(x[k], y[k], s);
+  s += s1;
   __shared__ double red[NT];
   red[threadIdx.x] = s;
   __syncthreads();
@@ -73,7 +82,7 @@ dots_kernel(long long len, const double* const* __restrict__ xs, const double* c
     if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) out[blockIdx.x * gridDim.y + blockIdx.y] = red[0];
 }
 
 // S += 1 for the matrix of state blockIdx.y (stride nn)
@@ -128,6 +137,7 @@ __global__ void fill_int_kernel(int len, int v, int* __restrict__ x) {
 // pivot is replaced by 1 and its column (1-based) recorded in info[s] if none
 // was before.
 constexpr int CB = 32;
+static_assert(CB == 32, "chol_block_kernel maps 64 threads onto 32 rows x 2 column parities");
 __global__ void __launch_bounds__(64)
 chol_block_kernel(int n, int k0, int b, const double* __restrict__ A0, double* __restrict__ L0,
                   double* __restrict__ X0, int* __restrict__ info) {
@@ -137,10 +147,10 @@ chol_block_kernel(int n, int k0, int b, const double* __restrict__ A0, double* _
   double* L = L0 + blockIdx.x * nn;
   double* X = X0 + blockIdx.x * nn;
   const int t = threadIdx.x;
-  for (int e = t; e < b * b; e += 64) {
-    const int i = e % b, j = e / b;
-    a[i][j] = (i >= j) ? A[(k0 + i) + (long long)(k0 + j) * n] : 0.0;
-  }
+  // 64 threads: row t & 31 of the columns of parity t >> 5 (no integer division)
+  const int ti = t & 31, tj = t >> 5;
+  for (int j = tj; j < b; j += 2)
+    if (ti < b) a[ti][j] = (ti >= j) ? A[(k0 + ti) + (long long)(k0 + j) * n] : 0.0;
   __syncthreads();
   int bad = 0;
   for (int kk = 0; kk < b; ++kk) {
@@ -154,11 +164,8 @@ chol_block_kernel(int n, int k0, int b, const double* __restrict__ A0, double* _
     if (t == 0) a[kk][kk] = d;
     for (int i = kk + 1 + t; i < b; i += 64) a[i][kk] /= d;
     __syncthreads();
-    const int m = b - kk - 1;
-    for (int e = t; e < m * m; e += 64) {
-      const int i = kk + 1 + e % m, j = kk + 1 + e / m;
-      if (i >= j) a[i][j] -= a[i][kk] * a[j][kk];
-    }
+    if (ti > kk && ti < b)
+      for (int j = kk + 1 + tj; j <= ti; j += 2) a[ti][j] -= a[ti][kk] * a[j][kk];
     __syncthreads();
   }
   // inverse of the lower-triangular block: thread j solves column j
@@ -173,12 +180,12 @@ chol_block_kernel(int n, int k0, int b, const double* __restrict__ A0, double* _
     }
   }
   __syncthreads();
-  for (int e = t; e < b * b; e += 64) {
-    const int i = e % b, j = e / b;
-    const long long o = (k0 + i) + (long long)(k0 + j) * n;
-    L[o] = (i >= j) ? a[i][j] : 0.0;
-    X[o] = x[i][j];
-  }
+  for (int j = tj; j < b; j += 2)
+    if (ti < b) {
+      const long long o = (k0 + ti) + (long long)(k0 + j) * n;
+      L[o] = (ti >= j) ? a[ti][j] : 0.0;
+      X[o] = x[ti][j];
+    }
   if (t == 0 && bad && info[blockIdx.x] == 0) info[blockIdx.x] = bad;
 }
 
@@ -278,11 +285,21 @@ int dots(const Iter& it, const std::vector<const double*>& xs, const std::vector
     h[p] = xs[p];
     h[np + p] = ys[p];
   }
+  // each dot in up to 32 slices (one block each) as the result area allows;
+  // the slices are summed here in a fixed order
+  const long long room = 3LL * it.S * MAXPAIRS - 2LL * np;
+  const int ds = (int)std::max(1LL, std::min(32LL, room / np));
   TG_HIP(hipMemcpyAsync(pl, h.data(), sizeof(double*) * 2 * np, hipMemcpyHostToDevice, it.s));
-  hipLaunchKernelGGL(dots_kernel, dim3(np), dim3(NT), 0, it.s, it.nn, pl, pl + np, res);
+  hipLaunchKernelGGL(dots_kernel, dim3(np, ds), dim3(NT), 0, it.s, it.nn, pl, pl + np, res);
   TG_HIP(hipGetLastError());
-  TG_HIP(hipMemcpyAsync(out.data(), res, sizeof(double) * np, hipMemcpyDeviceToHost, it.s));
+  std::vector<double> part((size_t)np * ds);
+  TG_HIP(hipMemcpyAsync(part.data(), res, sizeof(double) * np * ds, hipMemcpyDeviceToHost, it.s));
   TG_HIP(hipStreamSynchronize(it.s));
+  for (int p = 0; p < np; ++p) {
+    double acc = 0.0;
+    for (int q = 0; q < ds; ++q) acc += part[(size_t)p * ds + q];
+    out[p] = acc;
+  }
   return TG_OK;
 }
 
