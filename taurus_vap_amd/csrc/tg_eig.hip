@@ -138,16 +138,34 @@ tridiag_kernel(int n, const double* __restrict__ Ain, int lda, long long sA, dou
   const int nref = n - 2;
   for (int k0 = 0; k0 < nref; k0 += PB) {
     const int nbp = min(PB, nref - k0);
+    // the panel arrays start at zero: the unrolled loops below read all PB columns
+    for (int q = t; q < 2 * PB * n; q += TH) Vp[q] = 0.0;   // Vp and Wp are adjacent
+    __syncthreads();
     for (int j = 0; j < nbp; ++j) {
       const int i = k0 + j;
       const int m = n - i;   // rows i..n-1
-      // column i with the panel's earlier reflectors applied
-      for (int r = t; r < m; r += TH) {
-        const int row = i + r;
-        double a = A[row + (long long)i * n];
-        for (int jj = 0; jj < j; ++jj)
-          a -= Vp[row + jj * n] * Wp[i + jj * n] + Wp[row + jj * n] * Vp[i + jj * n];
-        col[r] = a;
+      // column i with the panel's earlier reflectors applied (columns jj >= j of
+      // Vp, Wp are still zero); LDS operands loaded into registers first
+      {
+        double ci_w[PB], ci_v[PB];
+#pragma unroll
+        for (int jj = 0; jj < PB; ++jj) {
+          ci_w[jj] = Wp[i + jj * n];
+          ci_v[jj] = Vp[i + jj * n];
+        }
+        for (int r = t; r < m; r += TH) {
+          const int row = i + r;
+          double rv[PB], rw[PB];
+#pragma unroll
+          for (int jj = 0; jj < PB; ++jj) {
+            rv[jj] = Vp[row + jj * n];
+            rw[jj] = Wp[row + jj * n];
+          }
+          double a = A[row + (long long)i * n];
+#pragma unroll
+          for (int jj = 0; jj < PB; ++jj) a = fma(-rv[jj], ci_w[jj], fma(-rw[jj], ci_v[jj], a));
+          col[r] = a;
+        }
       }
       __syncthreads();
       // the reflector of col[1 .. m-1]
@@ -185,12 +203,16 @@ tridiag_kernel(int n, const double* __restrict__ Ain, int lda, long long sA, dou
 #pragma unroll
         for (int u = 0; u < U; ++u) acc[u] = 0.0;
         int c = sl;
+        const long long step = (long long)ns * n;
         for (; c + (U - 1) * ns < mv; c += U * ns) {
-          double x[U];
+          double x[U], vv[U];
+          const double* pc = ar + (long long)c * n;
 #pragma unroll
-          for (int u = 0; u < U; ++u) x[u] = ar[(long long)(c + u * ns) * n];
+          for (int u = 0; u < U; ++u) x[u] = pc[u * step];
 #pragma unroll
-          for (int u = 0; u < U; ++u) acc[u] = fma(x[u], v[c + u * ns], acc[u]);
+          for (int u = 0; u < U; ++u) vv[u] = v[c + u * ns];
+#pragma unroll
+          for (int u = 0; u < U; ++u) acc[u] = fma(x[u], vv[u], acc[u]);
         }
         for (; c < mv; c += ns) acc[0] = fma(ar[(long long)c * n], v[c], acc[0]);
 #pragma unroll
@@ -205,12 +227,27 @@ tridiag_kernel(int n, const double* __restrict__ Ain, int lda, long long sA, dou
         if (lane == 0) cf[q] = acc;   // cf[0, j) = t1, cf[j, 2j) = t2
       }
       __syncthreads();
-      for (int r = t; r < mv; r += TH) {
-        double a = part[r];
-        for (int sl = 1; sl < ns; ++sl) a += part[sl * n + r];
-        for (int jj = 0; jj < j; ++jj)
-          a -= Vp[(i + 1 + r) + jj * n] * cf[jj] + Wp[(i + 1 + r) + jj * n] * cf[j + jj];
-        p[r] = tv * a;
+      {
+        double t1[PB], t2[PB];
+#pragma unroll
+        for (int jj = 0; jj < PB; ++jj) {
+          t1[jj] = (jj < j) ? cf[jj] : 0.0;
+          t2[jj] = (jj < j) ? cf[j + jj] : 0.0;
+        }
+        for (int r = t; r < mv; r += TH) {
+          const int row = i + 1 + r;
+          double rv[PB], rw[PB];
+#pragma unroll
+          for (int jj = 0; jj < PB; ++jj) {
+            rv[jj] = Vp[row + jj * n];
+            rw[jj] = Wp[row + jj * n];
+          }
+          double a = part[r];
+          for (int sl = 1; sl < ns; ++sl) a += part[sl * n + r];
+#pragma unroll
+          for (int jj = 0; jj < PB; ++jj) a = fma(-rv[jj], t1[jj], fma(-rw[jj], t2[jj], a));
+          p[r] = tv * a;
+        }
       }
       __syncthreads();
       double dot = 0.0;
@@ -230,18 +267,20 @@ tridiag_kernel(int n, const double* __restrict__ Ain, int lda, long long sA, dou
         double vr[PB], wr[PB];
 #pragma unroll
         for (int jj = 0; jj < PB; ++jj) {
-          vr[jj] = (jj < nbp) ? Vp[r + jj * n] : 0.0;
-          wr[jj] = (jj < nbp) ? Wp[r + jj * n] : 0.0;
+          vr[jj] = Vp[r + jj * n];
+          wr[jj] = Wp[r + jj * n];
         }
         for (int c = s0 + cg; c < n; c += ng) {
           double* ap = A + r + (long long)c * n;
-          double a = *ap;
+          double wc[PB], vc[PB];
 #pragma unroll
           for (int jj = 0; jj < PB; ++jj) {
-            const double wc = (jj < nbp) ? Wp[c + jj * n] : 0.0;
-            const double vc = (jj < nbp) ? Vp[c + jj * n] : 0.0;
-            a = fma(-vr[jj], wc, fma(-wr[jj], vc, a));
+            wc[jj] = Wp[c + jj * n];
+            vc[jj] = Vp[c + jj * n];
           }
+          double a = *ap;
+#pragma unroll
+          for (int jj = 0; jj < PB; ++jj) a = fma(-vr[jj], wc[jj], fma(-wr[jj], vc[jj], a));
           *ap = a;
         }
       }
@@ -314,6 +353,7 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
   for (int u = t; u < m; u += TH) {
     const double x = ov[u];
     int r = 0;
+    #pragma unroll 8
     for (int q = 0; q < m; ++q) {
       const double y = ov[q];
       r += (y < x) || (y == x && q < u);
@@ -371,6 +411,7 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
   for (int j = t; j < k; j += TH) {
     const double x = ov[j];
     int r = 0;
+    #pragma unroll 8
     for (int q = 0; q < k; ++q) {
       const double y = ov[q];
       r += (y < x) || (y == x && q < j);
@@ -396,6 +437,7 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
     if (j < k - 1) {
       const double mid = 0.5 * (dd[j + 1] - dd[j]);
       double fm = 1.0;
+      #pragma unroll 8
       for (int i = 0; i < k; ++i) fm += zh[i] / ((dd[i] - dd[j]) - mid);
       if (fm > 0.0) {
         o = j;
@@ -420,12 +462,14 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
       double psi = 0.0, phi = 0.0, dpsi = 0.0, dphi = 0.0;
       const int isplit = (jh < 0) ? k : jh;
       const double dob = dd[o];
+      #pragma unroll 8
       for (int i = 0; i < isplit; ++i) {
         const double rinv = recip((dd[i] - dob) - tau);
         const double term = zh[i] * rinv;
         psi += term;
         dpsi = fma(term, rinv, dpsi);
       }
+      #pragma unroll 8
       for (int i = isplit; i < k; ++i) {
         const double rinv = recip((dd[i] - dob) - tau);
         const double term = zh[i] * rinv;
@@ -472,6 +516,7 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
   // z from the roots (Loewner), column norms of the secular eigenvectors
   for (int i = t; i < k; i += TH) {
     double pr = (dd[og[k - 1]] - dd[i]) + tz[k - 1];
+    #pragma unroll 8
     for (int j = 0; j < k - 1; ++j) {
       const double num = (dd[og[j]] - dd[i]) + tz[j];
       const double den = (j < i) ? (dd[j] - dd[i]) : (dd[j + 1] - dd[i]);
@@ -483,6 +528,7 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
   __syncthreads();
   for (int j = t; j < k; j += TH) {
     double s = 0.0;
+    #pragma unroll 8
     for (int i = 0; i < k; ++i) {
       const double x = zh[i] / ((dd[i] - dd[og[j]]) - tz[j]);
       s = fma(x, x, s);
@@ -495,6 +541,7 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
   for (int p = t; p < m; p += TH) {
     const double x = ov[p];
     int r = 0;
+    #pragma unroll 8
     for (int q = 0; q < m; ++q) {
       const double y = ov[q];
       r += (y < x) || (y == x && q < p);
@@ -511,6 +558,7 @@ dc_merge_kernel(int n, const int* __restrict__ desc, const double* __restrict__ 
   for (int i = t; i < k; i += TH) {   // row nd[i]: consecutive threads, consecutive rows
     double* ur = Uf + (r0 + nd[i]);
     const double zi = zh[i], di = dd[i];
+    #pragma unroll 8
     for (int j = 0; j < k; ++j)
       ur[(long long)(r0 + fp[nd[j]]) * n] = zi / ((di - dd[og[j]]) - tz[j]) * cn[j];
   }
@@ -570,6 +618,7 @@ wy_t_kernel(int n, const double* __restrict__ Vm, const double* __restrict__ tau
       Vc[rr][a] = (rr < rows && a < nbp) ? Vm[(r0 + rr) + (long long)(k0 + a) * n] : 0.0;
     }
     __syncthreads();
+#pragma unroll 4
     for (int rr = 0; rr < rows; ++rr) {
       const double vc = Vc[rr][cc];
 #pragma unroll
@@ -643,6 +692,7 @@ wy_apply_kernel(int n, int npan, const double* __restrict__ Vm, const double* __
         const int a = g + NG * u;
         if (a < EG_NB) {
           double x = acc[u];
+#pragma unroll 8
           for (int rr = 0; rr < rows; ++rr) x = fma(Vc[rr * LDV + a], Zb[(r0 + rr) * LDZ + c], x);
           acc[u] = x;
         }
@@ -680,6 +730,7 @@ wy_apply_kernel(int n, int npan, const double* __restrict__ Vm, const double* __
       __syncthreads();
       for (int rr = g; rr < rows; rr += NG) {
         double z = Zb[(r0 + rr) * LDZ + c];
+#pragma unroll 8
         for (int a = 0; a < nbp; ++a) z = fma(-Vc[rr * LDV + a], X[a * CB + c], z);
         Zb[(r0 + rr) * LDZ + c] = z;
       }
