@@ -277,6 +277,22 @@ int tg_iterate(tg_ctx* ctx, double gradient_eps, double* pnp_scalars, double* gr
                double* lagrange_lambda1, int* adjust_iterations, int* converged);
 
 /*
+ * tg_iterate_canonical -- tg_canonical_basis of the resident states followed by
+ * tg_iterate: the loop body of src/taurus_vap.f90:77-123 with the canonical basis
+ * of src/module_wavefunctions.f90:777-945 at its start, as project_wavefunction
+ * builds it (src/module_projection.f90:212-234).  The results equal the two calls
+ * in sequence; the canonical basis runs on the context's second stream beside the
+ * projection's Thouless / field / H20 chain (only the overlap chain reads it).
+ * Outputs as tg_iterate and tg_canonical_basis.  TG_E_UNSUPPORTED (a degenerate
+ * kappa block wider than the device form) leaves the state as it was: the host
+ * route (tg_get_state, construct_canonical_basis, tg_set_state) and tg_iterate
+ * follow.
+ */
+int tg_iterate_canonical(tg_ctx* ctx, double canon_eps, double gradient_eps, double* pnp_scalars,
+                         double* gradient_norm, double* lagrange_lambda1, int* adjust_iterations,
+                         int* converged, int* nocc, int* nemp, double* ovacc, double* voveru);
+
+/*
  * tg_set_gradient_type -- gradient_type of the device iteration (tg_iterate):
  * 0 = the eta, mu of tg_set_gradient; 1 / 2 = the heavy-ball recipes of
  * calculate_gradient (src/module_gradient.f90:97-120) from the H11 / h spectra

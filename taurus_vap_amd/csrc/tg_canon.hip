@@ -200,6 +200,15 @@ bool skew_canonical(int m, const std::vector<double>& K, std::vector<double>& T)
 
 extern "C" int tg_canonical_basis(tg_ctx* c, double eps, int* nocc_out, int* nemp_out,
                                   double* ovac_out, double* voveru_out) {
+  if (!c) {
+    tg_set_error("tg_canonical_basis: invalid argument");
+    return TG_E_ARG;
+  }
+  return canonical_basis_on(c, c->stream, eps, nocc_out, nemp_out, ovac_out, voveru_out);
+}
+
+int canonical_basis_on(tg_ctx* c, hipStream_t s, double eps, int* nocc_out, int* nemp_out,
+                       double* ovac_out, double* voveru_out) {
   if (!c || !c->d_U.p || !c->d_V.p || c->ns < 1 || c->Mphip < 1 || !(eps > 0.0 && eps < 0.5)) {
     tg_set_error("tg_canonical_basis: invalid argument (states set once with tg_set_states?)");
     return TG_E_ARG;
@@ -233,7 +242,6 @@ extern "C" int tg_canonical_basis(tg_ctx* c, double eps, int* nocc_out, int* nem
                                            ((base + 7) / 8) * 8);
   const double* U = c->d_U.as<double>();
   const double* V = c->d_V.as<double>();
-  hipStream_t s = c->stream;
 
   // rho = V V^T; D its eigenvectors (upper triangle read, as dsyev('V','U'))
   TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, V, n, nn, V, n, nn, 0.0, R, n, nn, S));

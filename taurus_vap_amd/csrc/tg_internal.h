@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -237,6 +238,7 @@ struct tg_ctx {
   DevBuf d_int;                                   // int work
   DevBuf d_pfw;                                   // pfaffian work
   int have_state = 0;
+  int evolved = 0;   // the states are the last iteration's, their canonical basis not set
   int tuned_n = 0, tuned_chunk = 0, tuned_nb = -1;  // sizes the GEMM tiles were tuned for
   // items per launch sequence: batch_limit from tg_set_batch_limit (0: none),
   // mem_chunk chosen from the free device memory by the upload (0: all items)
@@ -275,6 +277,11 @@ struct tg_ctx {
   std::vector<double> lambda_prev;                // multipliers of the previous iteration (ns x dim)
   // canonical basis on the device (tg_canon.hip): work buffers
   DevBuf d_canon;
+  // tg_iterate_canonical: the projection enqueues its Thouless / field / H20 chain,
+  // runs defer_hook on the host (the canonical basis on the second stream and the
+  // overlap set-up of the new basis), then enqueues the overlap chain
+  int defer_overlap = 0;
+  std::function<int()> defer_hook;
   // timing: event pairs recorded without synchronisation, summed at collection
   struct Pending { std::string key; hipEvent_t e0, e1; };
   int timing = 0;
@@ -326,6 +333,11 @@ void gradient_recipe(int type, int n, const double* eig_hsp, const double* eig_h
 size_t sym_eigensystem_work_bytes(int n, int count);
 int batched_sym_eigensystem(hipStream_t s, int n, const double* A, int lda, long long sA, int count,
                             void* work, double* w, long long ws, double* E, long long sE, int* info);
+// tg_canonical_basis on a given stream (tg_canon.hip); the overlap set-up of the
+// resident states' canonical data uploaded on stream s (tg_project.hip)
+int canonical_basis_on(tg_ctx* c, hipStream_t s, double eps, int* nocc, int* nemp, double* ovacc,
+                       double* voveru);
+int upload_overlap_setup(tg_ctx* c, hipStream_t s);
 int batched_sym_eigvals(hipStream_t s, int n, const double* A, int lda, long long sA, int count,
                         double* work, double* w, long long ws);
 // The resident U, V, D and c->st of c->ns states are set: the tail of

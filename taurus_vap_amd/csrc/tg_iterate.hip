@@ -579,6 +579,7 @@ int iterate_post(tg_ctx* c, double gradient_eps, IterOut& out) {
   TG_HIP(hipGetLastError());
   TG_HIP(hipStreamSynchronize(s));
   c->have_state = 0;   // the canonical basis of the new states is still to be set
+  c->evolved = 1;
   return TG_OK;
 }
 
@@ -776,6 +777,65 @@ int tg_iterate(tg_ctx* c, double gradient_eps, double* pnp_scalars, double* grad
   const int r = tg_project_device(c);
   c->grad_on = grad_was;
   TG_TRY(r);
+  IterOut o;
+  TG_TRY(iterate_post(c, gradient_eps, o));
+  copy_out(c, o, pnp_scalars, gradient_norm, lagrange_lambda1, adjust_iterations, converged);
+  return TG_OK;
+}
+
+int tg_iterate_canonical(tg_ctx* c, double canon_eps, double gradient_eps, double* pnp_scalars,
+                         double* gradient_norm, double* lagrange_lambda1, int* adjust_iterations,
+                         int* converged, int* nocc, int* nemp, double* ovacc, double* voveru) {
+  if (!c) {
+    tg_set_error("tg_iterate_canonical: invalid argument");
+    return TG_E_ARG;
+  }
+  // The canonical basis runs beside the projection when the states are the last
+  // iteration's (the projection's set-up current but for the canonical data) and
+  // the projection runs eagerly in one batch; otherwise one after the other.
+  bool overlap = c->evolved && !c->state_dirty && !c->plain && !c->use_graph && c->grad_on;
+  if (overlap) {
+    const int B = c->ns * c->A;
+    overlap = (c->batch_limit <= 0 || c->batch_limit >= B) && c->mem_chunk <= 0 &&
+              std::max(c->Mphip, c->Mphin) > 1;
+  }
+  if (!overlap) {
+    TG_TRY(tg_canonical_basis(c, canon_eps, nocc, nemp, ovacc, voveru));
+    return tg_iterate(c, gradient_eps, pnp_scalars, gradient_norm, lagrange_lambda1,
+                      adjust_iterations, converged);
+  }
+  // update_constraints_qpbasis and the projection up to its overlaps read U, V
+  // only; the hook runs on the second stream after the projection's fork (the
+  // U, V products are done), before its join
+  c->have_state = 1;
+  if (const int r = iterate_pre(c)) {
+    c->have_state = 0;
+    return r;
+  }
+  c->defer_overlap = 1;
+  c->defer_hook = [&]() -> int {
+    ScopedTiming tm(c, "canonical_basis_beside", c->stream2);
+    TG_TRY(canonical_basis_on(c, c->stream2, canon_eps, nocc, nemp, ovacc, voveru));
+    TG_TRY(upload_overlap_setup(c, c->stream2));
+    c->state_dirty = 0;   // the rest of the projection's set-up did not change
+    return TG_OK;
+  };
+  const int grad_was = c->grad_on;
+  c->grad_on = 0;
+  const int r = tg_project_run(c);
+  c->grad_on = grad_was;
+  c->defer_overlap = 0;
+  c->defer_hook = nullptr;
+  if (r != TG_OK) {
+    // leave nothing in flight (a wide degenerate block, TG_E_UNSUPPORTED, leaves
+    // the state as it was: the caller takes the host route and tg_iterate)
+    (void)hipStreamSynchronize(c->stream2);
+    (void)hipStreamSynchronize(c->stream);
+    c->have_state = 0;
+    c->evolved = 1;
+    c->state_dirty = 1;
+    return r;
+  }
   IterOut o;
   TG_TRY(iterate_post(c, gradient_eps, o));
   copy_out(c, o, pnp_scalars, gradient_norm, lagrange_lambda1, adjust_iterations, converged);

@@ -917,6 +917,30 @@ RunPlan make_plan(tg_ctx* c) {
 // Allocations, uploads of the state-dependent set-up (angles, weights, overlap
 // scaling, ln-norm and prefactor of every state) and, on the first call at a
 // size, the GEMM tile autotuning.  Runs eagerly, never under stream capture.
+// Overlap set-up of every state from its canonical data: per state T (n) | sg (n)
+// | ovac (2) | pre0 (cplx) | v/u (n), as the reduced Pfaffian and the reference
+// sequence read them.
+std::vector<double> overlap_setup(const tg_ctx* c, const RunPlan& P) {
+  const int n = P.n, S = P.S;
+  std::vector<double> ovl((size_t)S * P.os, 0.0);
+  for (int st = 0; st < S; ++st) {
+    const OvlShape& o = P.sh[st];
+    const StateInfo& si = c->st[st];
+    double* Tsc = ovl.data() + (size_t)st * P.os;
+    double* sg = Tsc + n;
+    if (o.reduced)
+      for (int q = 0; q < o.nbloc / 2; ++q) {
+        const double x = si.voveru[o.nemp + 2 * q];
+        sg[q] = (x < 0) ? -1.0 : 1.0;
+        Tsc[2 * q] = Tsc[2 * q + 1] = sqrt(fabs(x));
+      }
+    sg[n] = (o.nbloc == 0) ? 0.0 : si.ovac;                             // d_ovac
+    sg[n + 2] = o.pre;                                                  // d_pre0 (re, im = 0)
+    for (int i = 0; i < o.nbloc; ++i) sg[n + 4 + i] = si.voveru[o.nemp + i];   // d_vou
+  }
+  return ovl;
+}
+
 int project_upload(tg_ctx* c) {
   // When all items do not fit into the free device memory, the per-item work
   // matrices are reused chunk by chunk (A_phi and H20 of every item are kept
@@ -955,22 +979,7 @@ int project_upload(tg_ctx* c) {
     hph[2 * a + 1] = cmk(cos(c->phin[a]), sin(c->phin[a]));
     hw[a] = c->weight[a];
   }
-  std::vector<double> ovl((size_t)S * P.os, 0.0);
-  for (int st = 0; st < S; ++st) {
-    const OvlShape& o = P.sh[st];
-    const StateInfo& si = c->st[st];
-    double* Tsc = ovl.data() + (size_t)st * P.os;
-    double* sg = Tsc + n;
-    if (o.reduced)
-      for (int q = 0; q < o.nbloc / 2; ++q) {
-        const double x = si.voveru[o.nemp + 2 * q];
-        sg[q] = (x < 0) ? -1.0 : 1.0;
-        Tsc[2 * q] = Tsc[2 * q + 1] = sqrt(fabs(x));
-      }
-    sg[n] = (o.nbloc == 0) ? 0.0 : si.ovac;                             // d_ovac
-    sg[n + 2] = o.pre;                                                  // d_pre0 (re, im = 0)
-    for (int i = 0; i < o.nbloc; ++i) sg[n + 4 + i] = si.voveru[o.nemp + i];   // d_vou
-  }
+  std::vector<double> ovl = overlap_setup(c, P);
   TG_HIP(hipMemcpyAsync(P.d_ph, hph.data(), sizeof(cplx) * 2 * A, hipMemcpyHostToDevice, s));
   TG_HIP(hipMemcpyAsync(P.d_w, hw.data(), sizeof(cplx) * A, hipMemcpyHostToDevice, s));
   TG_HIP(hipMemcpyAsync(P.d_mult, c->mult.data(), sizeof(double) * A, hipMemcpyHostToDevice, s));
@@ -1015,6 +1024,39 @@ int project_upload(tg_ctx* c) {
   return TG_OK;
 }
 
+}  // namespace
+
+// The overlap set-up of new canonical data on stream s (the deferred overlap of
+// tg_iterate_canonical): the overlap work buffer for the new shapes and the
+// per-state scaling; the rest of the projection's set-up is unchanged.
+int upload_overlap_setup(tg_ctx* c, hipStream_t s) {
+  RunPlan P = make_plan(c);
+  if (P.pfw_bytes) TG_TRY(c->d_pfw.ensure(P.pfw_bytes));
+  P = make_plan(c);
+  std::vector<double> ovl = overlap_setup(c, P);
+  TG_HIP(hipMemcpyAsync(P.d_T, ovl.data(), sizeof(double) * ovl.size(), hipMemcpyHostToDevice, s));
+  // tile shapes of the reduced-overlap bands for a new band size, timed on the
+  // overlap work buffer (the projection's work matrices are in use meanwhile)
+  const int nb0 = (P.uniform && P.sh[0].reduced && P.need_ovl) ? P.sh[0].nbloc : -1;
+  if (nb0 > 0 && nb0 != c->tuned_nb && q_tile_variant(P.n) < 0) {
+    const int nb = nb0, rb = q_band_rows(nb), chunk = P.chunk;
+    const long long mm = (long long)nb * nb, tot = mm * chunk;
+    cplx* Xa = c->d_pfw.as<cplx>();   // 4 nb^2 per item (reduced shape): X | T | Y
+    cplx* Ta = Xa + tot;
+    cplx* Ya = Ta + tot;
+    hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(2 * tot)), dim3(NT), 0, s, 2 * tot, cmk(0, 0), Xa);
+    TG_HIP(hipGetLastError());
+    for (int r0 = 0; r0 < nb; r0 += rb)
+      TG_TRY(gemm_tune_zz(s, OP_T, OP_N, std::min(rb, nb - r0), nb - r0, nb, Xa + (size_t)r0 * nb, nb,
+                          mm, Ta + (size_t)r0 * nb, nb, mm, Ya + r0 + (size_t)r0 * nb, nb, mm, chunk));
+    c->tuned_nb = nb0;
+  }
+  TG_HIP(hipStreamSynchronize(s));   // the host vector goes out of scope
+  return TG_OK;
+}
+
+namespace {
+
 // The projection proper: kernel launches only (capturable into a hipGraph).
 // Buffers and the uploaded set-up come from project_upload.
 int project_run(tg_ctx* c) {
@@ -1026,7 +1068,6 @@ int project_run(tg_ctx* c) {
   const bool proj = P.proj;
   const double* U = c->d_U.as<double>();
   const double* V = c->d_V.as<double>();
-  const double* D = c->d_D.as<double>();
   double* R6 = c->d_real6.as<double>();      // Kp Kn Lp Ln Mp Mn Pp Pn [parity Pp Pn] [Ds]
   const long long ks = (long long)S * nn;    // stride of one product over the S states
   const cplx one = cmk(1, 0), zero = cmk(0, 0);
@@ -1034,6 +1075,31 @@ int project_run(tg_ctx* c) {
     tg_set_error("projection: a chunk must hold whole states");
     return TG_E_ARG;
   }
+  // tg_iterate_canonical: the canonical basis of the state runs beside the
+  // Thouless / field / H20 chain; everything that reads D0 or the overlap shapes
+  // (the D0 products, the overlap chain) follows it on the second stream
+  const bool defer = c->defer_overlap && c->defer_hook && chunk == B && !c->plain;
+
+  // D0^T Pi D0 per species (R_n = D0^T R D0 of calculate_overlap :854-855) and,
+  // when printing, D0^T diag((-1)^l Pi) D0 (generate_rotation_parity :607-625)
+  auto d_products = [&](hipStream_t st) -> int {
+    const double* D = c->d_D.as<double>();   // written by a deferred canonical basis
+    for (int sp = 0; sp < 2; ++sp) {
+      const int r0 = sp * h;
+      TG_TRY(gemm_dd(st, OP_T, OP_N, n, n, h, 1.0, D + r0, n, nn, D + r0, n, nn, 0.0, R6 + (6 + sp) * ks, n, nn, S));
+    }
+    if (c->iprint) {
+      double* Ds = R6 + 10 * ks;
+      hipLaunchKernelGGL(row_sign_kernel, dim3(nblk(ks)), dim3(NT), 0, st, ks, n,
+                         c->d_psign.as<double>(), D, Ds);
+      TG_HIP(hipGetLastError());
+      for (int sp = 0; sp < 2; ++sp) {
+        const int r0 = sp * h;
+        TG_TRY(gemm_dd(st, OP_T, OP_N, n, n, h, 1.0, D + r0, n, nn, Ds + r0, n, nn, 0.0, R6 + (8 + sp) * ks, n, nn, S));
+      }
+    }
+    return TG_OK;
+  };
 
   // ---- angle-independent real products (once per iteration, every state)
   {
@@ -1043,19 +1109,8 @@ int project_run(tg_ctx* c) {
       TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, U + r0, n, nn, U + r0, n, nn, 0.0, R6 + (0 + sp) * ks, n, nn, S));
       TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, V + r0, n, nn, V + r0, n, nn, 0.0, R6 + (2 + sp) * ks, n, nn, S));
       TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, V + r0, n, nn, U + r0, n, nn, 0.0, R6 + (4 + sp) * ks, n, nn, S));
-      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, D + r0, n, nn, D + r0, n, nn, 0.0, R6 + (6 + sp) * ks, n, nn, S));
     }
-    if (c->iprint) {
-      // D0^T diag((-1)^l Pi_p) D0 and D0^T diag((-1)^l Pi_n) D0 (generate_rotation_parity :607-625)
-      double* Ds = R6 + 10 * ks;
-      hipLaunchKernelGGL(row_sign_kernel, dim3(nblk(ks)), dim3(NT), 0, s, ks, n,
-                         c->d_psign.as<double>(), D, Ds);
-      TG_HIP(hipGetLastError());
-      for (int sp = 0; sp < 2; ++sp) {
-        const int r0 = sp * h;
-        TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, h, 1.0, D + r0, n, nn, Ds + r0, n, nn, 0.0, R6 + (8 + sp) * ks, n, nn, S));
-      }
-    }
+    if (!defer) TG_TRY(d_products(s));
   }
   const double* Pp = R6 + 6 * ks;
   const double* Pn = R6 + 7 * ks;
@@ -1063,12 +1118,109 @@ int project_run(tg_ctx* c) {
   hipStream_t s2 = c->stream2;
   TG_HIP(hipEventRecord(c->ev_fork, s));
   TG_HIP(hipStreamWaitEvent(s2, c->ev_fork, 0));
+  const RunPlan* PP = &P;   // the plan of the overlap chain (the new one when deferred)
+  RunPlan P2;
 
   auto Wb = [&](int k) { return P.work + (size_t)k * chunk * nn; };
   cplx* Akeep = P.Akeep;                     // A_phi of every item
   cplx* Hkeep = P.Hkeep;                     // H20_phi of every item
   const long long os = P.os;
 
+  // overlaps <Phi|R|Phi> for the gauge rotation (rot_over) and, when printing,
+  // for parity x gauge (rot_pari, :279-284).  R_n = D0^T R D0 = ep*Pa + en*Pb.
+  // Items are processed in segments of one overlap shape: the whole chunk when
+  // every state has the same nocc / nemp, else one segment per state.
+  auto overlap_segment = [&](const double* Pa, const double* Pb, long long a0, long long b0, int cnt,
+                             const OvlShape& o, cplx* dst) -> int {
+    const RunPlan& P = *PP;
+    const int lo = (int)(b0 - a0);         // chunk-local offset of the segment
+    if (o.reduced) {
+      ScopedTiming tm(c, "overlap", s2);
+      const int nb = o.nbloc;
+      const long long mm = (long long)nb * nb;
+      cplx* Q = c->d_pfw.as<cplx>();
+      cplx* JQ = Q + (size_t)cnt * mm;
+      cplx* K = JQ + (size_t)cnt * mm;
+      cplx* PW = K + (size_t)cnt * mm;
+      hipLaunchKernelGGL(build_q_kernel, dim3(nblk(mm * cnt)), dim3(NT), 0, s2, n, o.nemp, nb, cnt,
+                         b0, A, Pa, Pb, P.d_T, P.d_sg, os, P.d_ph, Q, JQ);
+      TG_HIP(hipGetLastError());
+      {
+        // K = Q^T (J Q) is skew and the Pfaffian reads its upper triangle only:
+        // the product runs in bands of rows, each from its diagonal block on
+        ScopedTiming tz(c, "kernel:zgemm_q", s2);
+        const int rb = q_band_rows(nb);
+        // The bands run beside the inverse.  The 384-thread inverse (256 < n <= 384)
+        // leaves two SIMDs of each of its CUs half free, which the two-wave 64x32
+        // tile (256 VGPRs, 26 KB LDS) fits; other sizes take the tuned tile.
+        // TG_Q_TILE=k forces variant k (-1: tuned).
+        const int qtile = q_tile_variant(n);
+        for (int r0 = 0; r0 < nb; r0 += rb)
+          TG_TRY(gemm_zz(s2, OP_T, OP_N, std::min(rb, nb - r0), nb - r0, nb, one,
+                         Q + (size_t)r0 * nb, nb, mm, JQ + (size_t)r0 * nb, nb, mm, zero,
+                         K + r0 + (size_t)r0 * nb, nb, mm, cnt, 1, 1, qtile));
+      }
+      // one-wave workgroups: they fit beside the inverse's workgroups, so the
+      // Pfaffians can start on the CUs the inverse leaves free
+      hipLaunchKernelGGL(add_j_kernel, dim3((unsigned)(((long long)(nb / 2) * cnt + 63) / 64)),
+                         dim3(64), 0, s2, nb, cnt, b0, A, P.d_sg, os, K);
+      TG_HIP(hipGetLastError());
+      {
+        ScopedTiming tk(c, "kernel:pfaffian", s2);
+        TG_TRY(batched_pfaffian(s2, nb, K, nb, mm, cnt, PW, P.d_logabs + lo, P.d_phase + lo,
+                                P.d_zero + lo));
+      }
+      hipLaunchKernelGGL(fill_from_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt, b0, A, P.d_pre0,
+                         os / 2, P.d_pre + lo);
+      hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt, b0, A,
+                         P.d_logabs + lo, P.d_phase + lo, P.d_zero + lo, P.d_pre + lo, P.d_ovac,
+                         os, dst + lo);
+      TG_HIP(hipGetLastError());
+      return TG_OK;
+    }
+    // reference sequence: R2 -> inverse + det -> M_reg -> Pfaffian(nreg)
+    ScopedTiming tm(c, "overlap", s2);
+    const int nreg = 2 * o.nbloc, n2 = o.n2;
+    const long long m2 = (long long)n2 * n2, mr = (long long)nreg * nreg;
+    cplx* R2 = c->d_pfw.as<cplx>();
+    cplx* M = R2 + (size_t)cnt * m2;
+    cplx* PW = M + (size_t)cnt * mr;
+    cplx* det2 = PW + (size_t)cnt * mr;
+    int* info2 = reinterpret_cast<int*>(det2 + cnt);
+    int* perm2 = info2 + cnt;
+    hipLaunchKernelGGL(build_rn2_kernel, dim3(nblk(m2 * cnt)), dim3(NT), 0, s2, n, o.nemp, n2, cnt,
+                       b0, A, Pa, Pb, P.d_ph, R2);
+    TG_HIP(hipGetLastError());
+    TG_TRY(batched_inverse(s2, n2, R2, n2, m2, cnt, det2, info2, perm2));
+    if (o.nbloc > 0) {
+      hipLaunchKernelGGL(build_mreg_kernel, dim3(nblk(mr * cnt)), dim3(NT), 0, s2, n2, o.nbloc,
+                         o.nbloc, nreg, cnt, b0, A, P.d_vou, P.d_vou, os, R2, M);
+      TG_HIP(hipGetLastError());
+      TG_TRY(batched_pfaffian(s2, nreg, M, nreg, mr, cnt, PW, P.d_logabs + lo, P.d_phase + lo,
+                              P.d_zero + lo));
+    } else {
+      hipLaunchKernelGGL(unit_pfaffian_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt,
+                         P.d_logabs + lo, P.d_phase + lo, P.d_zero + lo);
+      TG_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(scale_det_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt,
+                       (double)o.signfac, det2, P.d_pre + lo);
+    hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt, b0, A,
+                       P.d_logabs + lo, P.d_phase + lo, P.d_zero + lo, P.d_pre + lo, P.d_ovac, os,
+                       dst + lo);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+  };
+  auto overlap = [&](const double* Pa, const double* Pb, long long a0, int na, cplx* dst) -> int {
+    const RunPlan& P = *PP;
+    if (P.uniform) return overlap_segment(Pa, Pb, a0, a0, na, P.sh[0], dst);
+    for (int st = (int)(a0 / A); st <= (int)((a0 + na - 1) / A); ++st) {
+      const long long b0 = std::max<long long>((long long)st * A, a0);
+      const long long b1 = std::min<long long>((long long)(st + 1) * A, a0 + na);
+      TG_TRY(overlap_segment(Pa, Pb, a0, b0, (int)(b1 - b0), P.sh[st], dst));
+    }
+    return TG_OK;
+  };
   // ---- loop over item chunks (one chunk of all items unless plain)
   for (int a0 = 0; a0 < B; a0 += chunk) {
     const int na = std::min(chunk, B - a0);
@@ -1090,107 +1242,14 @@ int project_run(tg_ctx* c) {
     cplx* Z = Wb(W_Z);
     cplx* H20 = Hkeep + (size_t)a0 * nn;
 
-    // overlaps <Phi|R|Phi> for the gauge rotation (rot_over) and, when printing,
-    // for parity x gauge (rot_pari, :279-284).  R_n = D0^T R D0 = ep*Pa + en*Pb.
-    // Items are processed in segments of one overlap shape: the whole chunk when
-    // every state has the same nocc / nemp, else one segment per state.
-    auto overlap_segment = [&](const double* Pa, const double* Pb, long long b0, int cnt,
-                               const OvlShape& o, cplx* dst) -> int {
-      const int lo = (int)(b0 - a0);         // chunk-local offset of the segment
-      if (o.reduced) {
-        ScopedTiming tm(c, "overlap", s2);
-        const int nb = o.nbloc;
-        const long long mm = (long long)nb * nb;
-        cplx* Q = c->d_pfw.as<cplx>();
-        cplx* JQ = Q + (size_t)cnt * mm;
-        cplx* K = JQ + (size_t)cnt * mm;
-        cplx* PW = K + (size_t)cnt * mm;
-        hipLaunchKernelGGL(build_q_kernel, dim3(nblk(mm * cnt)), dim3(NT), 0, s2, n, o.nemp, nb, cnt,
-                           b0, A, Pa, Pb, P.d_T, P.d_sg, os, P.d_ph, Q, JQ);
-        TG_HIP(hipGetLastError());
-        {
-          // K = Q^T (J Q) is skew and the Pfaffian reads its upper triangle only:
-          // the product runs in bands of rows, each from its diagonal block on
-          ScopedTiming tz(c, "kernel:zgemm_q", s2);
-          const int rb = q_band_rows(nb);
-          // The bands run beside the inverse.  The 384-thread inverse (256 < n <= 384)
-          // leaves two SIMDs of each of its CUs half free, which the two-wave 64x32
-          // tile (256 VGPRs, 26 KB LDS) fits; other sizes take the tuned tile.
-          // TG_Q_TILE=k forces variant k (-1: tuned).
-          const int qtile = q_tile_variant(n);
-          for (int r0 = 0; r0 < nb; r0 += rb)
-            TG_TRY(gemm_zz(s2, OP_T, OP_N, std::min(rb, nb - r0), nb - r0, nb, one,
-                           Q + (size_t)r0 * nb, nb, mm, JQ + (size_t)r0 * nb, nb, mm, zero,
-                           K + r0 + (size_t)r0 * nb, nb, mm, cnt, 1, 1, qtile));
-        }
-        // one-wave workgroups: they fit beside the inverse's workgroups, so the
-        // Pfaffians can start on the CUs the inverse leaves free
-        hipLaunchKernelGGL(add_j_kernel, dim3((unsigned)(((long long)(nb / 2) * cnt + 63) / 64)),
-                           dim3(64), 0, s2, nb, cnt, b0, A, P.d_sg, os, K);
-        TG_HIP(hipGetLastError());
-        {
-          ScopedTiming tk(c, "kernel:pfaffian", s2);
-          TG_TRY(batched_pfaffian(s2, nb, K, nb, mm, cnt, PW, P.d_logabs + lo, P.d_phase + lo,
-                                  P.d_zero + lo));
-        }
-        hipLaunchKernelGGL(fill_from_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt, b0, A, P.d_pre0,
-                           os / 2, P.d_pre + lo);
-        hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt, b0, A,
-                           P.d_logabs + lo, P.d_phase + lo, P.d_zero + lo, P.d_pre + lo, P.d_ovac,
-                           os, dst + lo);
-        TG_HIP(hipGetLastError());
-        return TG_OK;
-      }
-      // reference sequence: R2 -> inverse + det -> M_reg -> Pfaffian(nreg)
-      ScopedTiming tm(c, "overlap", s2);
-      const int nreg = 2 * o.nbloc, n2 = o.n2;
-      const long long m2 = (long long)n2 * n2, mr = (long long)nreg * nreg;
-      cplx* R2 = c->d_pfw.as<cplx>();
-      cplx* M = R2 + (size_t)cnt * m2;
-      cplx* PW = M + (size_t)cnt * mr;
-      cplx* det2 = PW + (size_t)cnt * mr;
-      int* info2 = reinterpret_cast<int*>(det2 + cnt);
-      int* perm2 = info2 + cnt;
-      hipLaunchKernelGGL(build_rn2_kernel, dim3(nblk(m2 * cnt)), dim3(NT), 0, s2, n, o.nemp, n2, cnt,
-                         b0, A, Pa, Pb, P.d_ph, R2);
-      TG_HIP(hipGetLastError());
-      TG_TRY(batched_inverse(s2, n2, R2, n2, m2, cnt, det2, info2, perm2));
-      if (o.nbloc > 0) {
-        hipLaunchKernelGGL(build_mreg_kernel, dim3(nblk(mr * cnt)), dim3(NT), 0, s2, n2, o.nbloc,
-                           o.nbloc, nreg, cnt, b0, A, P.d_vou, P.d_vou, os, R2, M);
-        TG_HIP(hipGetLastError());
-        TG_TRY(batched_pfaffian(s2, nreg, M, nreg, mr, cnt, PW, P.d_logabs + lo, P.d_phase + lo,
-                                P.d_zero + lo));
-      } else {
-        hipLaunchKernelGGL(unit_pfaffian_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt,
-                           P.d_logabs + lo, P.d_phase + lo, P.d_zero + lo);
-        TG_HIP(hipGetLastError());
-      }
-      hipLaunchKernelGGL(scale_det_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt,
-                         (double)o.signfac, det2, P.d_pre + lo);
-      hipLaunchKernelGGL(overlap_finish_kernel, dim3(nblk(cnt)), dim3(NT), 0, s2, cnt, b0, A,
-                         P.d_logabs + lo, P.d_phase + lo, P.d_zero + lo, P.d_pre + lo, P.d_ovac, os,
-                         dst + lo);
-      TG_HIP(hipGetLastError());
-      return TG_OK;
-    };
-    auto overlap = [&](const double* Pa, const double* Pb, cplx* dst) -> int {
-      if (P.uniform) return overlap_segment(Pa, Pb, a0, na, P.sh[0], dst);
-      for (int st = a0 / A; st <= (a0 + na - 1) / A; ++st) {
-        const long long b0 = std::max<long long>((long long)st * A, a0);
-        const long long b1 = std::min<long long>((long long)(st + 1) * A, a0 + na);
-        TG_TRY(overlap_segment(Pa, Pb, b0, (int)(b1 - b0), P.sh[st], dst));
-      }
-      return TG_OK;
-    };
     if (!proj) {
       hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(na)), dim3(NT), 0, s, na, cmk(1.0, 0.0),
                          P.d_over + a0);
       TG_HIP(hipGetLastError());
-    } else {
-      TG_TRY(overlap(Pp, Pn, P.d_over + a0));
+    } else if (!defer) {
+      TG_TRY(overlap(Pp, Pn, a0, na, P.d_over + a0));
     }
-    if (c->iprint) TG_TRY(overlap(R6 + 8 * ks, R6 + 9 * ks, P.d_pari + a0));
+    if (c->iprint && !defer) TG_TRY(overlap(R6 + 8 * ks, R6 + 9 * ks, a0, na, P.d_pari + a0));
 
     // Thouless matrix A_phi (calculate_thouless :684-718)
     {
@@ -1256,6 +1315,17 @@ int project_run(tg_ctx* c) {
                          n, Z, H20);
       TG_HIP(hipGetLastError());
     }
+  }
+
+  if (defer) {
+    // the canonical basis (second stream, host waits on that stream only), the
+    // overlap set-up of the new basis, then the overlap chain with its shapes
+    TG_TRY(c->defer_hook());
+    P2 = make_plan(c);
+    PP = &P2;
+    TG_TRY(d_products(s2));
+    if (proj) TG_TRY(overlap(Pp, Pn, 0, B, P.d_over));
+    if (c->iprint) TG_TRY(overlap(R6 + 8 * ks, R6 + 9 * ks, 0, B, P.d_pari));
   }
 
   // join the overlaps of the second stream
@@ -1540,6 +1610,7 @@ int tg_states_ready(tg_ctx* c) {
   if (c->grad_on) TG_TRY(tg_ensure_z(c));
   assign_angles(c, c->conj_pairs != 0);
   c->have_state = 1;
+  c->evolved = 0;
   c->state_dirty = 1;
   return TG_OK;
 }

@@ -41,7 +41,7 @@ EXPORTS = [
     "tg_group_iterate", "tg_group_iterate_results", "tg_project_observables",
     "tg_set_batch_limit", "tg_canonical_basis", "tg_set_gradient_type",
     "tg_get_hsp_h11_eigenvalues", "tg_diagonalize_hsp_and_H11", "tg_sym_eigenvalues",
-    "tg_zskpfa_noalloc", "tg_inverse", "tg_sym_eigensystem",
+    "tg_zskpfa_noalloc", "tg_inverse", "tg_sym_eigensystem", "tg_iterate_canonical",
 ]
 
 _lib = None
@@ -97,6 +97,7 @@ def load_library():
     lib.tg_project_observables.argtypes = [P, I, P, I, P, I, P, P, P]
     lib.tg_set_batch_limit.argtypes = [P, I]
     lib.tg_canonical_basis.argtypes = [P, D, P, P, P, P]
+    lib.tg_iterate_canonical.argtypes = [P, D, D, P, P, P, P, P, P, P, P, P]
     lib.tg_set_gradient_type.argtypes = [P, I]
     lib.tg_get_hsp_h11_eigenvalues.argtypes = [P, I, P, P, P, P]
     lib.tg_diagonalize_hsp_and_H11.argtypes = [P, I, P, P, P, P, P, P, P]
@@ -332,6 +333,24 @@ class Context:
                "tg_iterate")
         out = self._iterate_dicts(*bufs)
         return out[0] if self.nstates == 1 else out
+
+    def iterate_canonical(self, canon_eps=1.0e-8, gradient_eps=0.0):
+        """canonical_basis(canon_eps) followed by iterate(gradient_eps) in one call
+        (tg_iterate_canonical); the canonical basis runs beside the projection's
+        Thouless / field chain.  Returns (iterate results, canonical_basis list)."""
+        n, S = self.n, self.nstates
+        nocc = np.zeros(S, np.int32)
+        nemp = np.zeros(S, np.int32)
+        ovac = np.zeros(S)
+        vou = np.zeros(S * n)
+        bufs = self._iterate_buffers()
+        _check(load_library().tg_iterate_canonical(self._h, float(canon_eps), float(gradient_eps),
+                                                   *[_p(b) for b in bufs], _p(nocc), _p(nemp),
+                                                   _p(ovac), _p(vou)), "tg_iterate_canonical")
+        out = self._iterate_dicts(*bufs)
+        canon = [dict(nocc=int(nocc[k]), nemp=int(nemp[k]), ovac=float(ovac[k]),
+                      voveru=vou[k * n:k * n + n - int(nocc[k])].copy()) for k in range(S)]
+        return (out[0] if S == 1 else out), canon
 
     def set_gradient_type(self, gradient_type):
         """gradient_type of the device iteration: 0 (eta, mu as given), 1 / 2 (the

@@ -166,3 +166,54 @@ def test_wide_degenerate_subspace():
     assert abs(d["ovac"] - hc["ovac"]) <= 1e-9 * abs(hc["ovac"])
     assert np.allclose(np.abs(d["voveru"]), np.abs(hc["voveru"]), rtol=1e-9, atol=0)
     _same_projection(dev, hst)
+
+
+@pytest.mark.parametrize("case", ["batch_sd", "batch_pf"])
+def test_iterate_canonical_matches_sequential_calls(case):
+    """tg_iterate_canonical (canonical basis beside the projection) against
+    tg_canonical_basis + tg_iterate on a second context from the same states:
+    every iteration's results and canonical data agree to rounding, both follow
+    the reference histories, and the canonical basis ran on the second stream in
+    every iteration after the first (the first projection uploads its set-up)."""
+    from taurus_vap_amd import host
+    from test_gpu_batched import _check_iteration, _prepare
+    g = load_golden(case)
+    n = int(g["ndim"])
+    states = []
+    for k in range(len(g["betas"])):
+        U, V = np.asarray(g["U0"][k]), np.asarray(g["V0"][k])
+        states.append((U, V, host.construct_canonical_basis(U, V)))
+    seq, ovl = _ctx(g), _ctx(g)
+    try:
+        for ctx in (seq, ovl):
+            _prepare(ctx, g)
+            ctx.set_states(states, *_setup(g))
+        ovl.enable_timing(True)
+        niter = int(g["niter"])
+        for it in range(niter):
+            print("%s iteration %d" % (case, it + 1), flush=True)
+            if it == 0:
+                rs, rc = seq.iterate(0.0), ovl.iterate(0.0)
+            else:
+                cs = seq.canonical_basis()
+                rs = seq.iterate(0.0)
+                rc, cc = ovl.iterate_canonical()
+                for a, b in zip(cs, cc):
+                    assert (a["nocc"], a["nemp"]) == (b["nocc"], b["nemp"])
+                    assert abs(a["ovac"] - b["ovac"]) <= 1e-12 * max(1.0, abs(a["ovac"]))
+                    assert np.max(np.abs(a["voveru"] - b["voveru"]), initial=0.0) <= 1e-9
+            for k, (a, b) in enumerate(zip(rs, rc)):
+                _check_iteration(g, k, it, b)
+                for key in ("over", "ener", "prot", "neut", "prot2", "neut2"):
+                    assert abs(a[key] - b[key]) <= 1e-11 * max(1.0, abs(a[key])), (it, k, key)
+                assert abs(a["gradient_norm"] - b["gradient_norm"]) <= 1e-11 * a["gradient_norm"]
+                assert np.allclose(a["lagrange_lambda1"], b["lagrange_lambda1"], rtol=1e-10,
+                                   atol=1e-12)
+                assert a["adjust_iterations"] == b["adjust_iterations"]
+        assert ovl.timing("canonical_basis_beside")[1] == niter - 1
+        (Us, Vs), (Uo, Vo) = seq.get_state(), ovl.get_state()
+        for a, b in zip(Us + Vs, Uo + Vo):
+            assert np.max(np.abs(a - b)) <= 1e-11
+    finally:
+        seq.close()
+        ovl.close()
