@@ -207,17 +207,20 @@ extern "C" int tg_canonical_basis(tg_ctx* c, double eps, int* nocc_out, int* nem
   return canonical_basis_on(c, c->stream, eps, nocc_out, nemp_out, ovac_out, voveru_out);
 }
 
-int canonical_basis_on(tg_ctx* c, hipStream_t s, double eps, int* nocc_out, int* nemp_out,
-                       double* ovac_out, double* voveru_out) {
-  if (!c || !c->d_U.p || !c->d_V.p || c->ns < 1 || c->Mphip < 1 || !(eps > 0.0 && eps < 0.5)) {
-    tg_set_error("tg_canonical_basis: invalid argument (states set once with tg_set_states?)");
-    return TG_E_ARG;
-  }
+namespace {
+
+// Work areas of the canonical basis in c->d_canon: R (rho, then rho D), E
+// (eigenvectors D), T1, T2, K (kappac), W, diag, kd, then ints lohi, reach,
+// info, then the block transformations, then the eigen-solver's workspace.
+struct CanonWork {
+  double *R, *E, *T1, *T2, *K, *W, *dg, *kd, *Tdev;
+  int *lohi, *reach, *info;
+  void* eig;
+};
+
+int canon_work(tg_ctx* c, CanonWork& w) {
   const int n = c->n, S = c->ns;
   const long long nn = (long long)n * n;
-  // work: R (rho, then rho D), E (eigenvectors D), T1, T2, K (kappac), W, diag,
-  // kd, then ints lohi, reach, info, then the block transformations, then the
-  // eigen-solver's workspace
   const size_t mats = (size_t)5 * S * nn;
   const size_t vecs = (size_t)2 * S * n;
   const size_t ints = 2 * (size_t)S + (size_t)S * n + (size_t)S;
@@ -227,34 +230,73 @@ int canonical_basis_on(tg_ctx* c, hipStream_t s, double eps, int* nocc_out, int*
   const size_t eoff = ((base + 8 + sizeof(double) * tmax + 255) / 256) * 256;
   TG_TRY(c->d_canon.ensure(eoff + sym_eigensystem_work_bytes(n, S)));
   TG_TRY(c->d_D.ensure(sizeof(double) * nn * S));
-  double* R = c->d_canon.as<double>();
-  double* E = R + S * nn;
-  double* T1 = E + S * nn;
-  double* T2 = T1 + S * nn;
-  double* K = T2 + S * nn;
-  double* W = K + S * nn;             // eigenvalues
-  double* dg = W + (size_t)S * n;     // diag(D^T rho D)
-  double* kd = dg + (size_t)S * n;    // partner entries, 2 per column
-  int* lohi = reinterpret_cast<int*>(kd + 2 * (size_t)S * n);
-  int* reach = lohi + 2 * S;
-  int* info = reach + (size_t)S * n;
-  double* Tdev = reinterpret_cast<double*>(reinterpret_cast<char*>(c->d_canon.p) +
-                                           ((base + 7) / 8) * 8);
+  w.R = c->d_canon.as<double>();
+  w.E = w.R + S * nn;
+  w.T1 = w.E + S * nn;
+  w.T2 = w.T1 + S * nn;
+  w.K = w.T2 + S * nn;
+  w.W = w.K + S * nn;                    // eigenvalues
+  w.dg = w.W + (size_t)S * n;            // diag(D^T rho D)
+  w.kd = w.dg + (size_t)S * n;           // partner entries, 2 per column
+  w.lohi = reinterpret_cast<int*>(w.kd + 2 * (size_t)S * n);
+  w.reach = w.lohi + 2 * S;
+  w.info = w.reach + (size_t)S * n;
+  w.Tdev = reinterpret_cast<double*>(reinterpret_cast<char*>(c->d_canon.p) + ((base + 7) / 8) * 8);
+  w.eig = static_cast<char*>(c->d_canon.p) + eoff;
+  return TG_OK;
+}
+
+}  // namespace
+
+// The device part of the canonical basis up to its first host read, on stream
+// s: rho, its eigensystem, diag(D^T rho D) and kappac = D^T (V U^T) D.
+int canonical_launch(tg_ctx* c, hipStream_t s, double eps) {
+  if (!c || !c->d_U.p || !c->d_V.p || c->ns < 1 || c->Mphip < 1 || !(eps > 0.0 && eps < 0.5)) {
+    tg_set_error("tg_canonical_basis: invalid argument (states set once with tg_set_states?)");
+    return TG_E_ARG;
+  }
+  const int n = c->n, S = c->ns;
+  const long long nn = (long long)n * n;
+  CanonWork w;
+  TG_TRY(canon_work(c, w));
   const double* U = c->d_U.as<double>();
   const double* V = c->d_V.as<double>();
-
   // rho = V V^T; D its eigenvectors (upper triangle read, as dsyev('V','U'))
-  TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, V, n, nn, V, n, nn, 0.0, R, n, nn, S));
-  TG_TRY(batched_sym_eigensystem(s, n, R, n, nn, S, static_cast<char*>(c->d_canon.p) + eoff, W, n, E,
-                                 nn, info));
+  TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, V, n, nn, V, n, nn, 0.0, w.R, n, nn, S));
+  TG_TRY(batched_sym_eigensystem(s, n, w.R, n, nn, S, w.eig, w.W, n, w.E, nn, w.info));
   // rhoc diagonal, kappac = D^T (V U^T) D
-  TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, R, n, nn, E, n, nn, 0.0, T1, n, nn, S));
-  hipLaunchKernelGGL(diag_kernel, dim3(nblk(n), S), dim3(NT), 0, s, n, nn, E, T1, dg);
+  TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, w.R, n, nn, w.E, n, nn, 0.0, w.T1, n, nn, S));
+  hipLaunchKernelGGL(diag_kernel, dim3(nblk(n), S), dim3(NT), 0, s, n, nn, w.E, w.T1, w.dg);
   TG_HIP(hipGetLastError());
-  TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, V, n, nn, U, n, nn, 0.0, T1, n, nn, S));
-  TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, T1, n, nn, E, n, nn, 0.0, T2, n, nn, S));
-  TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, n, 1.0, E, n, nn, T2, n, nn, 0.0, K, n, nn, S));
+  TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, V, n, nn, U, n, nn, 0.0, w.T1, n, nn, S));
+  TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, w.T1, n, nn, w.E, n, nn, 0.0, w.T2, n, nn, S));
+  TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, n, 1.0, w.E, n, nn, w.T2, n, nn, 0.0, w.K, n, nn, S));
+  return TG_OK;
+}
 
+int canonical_basis_on(tg_ctx* c, hipStream_t s, double eps, int* nocc_out, int* nemp_out,
+                       double* ovac_out, double* voveru_out) {
+  TG_TRY(canonical_launch(c, s, eps));
+  return canonical_finish(c, s, eps, nocc_out, nemp_out, ovac_out, voveru_out);
+}
+
+// The rest, after canonical_launch on the same stream s: occupations, the
+// degenerate blocks' canonical form (host), partner entries and v/u; the new D
+// into the resident D.  The host waits on stream s only.
+int canonical_finish(tg_ctx* c, hipStream_t s, double eps, int* nocc_out, int* nemp_out,
+                     double* ovac_out, double* voveru_out) {
+  const int n = c->n, S = c->ns;
+  const long long nn = (long long)n * n;
+  CanonWork w;
+  TG_TRY(canon_work(c, w));
+  double* E = w.E;
+  double* K = w.K;
+  double* dg = w.dg;
+  double* kd = w.kd;
+  int* lohi = w.lohi;
+  int* reach = w.reach;
+  int* info = w.info;
+  double* Tdev = w.Tdev;
   std::vector<double> v2((size_t)S * n);
   std::vector<int> hinfo(S);
   TG_HIP(hipMemcpyAsync(v2.data(), dg, sizeof(double) * S * n, hipMemcpyDeviceToHost, s));

@@ -263,6 +263,7 @@ struct tg_ctx {
   int grad_ncons = 0;
   std::vector<double> grad_lambda;
   DevBuf d_c20, d_Z;                              // constraint_20; Zi, Zim1, lambda, norm
+  DevBuf d_cq;                                    // products of the constraint_20 of one state
   int c20_shared = 1;                             // constraint_20 / lambda common to all states
   int z_states = 0;                               // states the Z buffers hold
   // constraints of the device iteration (tg_set_constraints / tg_iterate)
@@ -277,11 +278,13 @@ struct tg_ctx {
   std::vector<double> lambda_prev;                // multipliers of the previous iteration (ns x dim)
   // canonical basis on the device (tg_canon.hip): work buffers
   DevBuf d_canon;
-  // tg_iterate_canonical: the projection enqueues its Thouless / field / H20 chain,
-  // runs defer_hook on the host (the canonical basis on the second stream and the
-  // overlap set-up of the new basis), then enqueues the overlap chain
+  // tg_iterate_canonical: the projection runs defer_hook(0) at its fork (the
+  // canonical basis's device work enqueued on the second stream), enqueues its
+  // Thouless / field / H20 chain, runs defer_hook(1) on the host (the rest of the
+  // canonical basis and the overlap set-up of the new basis), then enqueues the
+  // overlap chain
   int defer_overlap = 0;
-  std::function<int()> defer_hook;
+  std::function<int(int)> defer_hook;
   // timing: event pairs recorded without synchronisation, summed at collection
   struct Pending { std::string key; hipEvent_t e0, e1; };
   int timing = 0;
@@ -333,10 +336,15 @@ void gradient_recipe(int type, int n, const double* eig_hsp, const double* eig_h
 size_t sym_eigensystem_work_bytes(int n, int count);
 int batched_sym_eigensystem(hipStream_t s, int n, const double* A, int lda, long long sA, int count,
                             void* work, double* w, long long ws, double* E, long long sE, int* info);
-// tg_canonical_basis on a given stream (tg_canon.hip); the overlap set-up of the
-// resident states' canonical data uploaded on stream s (tg_project.hip)
+// tg_canonical_basis on a given stream (tg_canon.hip), also in two parts: the
+// device work up to the first host read (canonical_launch, no host wait) and
+// the rest (canonical_finish, same stream); the overlap set-up of the resident
+// states' canonical data uploaded on stream s (tg_project.hip)
 int canonical_basis_on(tg_ctx* c, hipStream_t s, double eps, int* nocc, int* nemp, double* ovacc,
                        double* voveru);
+int canonical_launch(tg_ctx* c, hipStream_t s, double eps);
+int canonical_finish(tg_ctx* c, hipStream_t s, double eps, int* nocc, int* nemp, double* ovacc,
+                     double* voveru);
 int upload_overlap_setup(tg_ctx* c, hipStream_t s);
 int batched_sym_eigvals(hipStream_t s, int n, const double* A, int lda, long long sA, int count,
                         double* work, double* w, long long ws);

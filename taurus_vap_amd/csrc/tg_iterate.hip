@@ -130,63 +130,60 @@ __global__ void fill_int_kernel(int len, int v, int* __restrict__ x) {
   if (i < len) x[i] = v;
 }
 
-// Diagonal block of the blocked Cholesky factorisation (one wave per state,
-// blockIdx.x; matrices at + s n^2): factors the b x b block A(k0:, k0:) (lower
-// triangle read, b <= CB) in LDS, writes it to L (zeros above the diagonal)
-// and its inverse to X, which is the diagonal block of L^-1.  A non-positive
-// pivot is replaced by 1 and its column (1-based) recorded in info[s] if none
-// was before.
+// Diagonal block of the blocked Cholesky factorisation (one workgroup per
+// state, blockIdx.x; matrices at + s n^2): factors the b x b block A(k0:, k0:)
+// (lower triangle read, b <= CB) in LDS, writes it to L (zeros above the
+// diagonal) and its inverse to X, which is the diagonal block of L^-1.  A
+// non-positive pivot is replaced by 1 and its column (1-based) recorded in
+// info[s] if none was before.
+// One thread per element (row ti, column tj).  Step k takes column k of the
+// factor: each thread scales the two column-k entries its update needs, so
+// column k of `a` is only read in that step and the factor column goes to lf.
+// One step behind, the forward substitution L X = I takes row k-1 of X (final
+// as x(k-1, :) / L(k-1, k-1)) off the rows below.  One barrier per step.
 constexpr int CB = 32;
-static_assert(CB == 32, "chol_block_kernel maps 64 threads onto 32 rows x 2 column parities");
-__global__ void __launch_bounds__(64)
+static_assert(CB == 32, "chol_block_kernel maps its 1024 threads onto 32 rows x 32 columns");
+__global__ void __launch_bounds__(CB * CB)
 chol_block_kernel(int n, int k0, int b, const double* __restrict__ A0, double* __restrict__ L0,
                   double* __restrict__ X0, int* __restrict__ info) {
-  __shared__ double a[CB][CB + 1], x[CB][CB + 1];
+  __shared__ double a[CB][CB + 1], lf[CB][CB + 1], x[CB][CB + 1];
   const long long nn = (long long)n * n;
   const double* A = A0 + blockIdx.x * nn;
   double* L = L0 + blockIdx.x * nn;
   double* X = X0 + blockIdx.x * nn;
-  const int t = threadIdx.x;
-  // 64 threads: row t & 31 of the columns of parity t >> 5 (no integer division)
-  const int ti = t & 31, tj = t >> 5;
-  for (int j = tj; j < b; j += 2)
-    if (ti < b) a[ti][j] = (ti >= j) ? A[(k0 + ti) + (long long)(k0 + j) * n] : 0.0;
+  const int ti = threadIdx.x & (CB - 1), tj = threadIdx.x >> 5;
+  const bool in = ti < b && tj < b;
+  a[ti][tj] = (in && ti >= tj) ? A[(k0 + ti) + (long long)(k0 + tj) * n] : 0.0;
+  lf[ti][tj] = 0.0;
+  x[ti][tj] = (in && ti == tj) ? 1.0 : 0.0;
   __syncthreads();
   int bad = 0;
-  for (int kk = 0; kk < b; ++kk) {
-    double d = a[kk][kk];
-    if (!(d > 0.0)) {
-      if (bad == 0) bad = k0 + kk + 1;
-      d = 1.0;
+  for (int k = 0; k <= b; ++k) {
+    if (k < b) {
+      double d = a[k][k];
+      if (!(d > 0.0)) {
+        if (bad == 0) bad = k0 + k + 1;
+        d = 1.0;
+      }
+      d = sqrt(d);
+      if (tj > k && ti >= tj && ti < b) {
+        const double li = a[ti][k] / d, lj = a[tj][k] / d;
+        a[ti][tj] -= li * lj;
+      }
+      if (tj == k && ti >= k && ti < b) lf[ti][k] = (ti == k) ? d : a[ti][k] / d;
     }
-    d = sqrt(d);
-    __syncthreads();
-    if (t == 0) a[kk][kk] = d;
-    for (int i = kk + 1 + t; i < b; i += 64) a[i][kk] /= d;
-    __syncthreads();
-    if (ti > kk && ti < b)
-      for (int j = kk + 1 + tj; j <= ti; j += 2) a[ti][j] -= a[ti][kk] * a[j][kk];
+    if (k > 0) {
+      const int q = k - 1;
+      if (tj <= q && ti > q && ti < b) x[ti][tj] -= lf[ti][q] * (x[q][tj] / lf[q][q]);
+    }
     __syncthreads();
   }
-  // inverse of the lower-triangular block: thread j solves column j
-  if (t < b) {
-    const int j = t;
-    for (int i = 0; i < j; ++i) x[i][j] = 0.0;
-    x[j][j] = 1.0 / a[j][j];
-    for (int i = j + 1; i < b; ++i) {
-      double sum = 0.0;
-      for (int k = j; k < i; ++k) sum = fma(a[i][k], x[k][j], sum);
-      x[i][j] = -sum / a[i][i];
-    }
+  if (in) {
+    const long long o = (k0 + ti) + (long long)(k0 + tj) * n;
+    L[o] = (ti >= tj) ? lf[ti][tj] : 0.0;
+    X[o] = (ti >= tj) ? x[ti][tj] / lf[ti][ti] : 0.0;
   }
-  __syncthreads();
-  for (int j = tj; j < b; j += 2)
-    if (ti < b) {
-      const long long o = (k0 + ti) + (long long)(k0 + j) * n;
-      L[o] = (ti >= j) ? a[ti][j] : 0.0;
-      X[o] = x[ti][j];
-    }
-  if (t == 0 && bad && info[blockIdx.x] == 0) info[blockIdx.x] = bad;
+  if (threadIdx.x == 0 && bad && info[blockIdx.x] == 0) info[blockIdx.x] = bad;
 }
 
 // Gaussian elimination with partial pivoting of a small dense system (host);
@@ -247,22 +244,53 @@ int constraints_qp(const Iter& it, const double* U, const double* V, double* c20
   tg_ctx* c = it.c;
   const int n = it.n, S = it.S, dim = c->cons_dim;
   const long long nn = it.nn;
-  double* X = it.M(IT_X);
-  double* T1 = it.M(IT_T1);
-  double* T2 = it.M(IT_T2);
-  for (int m = 0; m < dim; ++m) {
-    const double* O = c->d_cHO.as<double>() + (size_t)m * nn;
-    const bool pair = m >= dim - c->cons_pair;
-    // f20: T1 = U^T O V, T2 = V^T O^T U;  g20: T1 = U^T O U, T2 = V^T O V
-    TG_TRY(gemm_dd(it.s, OP_N, OP_N, n, n, n, 1.0, O, n, 0, pair ? U : V, n, nn, 0.0, X, n, nn, S));
-    TG_TRY(gemm_dd(it.s, OP_T, OP_N, n, n, n, 1.0, U, n, nn, X, n, nn, 0.0, T1, n, nn, S));
-    if (pair)
-      TG_TRY(gemm_dd(it.s, OP_N, OP_N, n, n, n, 1.0, O, n, 0, V, n, nn, 0.0, X, n, nn, S));
-    else
-      TG_TRY(gemm_dd(it.s, OP_T, OP_N, n, n, n, 1.0, O, n, 0, U, n, nn, 0.0, X, n, nn, S));
-    TG_TRY(gemm_dd(it.s, OP_T, OP_N, n, n, n, 1.0, V, n, nn, X, n, nn, 0.0, T2, n, nn, S));
-    hipLaunchKernelGGL(skew_half_kernel, dim3((n + 31) / 32, (n + 31) / 32, S), dim3(256), 0, it.s,
-                       n, T1, T2, c20 + (size_t)m * nn, (long long)dim * nn);
+  hipStream_t s = it.s;
+  if (dim == 0) return TG_OK;
+  const double* O = c->d_cHO.as<double>();
+  const int npair = c->cons_pair, nobo = dim - npair;
+  if (S > dim) {
+    // per constraint, every state in one launch
+    double* X = it.M(IT_X);
+    double* T1 = it.M(IT_T1);
+    double* T2 = it.M(IT_T2);
+    for (int m = 0; m < dim; ++m) {
+      const double* Om = O + (size_t)m * nn;
+      const bool pair = m >= nobo;
+      // f20: T1 = U^T O V, T2 = V^T O^T U;  g20: T1 = U^T O U, T2 = V^T O V
+      TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, Om, n, 0, pair ? U : V, n, nn, 0.0, X, n, nn, S));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, n, 1.0, U, n, nn, X, n, nn, 0.0, T1, n, nn, S));
+      TG_TRY(gemm_dd(s, pair ? OP_N : OP_T, OP_N, n, n, n, 1.0, Om, n, 0, pair ? V : U, n, nn, 0.0, X,
+                     n, nn, S));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, n, 1.0, V, n, nn, X, n, nn, 0.0, T2, n, nn, S));
+      hipLaunchKernelGGL(skew_half_kernel, dim3((n + 31) / 32, (n + 31) / 32, S), dim3(256), 0, s,
+                         n, T1, T2, c20 + (size_t)m * nn, (long long)dim * nn);
+      TG_HIP(hipGetLastError());
+    }
+    return TG_OK;
+  }
+  // per state, the one-body and the pair constraints each in one launch
+  TG_TRY(c->d_cq.ensure(sizeof(double) * 3 * (size_t)dim * nn));
+  double* X = c->d_cq.as<double>();
+  double* T1 = X + (size_t)dim * nn;
+  double* T2 = T1 + (size_t)dim * nn;
+  for (int st = 0; st < S; ++st) {
+    const double* Us = U + (size_t)st * nn;
+    const double* Vs = V + (size_t)st * nn;
+    if (nobo > 0) {
+      TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, O, n, nn, Vs, n, 0, 0.0, X, n, nn, nobo));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, n, 1.0, Us, n, 0, X, n, nn, 0.0, T1, n, nn, nobo));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, n, 1.0, O, n, nn, Us, n, 0, 0.0, X, n, nn, nobo));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, n, 1.0, Vs, n, 0, X, n, nn, 0.0, T2, n, nn, nobo));
+    }
+    if (npair > 0) {
+      const size_t o = (size_t)nobo * nn;
+      TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, O + o, n, nn, Us, n, 0, 0.0, X + o, n, nn, npair));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, n, 1.0, Us, n, 0, X + o, n, nn, 0.0, T1 + o, n, nn, npair));
+      TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, O + o, n, nn, Vs, n, 0, 0.0, X + o, n, nn, npair));
+      TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, n, 1.0, Vs, n, 0, X + o, n, nn, 0.0, T2 + o, n, nn, npair));
+    }
+    hipLaunchKernelGGL(skew_half_kernel, dim3((n + 31) / 32, (n + 31) / 32, dim), dim3(256), 0, s,
+                       n, T1, T2, c20 + (size_t)st * dim * nn, nn);
     TG_HIP(hipGetLastError());
   }
   return TG_OK;
@@ -340,7 +368,7 @@ int cholesky_inverse(const Iter& it, double* A, double* L, double* X, double* T,
   TG_HIP(hipGetLastError());
   for (int k0 = 0; k0 < n; k0 += CB) {
     const int b = std::min(CB, n - k0);
-    hipLaunchKernelGGL(chol_block_kernel, dim3(S), dim3(64), 0, s, n, k0, b, A, L, X, info);
+    hipLaunchKernelGGL(chol_block_kernel, dim3(S), dim3(CB * CB), 0, s, n, k0, b, A, L, X, info);
     TG_HIP(hipGetLastError());
     const int m = n - k0 - b;
     if (m == 0) break;
@@ -813,10 +841,24 @@ int tg_iterate_canonical(tg_ctx* c, double canon_eps, double gradient_eps, doubl
     return r;
   }
   c->defer_overlap = 1;
-  c->defer_hook = [&]() -> int {
-    ScopedTiming tm(c, "canonical_basis_beside", c->stream2);
-    TG_TRY(canonical_basis_on(c, c->stream2, canon_eps, nocc, nemp, ovacc, voveru));
+  hipEvent_t cb0 = nullptr;   // start of the canonical basis on the second stream (timing)
+  c->defer_hook = [&](int phase) -> int {
+    if (phase == 0) {
+      if (c->timing && hipEventCreate(&cb0) == hipSuccess) (void)hipEventRecord(cb0, c->stream2);
+      return canonical_launch(c, c->stream2, canon_eps);
+    }
+    TG_TRY(canonical_finish(c, c->stream2, canon_eps, nocc, nemp, ovacc, voveru));
     TG_TRY(upload_overlap_setup(c, c->stream2));
+    if (cb0) {
+      hipEvent_t cb1 = nullptr;
+      if (hipEventCreate(&cb1) == hipSuccess) {
+        (void)hipEventRecord(cb1, c->stream2);
+        c->pending.push_back({std::string("canonical_basis_beside"), cb0, cb1});
+      } else {
+        (void)hipEventDestroy(cb0);
+      }
+      cb0 = nullptr;
+    }
     c->state_dirty = 0;   // the rest of the projection's set-up did not change
     return TG_OK;
   };
@@ -826,6 +868,7 @@ int tg_iterate_canonical(tg_ctx* c, double canon_eps, double gradient_eps, doubl
   c->grad_on = grad_was;
   c->defer_overlap = 0;
   c->defer_hook = nullptr;
+  if (cb0) (void)hipEventDestroy(cb0);
   if (r != TG_OK) {
     // leave nothing in flight (a wide degenerate block, TG_E_UNSUPPORTED, leaves
     // the state as it was: the caller takes the host route and tg_iterate)

@@ -1118,6 +1118,9 @@ int project_run(tg_ctx* c) {
   hipStream_t s2 = c->stream2;
   TG_HIP(hipEventRecord(c->ev_fork, s));
   TG_HIP(hipStreamWaitEvent(s2, c->ev_fork, 0));
+  // the canonical basis's device work goes first on the second stream, so that
+  // its single-workgroup eigen-solver runs beside the inverse
+  if (defer) TG_TRY(c->defer_hook(0));
   const RunPlan* PP = &P;   // the plan of the overlap chain (the new one when deferred)
   RunPlan P2;
 
@@ -1320,7 +1323,7 @@ int project_run(tg_ctx* c) {
   if (defer) {
     // the canonical basis (second stream, host waits on that stream only), the
     // overlap set-up of the new basis, then the overlap chain with its shapes
-    TG_TRY(c->defer_hook());
+    TG_TRY(c->defer_hook(1));
     P2 = make_plan(c);
     PP = &P2;
     TG_TRY(d_products(s2));
@@ -1757,7 +1760,7 @@ int tg_finalize(tg_ctx* c) {
   graph_release(c);
   for (DevBuf* b : {&c->d_H1, &c->d_U, &c->d_V, &c->d_D, &c->d_ang, &c->d_real6, &c->d_work,
                     &c->d_blk, &c->d_scal, &c->d_out, &c->d_ovl, &c->d_int, &c->d_pfw, &c->d_c20,
-                    &c->d_Z, &c->d_psign, &c->d_cHO, &c->d_it, &c->d_obs, &c->d_canon})
+                    &c->d_Z, &c->d_psign, &c->d_cHO, &c->d_it, &c->d_obs, &c->d_canon, &c->d_cq})
     b->release();
   for (Channel* ch : {&c->ph, &c->pp})
     for (DevBuf* b : {&ch->d_inv, &ch->d_groups, &ch->d_W, &ch->d_tile_group, &ch->d_tile_n0})
