@@ -88,13 +88,18 @@ struct Tile {
     const int wm0 = (w % WM) * (BM / WM), wn0 = (w / WM) * (BN / WN);
     const int lr = lane & 15, lk = lane >> 4;
 
-    d4 cr[TM][TN], ci[TM][TN];
+    // complex x complex in three real products (Gauss): cr = sum Ar Br, ci = sum
+    // Ai Bi, c3 = sum (Ar + Ai)(Br + Bi); C = (cr - ci, c3 - cr - ci) at the end.
+    // A real operand takes two products: cr, ci are the real and imaginary parts.
+    constexpr bool CC = CA && CB;
+    d4 cr[TM][TN], ci[TM][TN], c3[CC ? TM : 1][CC ? TN : 1];
 #pragma unroll
     for (int x = 0; x < TM; ++x)
 #pragma unroll
       for (int y = 0; y < TN; ++y) {
         cr[x][y] = d4{0.0, 0.0, 0.0, 0.0};
         ci[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+        if constexpr (CC) c3[x][y] = d4{0.0, 0.0, 0.0, 0.0};
       }
 
     TA ra[NA];
@@ -160,16 +165,22 @@ struct Tile {
           br[y] = sm.br[idx];
           if (CB) bi[y] = sm.bi[idx];
         }
+        double sa[CC ? TM : 1], sb[CC ? TN : 1];
+        if constexpr (CC) {
+#pragma unroll
+          for (int x = 0; x < TM; ++x) sa[x] = ar[x] + ai[x];
+#pragma unroll
+          for (int y = 0; y < TN; ++y) sb[y] = br[y] + bi[y];
+        }
 #pragma unroll
         for (int x = 0; x < TM; ++x)
 #pragma unroll
           for (int y = 0; y < TN; ++y) {
-            if (CA && CB) {
+            if constexpr (CC) {
               cr[x][y] = mfma(ar[x], br[y], cr[x][y]);
-              cr[x][y] = mfma(-ai[x], bi[y], cr[x][y]);
-              ci[x][y] = mfma(ar[x], bi[y], ci[x][y]);
-              ci[x][y] = mfma(ai[x], br[y], ci[x][y]);
-            } else if (CA) {
+              ci[x][y] = mfma(ai[x], bi[y], ci[x][y]);
+              c3[x][y] = mfma(sa[x], sb[y], c3[x][y]);
+            } else if constexpr (CA) {
               cr[x][y] = mfma(ar[x], br[y], cr[x][y]);
               ci[x][y] = mfma(ai[x], br[y], ci[x][y]);
             } else {
@@ -193,7 +204,12 @@ struct Tile {
           const int gi = m0 + wm0 + 16 * x + lk + 4 * r;
           if (gi >= M) continue;
           cplx* cp = C + gi + (long long)gj * ldc;
-          cplx v = cmul(alpha, cmk(cr[x][y][r], ci[x][y][r]));
+          cplx p;
+          if constexpr (CC)
+            p = cmk(cr[x][y][r] - ci[x][y][r], c3[x][y][r] - cr[x][y][r] - ci[x][y][r]);
+          else
+            p = cmk(cr[x][y][r], ci[x][y][r]);
+          cplx v = cmul(alpha, p);
           if (use_beta) v = cadd(v, cmul(beta, *cp));
           *cp = v;
         }
