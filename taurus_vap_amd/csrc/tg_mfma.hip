@@ -43,16 +43,23 @@ template <> __device__ inline cplx zero_v<cplx>() { return cmk(0.0, 0.0); }
 
 constexpr int XCDS = 8;
 
-template <typename TA, typename TB, int OPA, int OPB, int BM, int BN, int WM, int WN>
+// KM = true stages both operands k-major in LDS ([k][i], [k][j], rows of BM / BN
+// plus a pad) whatever their layout in memory; false keeps each operand's
+// memory order (the row of a k-contiguous operand then holds BK + 1).
+template <typename TA, typename TB, int OPA, int OPB, int BM, int BN, int WM, int WN,
+          bool KM = false>
 struct Tile {
   static constexpr int BK = 16;
   static constexpr int NW = WM * WN, NTH = 64 * NW;
   static constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
   static constexpr bool CA = IsC<TA>::v, CB = IsC<TB>::v;
-  // A(i,k): OP_N -> contiguous in i: LDS [k][i]; else contiguous in k: LDS [i][k]
-  static constexpr bool AK = (OPA == OP_N);
-  // B(k,j): OP_N -> contiguous in k: LDS [j][k]; else contiguous in j: LDS [k][j]
-  static constexpr bool BJ = (OPB == OP_N);
+  // A(i,k) in memory: OP_N -> contiguous in i (AG), else contiguous in k
+  static constexpr bool AG = (OPA == OP_N);
+  // B(k,j) in memory: OP_N -> contiguous in k (BG), else contiguous in j
+  static constexpr bool BG = (OPB == OP_N);
+  // LDS: A as [k][i] (AK) or [i][k]; B as [j][k] (BJ) or [k][j]
+  static constexpr bool AK = AG || KM;
+  static constexpr bool BJ = BG && !KM;
   static constexpr int PADR = 4;                   // pad of rows of length BM / BN
   static constexpr int LDA_ = AK ? BM + PADR : BK + 1;
   static constexpr int LDB_ = BJ ? BK + 1 : BN + PADR;
@@ -73,11 +80,12 @@ struct Tile {
   };
 
   // element (i, k) of op(A) and (k, j) of op(B) handled by staging slot e
+  // (consecutive slots follow the memory order: coalesced loads)
   __device__ static inline void slot_a(int e, int& i, int& k) {
-    if (AK) { i = e % BM; k = e / BM; } else { k = e % BK; i = e / BK; }
+    if (AG) { i = e % BM; k = e / BM; } else { k = e % BK; i = e / BK; }
   }
   __device__ static inline void slot_b(int e, int& k, int& j) {
-    if (BJ) { k = e % BK; j = e / BK; } else { j = e % BN; k = e / BN; }
+    if (BG) { k = e % BK; j = e / BK; } else { j = e % BN; k = e / BN; }
   }
 
   // always inlined: several kernels share one tile body
@@ -111,7 +119,7 @@ struct Tile {
         slot_a(t + r * NTH, i, k);
         const int gi = m0 + i, gk = k0 + k;
         TA v = zero_v<TA>();
-        if (gi < M && gk < K) v = AK ? A[gi + (long long)gk * lda] : A[gk + (long long)gi * lda];
+        if (gi < M && gk < K) v = AG ? A[gi + (long long)gk * lda] : A[gk + (long long)gi * lda];
         ra[r] = v;
       }
 #pragma unroll
@@ -120,7 +128,7 @@ struct Tile {
         slot_b(t + r * NTH, k, j);
         const int gk = k0 + k, gj = n0 + j;
         TB v = zero_v<TB>();
-        if (gk < K && gj < N) v = BJ ? B[gk + (long long)gj * ldb] : B[gj + (long long)gk * ldb];
+        if (gk < K && gj < N) v = BG ? B[gk + (long long)gj * ldb] : B[gj + (long long)gk * ldb];
         rb[r] = v;
       }
     };
@@ -225,13 +233,13 @@ struct Tile {
 // are consecutive on their XCD.  OCC > 1 bounds the registers for OCC times the
 // workgroup's waves per CU.
 template <typename TA, typename TB, int OPA, int OPB, int BM, int BN, int WM, int WN, bool BAL,
-          int OCC>
+          int OCC, bool KM>
 __global__ void __launch_bounds__(64 * WM * WN * OCC)
 mfma_batched_kernel(int M, int N, int K, cplx alpha, const TA* __restrict__ A, int lda,
                     long long sA, const TB* __restrict__ B, int ldb, long long sB, cplx beta,
                     int use_beta, cplx* __restrict__ C, int ldc, long long sC, int batch,
                     int tiles_m, int tiles, long long per, int divA, int divB) {
-  typedef Tile<TA, TB, OPA, OPB, BM, BN, WM, WN> T;
+  typedef Tile<TA, TB, OPA, OPB, BM, BN, WM, WN, KM> T;
   __shared__ typename T::Smem sm;
   const int L = blockIdx.x;
   const int xcd = L % XCDS, q = L / XCDS;
@@ -335,7 +343,7 @@ mfma_grouped_kernel(int M, const cplx* __restrict__ X, cplx* __restrict__ C,
 }
 
 template <typename TA, typename TB, int OPA, int OPB, int BM, int BN, int WM, int WN,
-          bool BAL = false, int OCC = 1>
+          bool BAL = false, int OCC = 1, bool KM = false>
 int launch_batched(hipStream_t s, int M, int N, int K, cplx alpha, const TA* A, int lda,
                    long long sA, const TB* B, int ldb, long long sB, cplx beta, int use_beta,
                    cplx* C, int ldc, long long sC, int batch, int divA, int divB) {
@@ -349,7 +357,7 @@ int launch_batched(hipStream_t s, int M, int N, int K, cplx alpha, const TA* A, 
     return TG_E_ARG;
   }
   if (grid == 0) return TG_OK;
-  hipLaunchKernelGGL((mfma_batched_kernel<TA, TB, OPA, OPB, BM, BN, WM, WN, BAL, OCC>),
+  hipLaunchKernelGGL((mfma_batched_kernel<TA, TB, OPA, OPB, BM, BN, WM, WN, BAL, OCC, KM>),
                      dim3((unsigned)grid), dim3(64 * WM * WN), 0, s, M, N, K, alpha, A, lda, sA, B,
                      ldb, sB, beta, use_beta, C, ldc, sC, batch, tiles_m, tiles, per, divA, divB);
   TG_HIP(hipGetLastError());
@@ -389,6 +397,12 @@ int launch_variant(int variant, hipStream_t s, int M, int N, int K, cplx alpha, 
           s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, use_beta, C, ldc, sC, batch, divA, divB);
     case 8:
       return launch_batched<TA, TB, OPA, OPB, 64, 32, 1, 2, true, 4>(
+          s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, use_beta, C, ldc, sC, batch, divA, divB);
+    case 9:
+      return launch_batched<TA, TB, OPA, OPB, 48, 48, 1, 3, true, 4, true>(
+          s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, use_beta, C, ldc, sC, batch, divA, divB);
+    case 10:
+      return launch_batched<TA, TB, OPA, OPB, 48, 48, 1, 3, true, 1, true>(
           s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, use_beta, C, ldc, sC, batch, divA, divB);
   }
   tg_set_error("mfma gemm: bad tile variant");
