@@ -546,9 +546,10 @@ def main():
             "workload": "%s: HO basis of %d sp states, %dx%d gauge angles, %dp/%dn, constraints "
                         "<Z>, <N>, Q20 = 0.2, Jx = 0, seeded synthetic JT interaction (%d stored "
                         "m-scheme TBMEs, TBMEs %s / FP64 arithmetic), state = the reference's "
-                        "first iteration; step = one full gradient iteration (tg_iterate: "
-                        "projection, multipliers, gradient, adjust_constraints; then the device "
-                        "canonical basis), state resident in HBM" % (
+                        "first iteration; step = one full gradient iteration "
+                        "(tg_iterate_canonical: the device canonical basis of the state beside "
+                        "the projection, then multipliers, gradient, adjust_constraints), state "
+                        "resident in HBM" % (
                             args.workload, n, M, M, w.Z, w.N, len(ham["trperm"]),
                             args.tbme.upper()),
             "mode": "batched" + ("+graph" if args.graph else "") +

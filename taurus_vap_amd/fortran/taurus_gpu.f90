@@ -178,6 +178,20 @@ interface
     real(c_double), intent(out) :: ovacc(*), voveru(*)
   end function
 
+  ! tg_canonical_basis then tg_iterate in one call (the canonical basis beside the projection)
+  integer(c_int) function tg_iterate_canonical(ctx, canon_eps, gradient_eps, pnp_scalars, &
+                                               gradient_norm, lagrange_lambda1,              &
+                                               adjust_iterations, converged, nocc, nemp,      &
+                                               ovacc, voveru) bind(C, name='tg_iterate_canonical')
+    import
+    type(c_ptr), value :: ctx
+    real(c_double), value :: canon_eps, gradient_eps
+    complex(c_double_complex), intent(out) :: pnp_scalars(*)
+    real(c_double), intent(out) :: gradient_norm(*), lagrange_lambda1(*)
+    integer(c_int), intent(out) :: adjust_iterations(*), converged(*), nocc(*), nemp(*)
+    real(c_double), intent(out) :: ovacc(*), voveru(*)
+  end function
+
   ! diagonalize_hsp_and_H11(0, ndim) of the resident state
   integer(c_int) function tg_diagonalize_hsp_and_H11(ctx, constraint_dim, lagrange_lambda1, &
                                                      eigen_hsp, eigen_H11, info_hsp, info_H11, &
