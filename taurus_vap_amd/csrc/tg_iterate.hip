@@ -358,7 +358,11 @@ int gram(const Iter& it, const double* c20, const std::vector<int>& states,
 // update as GEMMs), then the off-diagonal blocks of L^-1 row block by row
 // block: X(i, :i) = -X(i, i) L(i, :i) X(:i, :i).  L and T are scratch (T:
 // CB x n per state).  info[s] = 0 or the first non-positive pivot column.
-int cholesky_inverse(const Iter& it, double* A, double* L, double* X, double* T, int* info) {
+// The block rows of L^-1 go to stream s2 (T is its scratch), each after the
+// diagonal block it needs (event ev), beside the Cholesky's later steps on the
+// context stream; s2 is not joined here.
+int cholesky_inverse(const Iter& it, double* A, double* L, double* X, double* T, int* info,
+                     hipStream_t s2, hipEvent_t ev) {
   const int n = it.n, S = it.S;
   const long long nn = it.nn;
   hipStream_t s = it.s;
@@ -370,6 +374,16 @@ int cholesky_inverse(const Iter& it, double* A, double* L, double* X, double* T,
     const int b = std::min(CB, n - k0);
     hipLaunchKernelGGL(chol_block_kernel, dim3(S), dim3(CB * CB), 0, s, n, k0, b, A, L, X, info);
     TG_HIP(hipGetLastError());
+    if (k0 > 0) {
+      // X(k0:k0+b, :k0) = -X(k0, k0) L(k0:k0+b, :k0) X(:k0, :k0): L's rows of this block
+      // are final (earlier panels), X(:k0, :k0) is the diagonal blocks and the earlier
+      // block rows (this stream)
+      TG_HIP(hipEventRecord(ev, s));
+      TG_HIP(hipStreamWaitEvent(s2, ev, 0));
+      TG_TRY(gemm_dd(s2, OP_N, OP_N, b, k0, k0, 1.0, L + k0, n, nn, X, n, nn, 0.0, T, CB, nn, S));
+      TG_TRY(gemm_dd(s2, OP_N, OP_N, b, k0, b, -1.0, X + k0 + (long long)k0 * n, n, nn, T, CB, nn,
+                     0.0, X + k0, n, nn, S));
+    }
     const int m = n - k0 - b;
     if (m == 0) break;
     const long long pk = k0 + (long long)k0 * n, pb = (k0 + b) + (long long)k0 * n;
@@ -378,13 +392,6 @@ int cholesky_inverse(const Iter& it, double* A, double* L, double* X, double* T,
     // trailing update A(k0+b:, k0+b:) -= L_panel L_panel^T
     TG_TRY(gemm_dd(s, OP_N, OP_T, m, m, b, -1.0, L + pb, n, nn, L + pb, n, nn, 1.0,
                    A + (k0 + b) + (long long)(k0 + b) * n, n, nn, S));
-  }
-  for (int i0 = CB; i0 < n; i0 += CB) {
-    const int bi = std::min(CB, n - i0);
-    // T = L(i0:i0+bi, :i0) X(:i0, :i0);  X(i0:i0+bi, :i0) = -X(i0:, i0:) T
-    TG_TRY(gemm_dd(s, OP_N, OP_N, bi, i0, i0, 1.0, L + i0, n, nn, X, n, nn, 0.0, T, CB, nn, S));
-    TG_TRY(gemm_dd(s, OP_N, OP_N, bi, i0, bi, -1.0, X + i0 + (long long)i0 * n, n, nn, T, CB, nn, 0.0,
-                   X + i0, n, nn, S));
   }
   return TG_OK;
 }
@@ -519,15 +526,23 @@ int iterate_post(tg_ctx* c, double gradient_eps, IterOut& out) {
     TG_TRY(gemm_dd(s, OP_T, OP_N, n, n, n, 1.0, Z, n, nn, Z, n, nn, 0.0, it.M(IT_S), n, nn, S));
     hipLaunchKernelGGL(add_identity_kernel, dim3(nblk(n), S), dim3(NT), 0, s, n, it.M(IT_S));
     TG_HIP(hipGetLastError());
-    TG_TRY(cholesky_inverse(it, it.M(IT_S), it.M(IT_T2), it.M(IT_LINV), it.M(IT_X), dinfo));
+    // evolve_wavefunction's U0 + V0 Z, V0 + U0 Z on the second stream beside the
+    // Cholesky (and then the block rows of L^-1 there)
+    hipStream_t s2 = c->stream2;
+    TG_HIP(hipEventRecord(c->ev_fork, s));
+    TG_HIP(hipStreamWaitEvent(s2, c->ev_fork, 0));
+    hipLaunchKernelGGL(copy_kernel, dim3(nblk(S * nn)), dim3(NT), 0, s2, S * nn, U0, it.M(IT_A1));
+    hipLaunchKernelGGL(copy_kernel, dim3(nblk(S * nn)), dim3(NT), 0, s2, S * nn, V0, it.M(IT_A2));
+    TG_HIP(hipGetLastError());
+    TG_TRY(gemm_dd(s2, OP_N, OP_N, n, n, n, 1.0, V0, n, nn, Z, n, nn, 1.0, it.M(IT_A1), n, nn, S));
+    TG_TRY(gemm_dd(s2, OP_N, OP_N, n, n, n, 1.0, U0, n, nn, Z, n, nn, 1.0, it.M(IT_A2), n, nn, S));
+    TG_TRY(cholesky_inverse(it, it.M(IT_S), it.M(IT_T2), it.M(IT_LINV), it.M(IT_X), dinfo, s2,
+                            c->ev_fork));
+    TG_HIP(hipEventRecord(c->ev_join, s2));
+    TG_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
     std::vector<int> info(S, 0);
     TG_HIP(hipMemcpyAsync(info.data(), dinfo, sizeof(int) * S, hipMemcpyDeviceToHost, s));
     // evolve_wavefunction: U1 = (U0 + V0 Z) L^-T, V1 = (V0 + U0 Z) L^-T
-    hipLaunchKernelGGL(copy_kernel, dim3(nblk(S * nn)), dim3(NT), 0, s, S * nn, U0, it.M(IT_A1));
-    hipLaunchKernelGGL(copy_kernel, dim3(nblk(S * nn)), dim3(NT), 0, s, S * nn, V0, it.M(IT_A2));
-    TG_HIP(hipGetLastError());
-    TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, V0, n, nn, Z, n, nn, 1.0, it.M(IT_A1), n, nn, S));
-    TG_TRY(gemm_dd(s, OP_N, OP_N, n, n, n, 1.0, U0, n, nn, Z, n, nn, 1.0, it.M(IT_A2), n, nn, S));
     TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, it.M(IT_A1), n, nn, it.M(IT_LINV), n, nn, 0.0,
                    it.M(IT_U1), n, nn, S));
     TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, it.M(IT_A2), n, nn, it.M(IT_LINV), n, nn, 0.0,
