@@ -217,6 +217,7 @@ bool small_solve(int m, std::vector<double> A, std::vector<double>& b) {
 //   [0, 3 * S * MAXPAIRS)          dot-product pointer lists and results
 //   [TX, TX + S * TG_MAX_CONS)     constraint corrections x per state
 //   [TI, ...)                      Cholesky info and copy masks (ints, S each)
+//   [T2, T2 + 3 S MAXPAIRS)        a second dot-product area (two batches in flight)
 enum { IT_S, IT_LINV, IT_A1, IT_A2, IT_U1, IT_V1, IT_R, IT_T1, IT_T2, IT_X, IT_COUNT };
 
 struct Iter {
@@ -229,13 +230,15 @@ struct Iter {
   double* tail() const { return w + (size_t)IT_COUNT * S * nn; }
   size_t tx() const { return 3 * (size_t)S * MAXPAIRS; }
   size_t ti() const { return tx() + (size_t)S * TG_MAX_CONS; }
+  size_t t2() const { return ti() + (size_t)S + 64; }
   static size_t bytes(int n, int S) {
-    return sizeof(double) * ((size_t)IT_COUNT * S * n * n + 3 * (size_t)S * MAXPAIRS +
+    return sizeof(double) * ((size_t)IT_COUNT * S * n * n + 6 * (size_t)S * MAXPAIRS +
                              (size_t)S * TG_MAX_CONS + (size_t)S + 64);
   }
   double* x() const { return tail() + tx(); }
   int* info() const { return reinterpret_cast<int*>(tail() + ti()); }
   int* mask() const { return info() + S; }
+  double* tail2() const { return tail() + t2(); }   // a second dot-product area
 };
 
 // constraint_20 of every constraint at (U, V) of every state into c20
@@ -296,44 +299,68 @@ int constraints_qp(const Iter& it, const double* U, const double* V, double* c20
   return TG_OK;
 }
 
-// host copies of sum_k X_p[k] Y_p[k] for the pairs (xs[p], ys[p])
-int dots(const Iter& it, const std::vector<const double*>& xs, const std::vector<const double*>& ys,
-         std::vector<double>& out) {
+// A batch of dot products sum_k X_p[k] Y_p[k] in flight on the iteration's stream:
+// its pointer lists (uploaded from h) and slice sums (read back into part) live in
+// a device area of 3 S MAXPAIRS doubles; the sums are formed on the host after the
+// caller has synchronised the stream (dots_finish).
+struct DotsJob {
+  int np = 0, ds = 1;
+  std::vector<const double*> h;
+  std::vector<double> part;
+};
+
+int dots_launch(const Iter& it, double* area, const std::vector<const double*>& xs,
+                const std::vector<const double*>& ys, DotsJob& job) {
   const int np = (int)xs.size();
-  out.assign(np, 0.0);
+  job.np = np;
   if (np == 0) return TG_OK;
   if (np > it.S * MAXPAIRS) {
     tg_set_error("constraint dot products: too many pairs");
     return TG_E_ARG;
   }
-  const double** pl = reinterpret_cast<const double**>(it.tail());
+  const double** pl = reinterpret_cast<const double**>(area);
   double* res = reinterpret_cast<double*>(pl + 2 * np);
-  std::vector<const double*> h(2 * np);
+  job.h.assign(2 * (size_t)np, nullptr);
   for (int p = 0; p < np; ++p) {
-    h[p] = xs[p];
-    h[np + p] = ys[p];
+    job.h[p] = xs[p];
+    job.h[np + p] = ys[p];
   }
   // each dot in up to 32 slices (one block each) as the result area allows;
-  // the slices are summed here in a fixed order
+  // the slices are summed on the host in a fixed order
   const long long room = 3LL * it.S * MAXPAIRS - 2LL * np;
-  const int ds = (int)std::max(1LL, std::min(32LL, room / np));
-  TG_HIP(hipMemcpyAsync(pl, h.data(), sizeof(double*) * 2 * np, hipMemcpyHostToDevice, it.s));
-  hipLaunchKernelGGL(dots_kernel, dim3(np, ds), dim3(NT), 0, it.s, it.nn, pl, pl + np, res);
+  job.ds = (int)std::max(1LL, std::min(32LL, room / np));
+  job.part.assign((size_t)np * job.ds, 0.0);
+  TG_HIP(hipMemcpyAsync(pl, job.h.data(), sizeof(double*) * 2 * np, hipMemcpyHostToDevice, it.s));
+  hipLaunchKernelGGL(dots_kernel, dim3(np, job.ds), dim3(NT), 0, it.s, it.nn, pl, pl + np, res);
   TG_HIP(hipGetLastError());
-  std::vector<double> part((size_t)np * ds);
-  TG_HIP(hipMemcpyAsync(part.data(), res, sizeof(double) * np * ds, hipMemcpyDeviceToHost, it.s));
-  TG_HIP(hipStreamSynchronize(it.s));
-  for (int p = 0; p < np; ++p) {
-    double acc = 0.0;
-    for (int q = 0; q < ds; ++q) acc += part[(size_t)p * ds + q];
-    out[p] = acc;
-  }
+  TG_HIP(hipMemcpyAsync(job.part.data(), res, sizeof(double) * np * job.ds, hipMemcpyDeviceToHost,
+                        it.s));
   return TG_OK;
 }
 
-// Gram matrices A_s,ij = Tr(C20_s,i^T C20_s,j) of the listed states (host, dim x dim each)
-int gram(const Iter& it, const double* c20, const std::vector<int>& states,
-         std::vector<std::vector<double>>& A) {
+void dots_finish(const DotsJob& job, std::vector<double>& out) {
+  out.assign(job.np, 0.0);
+  for (int p = 0; p < job.np; ++p) {
+    double acc = 0.0;
+    for (int q = 0; q < job.ds; ++q) acc += job.part[(size_t)p * job.ds + q];
+    out[p] = acc;
+  }
+}
+
+// host copies of sum_k X_p[k] Y_p[k] for the pairs (xs[p], ys[p])
+int dots(const Iter& it, const std::vector<const double*>& xs, const std::vector<const double*>& ys,
+         std::vector<double>& out) {
+  DotsJob job;
+  TG_TRY(dots_launch(it, it.tail(), xs, ys, job));
+  TG_HIP(hipStreamSynchronize(it.s));
+  dots_finish(job, out);
+  return TG_OK;
+}
+
+// Gram matrices A_s,ij = Tr(C20_s,i^T C20_s,j) of the listed states (dim x dim each):
+// the dot products, then (after a synchronisation) the matrices
+int gram_launch(const Iter& it, double* area, const double* c20, const std::vector<int>& states,
+                DotsJob& job) {
   const int m = it.c->cons_dim;
   std::vector<const double*> xs, ys;
   for (int s : states)
@@ -342,13 +369,27 @@ int gram(const Iter& it, const double* c20, const std::vector<int>& states,
         xs.push_back(c20 + ((size_t)s * m + i) * it.nn);
         ys.push_back(c20 + ((size_t)s * m + j) * it.nn);
       }
+  return dots_launch(it, area, xs, ys, job);
+}
+
+void gram_finish(const Iter& it, const DotsJob& job, size_t nstates,
+                 std::vector<std::vector<double>>& A) {
+  const int m = it.c->cons_dim;
   std::vector<double> d;
-  TG_TRY(dots(it, xs, ys, d));
-  A.assign(states.size(), std::vector<double>((size_t)m * m, 0.0));
+  dots_finish(job, d);
+  A.assign(nstates, std::vector<double>((size_t)m * m, 0.0));
   int k = 0;
-  for (size_t q = 0; q < states.size(); ++q)
+  for (size_t q = 0; q < nstates; ++q)
     for (int i = 0; i < m; ++i)
       for (int j = i; j < m; ++j, ++k) A[q][i + j * m] = A[q][j + i * m] = d[k];
+}
+
+int gram(const Iter& it, const double* c20, const std::vector<int>& states,
+         std::vector<std::vector<double>>& A) {
+  DotsJob job;
+  TG_TRY(gram_launch(it, it.tail(), c20, states, job));
+  TG_HIP(hipStreamSynchronize(it.s));
+  gram_finish(it, job, states.size(), A);
   return TG_OK;
 }
 
@@ -519,6 +560,8 @@ int iterate_post(tg_ctx* c, double gradient_eps, IterOut& out) {
   for (int st : evolve) open_state[st] = 1;
   double* xd = it.x();
   int* dinfo = it.info();
+  std::vector<int> hinfo(S, 0);
+  std::vector<double> xh((size_t)S * TG_MAX_CONS, 0.0);
   for (int k = 1; k <= maxit; ++k) {
     for (int st = 0; st < S; ++st)
       if (open_state[st]) out.nadj[st] = k;
@@ -540,22 +583,29 @@ int iterate_post(tg_ctx* c, double gradient_eps, IterOut& out) {
                             c->ev_fork));
     TG_HIP(hipEventRecord(c->ev_join, s2));
     TG_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
-    std::vector<int> info(S, 0);
-    TG_HIP(hipMemcpyAsync(info.data(), dinfo, sizeof(int) * S, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipMemcpyAsync(hinfo.data(), dinfo, sizeof(int) * S, hipMemcpyDeviceToHost, s));
     // evolve_wavefunction: U1 = (U0 + V0 Z) L^-T, V1 = (V0 + U0 Z) L^-T
     TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, it.M(IT_A1), n, nn, it.M(IT_LINV), n, nn, 0.0,
                    it.M(IT_U1), n, nn, S));
     TG_TRY(gemm_dd(s, OP_N, OP_T, n, n, n, 1.0, it.M(IT_A2), n, nn, it.M(IT_LINV), n, nn, 0.0,
                    it.M(IT_V1), n, nn, S));
-    TG_HIP(hipStreamSynchronize(s));
-    for (int st : evolve)
-      if (info[st] != 0) {
-        tg_set_error("Cholesky factorization has failed, info = " + std::to_string(info[st]) +
-                     " (state " + std::to_string(st) + ")");
-        return TG_E_SINGULAR;
-      }
-    if (dim == 0) break;
-    // expectation values <O_m>: Tr(O rho) (one-body), Tr(O kappa^T) (pair)
+    auto cholesky_ok = [&]() -> int {
+      for (int st : evolve)
+        if (hinfo[st] != 0) {
+          tg_set_error("Cholesky factorization has failed, info = " + std::to_string(hinfo[st]) +
+                       " (state " + std::to_string(st) + ")");
+          return TG_E_SINGULAR;
+        }
+      return TG_OK;
+    };
+    if (dim == 0) {
+      TG_HIP(hipStreamSynchronize(s));
+      TG_TRY(cholesky_ok());
+      break;
+    }
+    // expectation values <O_m>: Tr(O rho) (one-body), Tr(O kappa^T) (pair), the
+    // constraint_20 at the evolved states and their Gram matrices, read back with the
+    // Cholesky info at one synchronisation
     const bool any_pair = c->cons_pair > 0, any_obo = dim > c->cons_pair;
     double* rho = it.M(IT_R);
     double* kap = it.M(IT_T1);
@@ -574,8 +624,14 @@ int iterate_post(tg_ctx* c, double gradient_eps, IterOut& out) {
         xs.push_back(c->d_cHO.as<double>() + (size_t)m * nn);
         ys.push_back((m >= dim - c->cons_pair ? kap : rho) + (size_t)st * nn);
       }
+    DotsJob evj, grj;
+    TG_TRY(dots_launch(it, it.tail(), xs, ys, evj));
+    TG_TRY(constraints_qp(it, it.M(IT_U1), it.M(IT_V1), c20));
+    TG_TRY(gram_launch(it, it.tail2(), c20, act, grj));
+    TG_HIP(hipStreamSynchronize(s));
+    TG_TRY(cholesky_ok());
     std::vector<double> ev;
-    TG_TRY(dots(it, xs, ys, ev));
+    dots_finish(evj, ev);
     std::vector<std::vector<double>> Bv(act.size(), std::vector<double>(dim));
     std::vector<int> still;
     for (size_t q = 0; q < act.size(); ++q) {
@@ -587,27 +643,23 @@ int iterate_post(tg_ctx* c, double gradient_eps, IterOut& out) {
       if (open == 0) open_state[act[q]] = 0;
       else still.push_back((int)q);
     }
-    // constraint_20 at the evolved states
-    TG_TRY(constraints_qp(it, it.M(IT_U1), it.M(IT_V1), c20));
     if (still.empty()) break;
-    std::vector<int> sts;
-    for (int q : still) sts.push_back(act[q]);
     std::vector<std::vector<double>> A;
-    TG_TRY(gram(it, c20, sts, A));
-    std::vector<double> xh((size_t)S * TG_MAX_CONS, 0.0);
-    for (size_t r = 0; r < still.size(); ++r) {
-      std::vector<double> b = Bv[still[r]];
-      if (!small_solve(dim, A[r], b)) {
+    gram_finish(it, grj, act.size(), A);
+    std::fill(xh.begin(), xh.end(), 0.0);
+    for (int q : still) {
+      std::vector<double> b = Bv[q];
+      if (!small_solve(dim, A[q], b)) {
         tg_set_error("adjust_constraints: singular constraint system (state " +
-                     std::to_string(sts[r]) + ")");
+                     std::to_string(act[q]) + ")");
         return TG_E_SINGULAR;
       }
-      std::copy(b.begin(), b.end(), xh.begin() + (size_t)sts[r] * TG_MAX_CONS);
+      std::copy(b.begin(), b.end(), xh.begin() + (size_t)act[q] * TG_MAX_CONS);
     }
+    // xh is rewritten only after the next synchronisation of the stream
     TG_HIP(hipMemcpyAsync(xd, xh.data(), sizeof(double) * xh.size(), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(axpy_cons_kernel, dim3(nblk(nn), S), dim3(NT), 0, s, nn, dim, xd, c20, Z);
     TG_HIP(hipGetLastError());
-    TG_HIP(hipStreamSynchronize(s));
   }
   // the evolved states and the momentum term of their next step
   std::vector<int> mask(S, 0);
