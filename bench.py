@@ -64,17 +64,11 @@ def algorithmic_flops(ctx_info, n, A, nb):
         # complex batched GEMMs of the loop body on the main stream: A_phi (n^3),
         # the two H20 products over the inner dimension 2n and the one over n
         "kernel:zgemm": 8.0 * A * 6 * n ** 3,
-        # the reduced-overlap product Q^T (J Q) on the second stream: its upper
-        # row bands (the skew result's upper triangle is what the Pfaffian reads)
-        "kernel:zgemm_q": 8.0 * A * nb * sum(min(rb, nb - r0) * (nb - r0)
-                                             for rb in [q_band_rows(nb)]
-                                             for r0 in range(0, nb, rb)),
+        # the reduced overlap's angle-independent parts of K on the second stream:
+        # three real nb x nb x nb products per state (the per-angle combination is
+        # element-wise)
+        "kernel:k_parts": 3 * 2.0 * nb ** 3,
     }
-
-
-def q_band_rows(nb):
-    """Row band height of the reduced-overlap product (as the library's)."""
-    return (((nb + 2) // 3 + 15) // 16) * 16 if nb >= 128 else nb
 
 
 def measured_traffic(workload, angles_per_launch, kernel):
@@ -197,7 +191,7 @@ def iterate_gpu(args, ham, fix, plain, fp32, steps, warmup, timing=True, graph=F
     kt = {}
     if timing:
         for key in ("total", "kernel:grouped_ph", "kernel:grouped_pp", "kernel:gj_inverse",
-                    "kernel:pfaffian", "kernel:zgemm", "kernel:zgemm_q", "contraction", "gemm",
+                    "kernel:pfaffian", "kernel:zgemm", "kernel:k_parts", "contraction", "gemm",
                     "gemm_h20", "overlap", "thouless_inverse", "canonical_basis_beside"):
             kt[key] = ctx.timing(key)
         ctx.enable_timing(False)

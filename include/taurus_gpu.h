@@ -429,8 +429,10 @@ int tg_group_iterate_results(tg_group* group, int index, double* pnp_scalars,
  * the number of intervals recorded under `key` ("total", "kernel:gj_inverse",
  * "kernel:pfaffian", "kernel:grouped_ph", "kernel:grouped_pp", "kernel:zgemm"
  * (the complex batched GEMMs of the loop body on the context stream),
- * "kernel:zgemm_q" (the reduced-overlap product on the second stream),
- * "contraction", "gemm", "gemm_h20", "overlap", "thouless_inverse").
+ * "kernel:k_parts" (the reduced overlap's angle-independent parts of its skew
+ * matrix, on the second stream), "contraction", "gemm", "gemm_h20", "overlap",
+ * "thouless_inverse", and under tg_iterate_canonical "canonical_basis_beside"
+ * (the canonical basis on the second stream)).
  */
 int tg_enable_timing(tg_ctx* ctx, int on);
 int tg_reset_timing(tg_ctx* ctx);

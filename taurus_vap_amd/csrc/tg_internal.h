@@ -239,7 +239,7 @@ struct tg_ctx {
   DevBuf d_pfw;                                   // pfaffian work
   int have_state = 0;
   int evolved = 0;   // the states are the last iteration's, their canonical basis not set
-  int tuned_n = 0, tuned_chunk = 0, tuned_nb = -1;  // sizes the GEMM tiles were tuned for
+  int tuned_n = 0, tuned_chunk = 0;                // sizes the GEMM tiles were tuned for
   // items per launch sequence: batch_limit from tg_set_batch_limit (0: none),
   // mem_chunk chosen from the free device memory by the upload (0: all items)
   int batch_limit = 0, mem_chunk = 0;

@@ -181,20 +181,6 @@ skew_part_kernel(int n, const cplx* __restrict__ G, cplx* __restrict__ H) {
   }
 }
 
-// Row bands of the reduced-overlap product K = Q^T (J Q) (nb x nb): three bands of
-// a multiple of 16 rows (even, so no canonical pair straddles two bands) from
-// nb = 128 on, one band below.
-inline int q_band_rows(int nb) { return nb >= 128 ? (((nb + 2) / 3 + 15) / 16) * 16 : nb; }
-
-// Tile variant of the reduced-overlap row bands: the two-wave 64x32 tile (8) that
-// fits beside the 384-thread inverse for 256 < n <= 384, else tuned (-1);
-// TG_Q_TILE=k forces variant k (-1: tuned).
-int q_tile_variant(int n) {
-  const char* e = std::getenv("TG_Q_TILE");   // read per call: tests set it at run time
-  if (e) return std::atoi(e);
-  return (n > 256 && n <= 384) ? 8 : -1;
-}
-
 // Per-angle traces (calculate_particle_number :78-143, calculate_expectval_energy
 // module_hamiltonian.f90:1098-1126).  out[a*9 + k]:
 // 0 prot, 1 neut, 2 Tr(rho_p^2), 3 Tr(kRL_p^T kLR_p), 4 Tr(rho_n^2),
@@ -271,59 +257,79 @@ __global__ void traces_sum_kernel(int na, int bands, const cplx* __restrict__ pa
   out[idx] = s;
 }
 
-// Q = T (e_p Pp + e_n Pn)[nemp:, nemp:] T and JQ = J Q (reduced overlap).
-// JQ row 2j = s_j Q row 2j+1, row 2j+1 = -s_j Q row 2j.  Item b0 + a is angle
-// (b0 + a) % A of state (b0 + a) / A: Pp, Pn of state s at + s * n^2, its
-// scaling Tsc and signs sgn at + s * os.
-__global__ void build_q_kernel(int n, int nemp, int nb, int na, long long b0, int A,
+// The reduced overlap's skew matrix per item is K = Q^T (J Q) + J with
+// Q = T (e_p Pp + e_n Pn)[nemp:, nemp:] T (T = diag(Tsc), J the sgn-weighted
+// symplectic unit: (J Q) row 2j = s_j Q row 2j+1, row 2j+1 = -s_j Q row 2j).
+// Q is linear in the two gauge phases with real, angle-independent parts
+// Qp = T Pp T, Qn = T Pn T, so
+//   K = e_p^2 Qp^T J Qp + e_p e_n (X - X^T) + e_n^2 Qn^T J Qn + J,  X = Qp^T J Qn
+// (J^T = -J): three real products per state, then one element-wise pass per item.
+// kq_prep_kernel: Qp, Qn, J Qp, J Qn of states st0 .. st0 + nst - 1 (state s of
+// the segment at + s nb^2; Pp, Pn of state st at + st n^2, its Tsc and sgn at
+// + st * os).
+__global__ void kq_prep_kernel(int n, int nemp, int nb, int nst, int st0,
                                const double* __restrict__ Pp0, const double* __restrict__ Pn0,
                                const double* __restrict__ Tsc0, const double* __restrict__ sgn0,
-                               long long os, const cplx* __restrict__ ph,
-                               cplx* __restrict__ Q, cplx* __restrict__ JQ) {
+                               long long os, double* __restrict__ Qp, double* __restrict__ Qn,
+                               double* __restrict__ JQp, double* __restrict__ JQn) {
   const long long mm = (long long)nb * nb;
-  long long idx = (long long)blockIdx.x * NT + threadIdx.x;
-  if (idx >= mm * na) return;
-  const long long a = idx / mm;
-  const long long st = (b0 + a) / A, ang = (b0 + a) % A;
+  const long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= mm * nst) return;
+  const int sl = (int)(idx / mm);
+  const long long st = st0 + sl;
   const double* Pp = Pp0 + st * n * n;
   const double* Pn = Pn0 + st * n * n;
   const double* Tsc = Tsc0 + st * os;
   const double* sgn = sgn0 + st * os;
   const int r = (int)(idx % mm);
   const int i = r % nb, j = r / nb;
-  const cplx ep = ph[2 * ang], en = ph[2 * ang + 1];
-  auto q = [&](int ii) {
-    const long long o = (ii + nemp) + (long long)(j + nemp) * n;
-    const double pp = Pp[o], pn = Pn[o];
-    const double s = Tsc[ii] * Tsc[j];
-    return cmk(s * (ep.x * pp + en.x * pn), s * (ep.y * pp + en.y * pn));
+  auto q = [&](const double* Px, int ii) {
+    return Tsc[ii] * Tsc[j] * Px[(ii + nemp) + (long long)(j + nemp) * n];
   };
-  Q[idx] = q(i);
-  const int pr = i >> 1;
-  const double sj = sgn[pr];
-  if ((i & 1) == 0) {
-    const cplx v = q(i + 1);
-    JQ[idx] = cmk(sj * v.x, sj * v.y);
-  } else {
-    const cplx v = q(i - 1);
-    JQ[idx] = cmk(-sj * v.x, -sj * v.y);
-  }
+  Qp[idx] = q(Pp, i);
+  Qn[idx] = q(Pn, i);
+  const double sj = sgn[i >> 1];
+  const int pi = (i & 1) ? i - 1 : i + 1;
+  const double f = (i & 1) ? -sj : sj;
+  JQp[idx] = f * q(Pp, pi);
+  JQn[idx] = f * q(Pn, pi);
 }
 
-// K += J (the sgn-weighted symplectic unit; signs of state s at sgn0 + s * os)
-__global__ void add_j_kernel(int nb, int na, long long b0, int A, const double* __restrict__ sgn0,
-                             long long os, cplx* __restrict__ K) {
-  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long pairs = (long long)(nb / 2) * na;
-  if (idx >= pairs) return;
-  const long long a = idx / (nb / 2);
-  const double* sgn = sgn0 + ((b0 + a) / A) * os;
-  const int p = (int)(idx % (nb / 2));
-  const int i = 2 * p;
+// Kpn = X - X^T of every state of the segment (nst nb x nb matrices)
+__global__ void kq_skew_kernel(int nb, int nst, const double* __restrict__ X,
+                               double* __restrict__ Kpn) {
   const long long mm = (long long)nb * nb;
-  cplx* k = K + a * mm;
-  k[i + (long long)(i + 1) * nb].x += sgn[p];
-  k[(i + 1) + (long long)i * nb].x -= sgn[p];
+  const long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= mm * nst) return;
+  const long long so = (idx / mm) * mm;
+  const int r = (int)(idx % mm);
+  const int i = r % nb, j = r / nb;
+  Kpn[idx] = X[idx] - X[so + j + (long long)i * nb];
+}
+
+// K of the segment's items (item b0 + a: angle (b0 + a) % A of state (b0 + a) / A,
+// its parts at + ((b0 + a) / A - st0) nb^2), strictly upper triangle only: the
+// Pfaffian reads nothing else.
+__global__ void kq_combine_kernel(int nb, int na, long long b0, int A, int st0,
+                                  const cplx* __restrict__ ph, const double* __restrict__ Kpp,
+                                  const double* __restrict__ Knn, const double* __restrict__ Kpn,
+                                  const double* __restrict__ sgn0, long long os,
+                                  cplx* __restrict__ K) {
+  const long long mm = (long long)nb * nb;
+  const long long idx = (long long)blockIdx.x * NT + threadIdx.x;
+  if (idx >= mm * na) return;
+  const long long a = idx / mm;
+  const int r = (int)(idx % mm);
+  const int i = r % nb, j = r / nb;
+  if (i >= j) return;
+  const long long st = (b0 + a) / A, ang = (b0 + a) % A;
+  const long long so = (st - st0) * mm;
+  const cplx ep = ph[2 * ang], en = ph[2 * ang + 1];
+  const cplx ep2 = cmul(ep, ep), epn = cmul(ep, en), en2 = cmul(en, en);
+  const double kpp = Kpp[so + r], knn = Knn[so + r], kpn = Kpn[so + r];
+  cplx v = cmk(ep2.x * kpp + epn.x * kpn + en2.x * knn, ep2.y * kpp + epn.y * kpn + en2.y * knn);
+  if (j == i + 1 && (i & 1) == 0) v.x += sgn0[st * os + (i >> 1)];
+  K[idx] = v;
 }
 
 // Rn2 = (e_p Pp + e_n Pn)[nemp:, nemp:] for the direct overlap
@@ -867,14 +873,19 @@ RunPlan make_plan(tg_ctx* c) {
   P.ovl_bytes = sizeof(double) * (size_t)S * P.os;
   // overlap work (its own buffers: the overlap runs on the second stream,
   // concurrently with the Thouless / field / H20 chain), per item of a segment
-  //   reduced: Q | JQ | K | Pfaffian work, nb^2 complex each
+  //   reduced: K | Pfaffian work, nb^2 complex each, then per state of the
+  //            segment Qp | Qn | JQp | JQn | Kpp | Knn | Xpn, nb^2 doubles each
   //   direct:  R2 | M_reg | Pfaffian work | det | info (int) | perm (n int)
   P.pfw_bytes = 0;
   if (P.need_ovl)
     for (const OvlShape& o : P.sh) {
       size_t per;
       if (o.reduced) {
-        per = sizeof(cplx) * 4 * (size_t)o.nbloc * o.nbloc;
+        const size_t mm = (size_t)o.nbloc * o.nbloc;
+        per = sizeof(cplx) * 2 * mm;
+        const size_t nst = (size_t)std::min(S, P.chunk / A + 2);   // states a segment spans
+        P.pfw_bytes = std::max(P.pfw_bytes, per * (size_t)P.chunk + sizeof(double) * 7 * mm * nst + 256);
+        continue;
       } else {
         const long long nreg = 2 * (long long)o.nbloc;
         per = sizeof(cplx) * ((size_t)o.n2 * o.n2 + 2 * (size_t)(nreg * nreg) + 1) +
@@ -989,8 +1000,7 @@ int project_upload(tg_ctx* c) {
   // first iteration at this size: autotune the tile shapes of the batched GEMMs
   // of the loop body on zeroed scratch operands (W_T, W_X -> W_Y)
   const int chunk = P.chunk;
-  const int nb0 = (P.uniform && P.sh[0].reduced && P.need_ovl) ? P.sh[0].nbloc : -1;
-  if (c->tuned_n != n || c->tuned_chunk != chunk || c->tuned_nb != nb0) {
+  if (c->tuned_n != n || c->tuned_chunk != chunk) {
     auto Wb = [&](int k) { return P.work + (size_t)k * chunk * nn; };
     const double* U = c->d_U.as<double>();
     cplx* Ta = Wb(W_T);
@@ -1005,20 +1015,8 @@ int project_upload(tg_ctx* c) {
     TG_TRY(gemm_tune_dz(s, OP_N, OP_N, n, n, n, U, n, 0, Xa, n, nn, Ya, n, nn, chunk));
     TG_TRY(gemm_tune_dz(s, OP_N, OP_T, n, n, n, U, n, 0, Xa, n, nn, Ya, n, nn, chunk));
     TG_TRY(gemm_tune_zd(s, OP_N, OP_T, n, n, n, Xa, n, nn, U, n, 0, Ya, n, nn, chunk));
-    // the reduced-overlap bands (not when their tile is forced: the variant
-    // beside the inverse at 256 < n <= 384, or TG_Q_TILE; nbloc follows the state,
-    // so a tuned band shape may be new in every iteration)
-    if (nb0 > 0 && q_tile_variant(n) < 0) {
-      const int nb = nb0, rb = q_band_rows(nb);
-      const long long mm = (long long)nb * nb;
-      for (int r0 = 0; r0 < nb; r0 += rb)
-        TG_TRY(gemm_tune_zz(s, OP_T, OP_N, std::min(rb, nb - r0), nb - r0, nb, Xa + (size_t)r0 * nb, nb,
-                            mm, Ta + (size_t)r0 * nb, nb, mm, Ya + r0 + (size_t)r0 * nb, nb, mm,
-                            chunk));
-    }
     c->tuned_n = n;
     c->tuned_chunk = chunk;
-    c->tuned_nb = nb0;
   }
   c->state_dirty = 0;
   return TG_OK;
@@ -1035,22 +1033,6 @@ int upload_overlap_setup(tg_ctx* c, hipStream_t s) {
   P = make_plan(c);
   std::vector<double> ovl = overlap_setup(c, P);
   TG_HIP(hipMemcpyAsync(P.d_T, ovl.data(), sizeof(double) * ovl.size(), hipMemcpyHostToDevice, s));
-  // tile shapes of the reduced-overlap bands for a new band size, timed on the
-  // overlap work buffer (the projection's work matrices are in use meanwhile)
-  const int nb0 = (P.uniform && P.sh[0].reduced && P.need_ovl) ? P.sh[0].nbloc : -1;
-  if (nb0 > 0 && nb0 != c->tuned_nb && q_tile_variant(P.n) < 0) {
-    const int nb = nb0, rb = q_band_rows(nb), chunk = P.chunk;
-    const long long mm = (long long)nb * nb, tot = mm * chunk;
-    cplx* Xa = c->d_pfw.as<cplx>();   // 4 nb^2 per item (reduced shape): X | T | Y
-    cplx* Ta = Xa + tot;
-    cplx* Ya = Ta + tot;
-    hipLaunchKernelGGL(fill_cplx_kernel, dim3(nblk(2 * tot)), dim3(NT), 0, s, 2 * tot, cmk(0, 0), Xa);
-    TG_HIP(hipGetLastError());
-    for (int r0 = 0; r0 < nb; r0 += rb)
-      TG_TRY(gemm_tune_zz(s, OP_T, OP_N, std::min(rb, nb - r0), nb - r0, nb, Xa + (size_t)r0 * nb, nb,
-                          mm, Ta + (size_t)r0 * nb, nb, mm, Ya + r0 + (size_t)r0 * nb, nb, mm, chunk));
-    c->tuned_nb = nb0;
-  }
   TG_HIP(hipStreamSynchronize(s));   // the host vector goes out of scope
   return TG_OK;
 }
@@ -1141,33 +1123,31 @@ int project_run(tg_ctx* c) {
       ScopedTiming tm(c, "overlap", s2);
       const int nb = o.nbloc;
       const long long mm = (long long)nb * nb;
-      cplx* Q = c->d_pfw.as<cplx>();
-      cplx* JQ = Q + (size_t)cnt * mm;
-      cplx* K = JQ + (size_t)cnt * mm;
+      cplx* K = c->d_pfw.as<cplx>();
       cplx* PW = K + (size_t)cnt * mm;
-      hipLaunchKernelGGL(build_q_kernel, dim3(nblk(mm * cnt)), dim3(NT), 0, s2, n, o.nemp, nb, cnt,
-                         b0, A, Pa, Pb, P.d_T, P.d_sg, os, P.d_ph, Q, JQ);
-      TG_HIP(hipGetLastError());
+      // the segment's states and their angle-independent parts of K
+      const int st0 = (int)(b0 / A), nst = (int)((b0 + cnt - 1) / A) - st0 + 1;
+      double* Qp = reinterpret_cast<double*>(PW + (size_t)cnt * mm);
+      double* Qn = Qp + (size_t)nst * mm;
+      double* JQp = Qn + (size_t)nst * mm;
+      double* JQn = JQp + (size_t)nst * mm;
+      double* Kpp = JQn + (size_t)nst * mm;
+      double* Knn = Kpp + (size_t)nst * mm;
+      double* Xpn = Knn + (size_t)nst * mm;
       {
-        // K = Q^T (J Q) is skew and the Pfaffian reads its upper triangle only:
-        // the product runs in bands of rows, each from its diagonal block on
-        ScopedTiming tz(c, "kernel:zgemm_q", s2);
-        const int rb = q_band_rows(nb);
-        // The bands run beside the inverse.  The 384-thread inverse (256 < n <= 384)
-        // leaves two SIMDs of each of its CUs half free, which the two-wave 64x32
-        // tile (256 VGPRs, 26 KB LDS) fits; other sizes take the tuned tile.
-        // TG_Q_TILE=k forces variant k (-1: tuned).
-        const int qtile = q_tile_variant(n);
-        for (int r0 = 0; r0 < nb; r0 += rb)
-          TG_TRY(gemm_zz(s2, OP_T, OP_N, std::min(rb, nb - r0), nb - r0, nb, one,
-                         Q + (size_t)r0 * nb, nb, mm, JQ + (size_t)r0 * nb, nb, mm, zero,
-                         K + r0 + (size_t)r0 * nb, nb, mm, cnt, 1, 1, qtile));
+        ScopedTiming tz(c, "kernel:k_parts", s2);
+        hipLaunchKernelGGL(kq_prep_kernel, dim3(nblk(mm * nst)), dim3(NT), 0, s2, n, o.nemp, nb, nst,
+                           st0, Pa, Pb, P.d_T, P.d_sg, os, Qp, Qn, JQp, JQn);
+        TG_HIP(hipGetLastError());
+        TG_TRY(gemm_dd(s2, OP_T, OP_N, nb, nb, nb, 1.0, Qp, nb, mm, JQp, nb, mm, 0.0, Kpp, nb, mm, nst));
+        TG_TRY(gemm_dd(s2, OP_T, OP_N, nb, nb, nb, 1.0, Qn, nb, mm, JQn, nb, mm, 0.0, Knn, nb, mm, nst));
+        TG_TRY(gemm_dd(s2, OP_T, OP_N, nb, nb, nb, 1.0, Qp, nb, mm, JQn, nb, mm, 0.0, Xpn, nb, mm, nst));
+        double* Kpn = JQp;   // free after the products
+        hipLaunchKernelGGL(kq_skew_kernel, dim3(nblk(mm * nst)), dim3(NT), 0, s2, nb, nst, Xpn, Kpn);
+        hipLaunchKernelGGL(kq_combine_kernel, dim3(nblk(mm * cnt)), dim3(NT), 0, s2, nb, cnt, b0, A,
+                           st0, P.d_ph, Kpp, Knn, Kpn, P.d_sg, os, K);
+        TG_HIP(hipGetLastError());
       }
-      // one-wave workgroups: they fit beside the inverse's workgroups, so the
-      // Pfaffians can start on the CUs the inverse leaves free
-      hipLaunchKernelGGL(add_j_kernel, dim3((unsigned)(((long long)(nb / 2) * cnt + 63) / 64)),
-                         dim3(64), 0, s2, nb, cnt, b0, A, P.d_sg, os, K);
-      TG_HIP(hipGetLastError());
       {
         ScopedTiming tk(c, "kernel:pfaffian", s2);
         TG_TRY(batched_pfaffian(s2, nb, K, nb, mm, cnt, PW, P.d_logabs + lo, P.d_phase + lo,

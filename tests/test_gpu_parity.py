@@ -183,15 +183,10 @@ def test_projection_own_canonical_basis():
     ctx.close()
 
 
-@pytest.mark.parametrize("band_tile", [None, "8"])
-def test_projection_emax4_vs_oracle(band_tile, monkeypatch):
+def test_projection_emax4_vs_oracle():
     """No-core e_max=4 basis (140 states, 5.5e5 stored TBMEs from the native
     producer), 3x3 angles: the GPU path against the CPU oracle on the same
-    seeded inputs (blocked inverse and reduced Pfaffian at a mid size).
-    band_tile "8": the row bands of the reduced-overlap product on the two-wave
-    64x32 tile that e_max = 6 uses beside the inverse (partial 48-row bands here)."""
-    if band_tile is not None:
-        monkeypatch.setenv("TG_Q_TILE", band_tile)
+    seeded inputs (blocked inverse and reduced Pfaffian at a mid size)."""
     from taurus_vap_amd import hamiltonian as H, host, inputs as I
     w = I.WORKLOADS["emax4"]
     h = H.workload_hamiltonian(w, 1234)
