@@ -331,28 +331,36 @@ int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const c
   TG_TRY(gemm_tune_grouped(s, two * na, Xpp, Cpp, c->pp.d_W.p, c->w_fp32,
                            c->pp.d_groups.as<GroupDesc>(), c->pp.d_tile_group.as<int>(),
                            c->pp.d_tile_n0.as<int>(), c->pp.ntiles_total, 1));
+  // the pp channel on the third stream beside the ph one: the groups are of very
+  // uneven size, so each kernel's tail leaves CUs the other can use
+  hipStream_t s3 = c->stream3;
+  TG_HIP(hipEventRecord(c->ev_fork3, s));
+  TG_HIP(hipStreamWaitEvent(s3, c->ev_fork3, 0));
+  {
+    ScopedTiming tk(c, "kernel:grouped_pp", s3);
+    TG_TRY(gemm_grouped(s3, two * na, Xpp, Cpp, c->pp.d_W.p, c->w_fp32,
+                        c->pp.d_groups.as<GroupDesc>(), c->pp.d_tile_group.as<int>(),
+                        c->pp.d_tile_n0.as<int>(), 1, c->pp.ntiles_total));
+  }
+  hipLaunchKernelGGL(scatter_tiled_kernel, tiles, dim3(NT), 0, s3, na, nn, ipp, Cpp, two * na, 0,
+                     dLR, (const cplx*)nullptr, (cplx*)nullptr);
+  if (two == 2)
+    hipLaunchKernelGGL(scatter_tiled_kernel, tiles, dim3(NT), 0, s3, na, nn, ipp, Cpp, two * na, na,
+                       dRL, (const cplx*)nullptr, (cplx*)nullptr);
+  TG_HIP(hipGetLastError());
   {
     ScopedTiming tk(c, "kernel:grouped_ph");
     TG_TRY(gemm_grouped(s, na, Xph, Cph, c->ph.d_W.p, c->w_fp32, c->ph.d_groups.as<GroupDesc>(),
                         c->ph.d_tile_group.as<int>(), c->ph.d_tile_n0.as<int>(), 0,
                         c->ph.ntiles_total));
   }
-  {
-    ScopedTiming tk(c, "kernel:grouped_pp");
-    TG_TRY(gemm_grouped(s, two * na, Xpp, Cpp, c->pp.d_W.p, c->w_fp32,
-                        c->pp.d_groups.as<GroupDesc>(), c->pp.d_tile_group.as<int>(),
-                        c->pp.d_tile_n0.as<int>(), 1, c->pp.ntiles_total));
-  }
   // Gamma (and h = Gamma + H1 when no hermitian completion follows), DeltaLR, DeltaRL
   const bool fuse_h = hsp != nullptr && !hermitian;
   hipLaunchKernelGGL(scatter_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, iph, Cph, na, 0, gamma,
                      fuse_h ? H1 : nullptr, fuse_h ? hsp : nullptr);
-  hipLaunchKernelGGL(scatter_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, ipp, Cpp, two * na, 0,
-                     dLR, (const cplx*)nullptr, (cplx*)nullptr);
-  if (two == 2)
-    hipLaunchKernelGGL(scatter_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, ipp, Cpp, two * na, na,
-                       dRL, (const cplx*)nullptr, (cplx*)nullptr);
   TG_HIP(hipGetLastError());
+  TG_HIP(hipEventRecord(c->ev_join3, s3));
+  TG_HIP(hipStreamWaitEvent(s, c->ev_join3, 0));
   if (hermitian)
     hipLaunchKernelGGL(hermitize_kernel, dim3(nblk((long long)na * nn)), dim3(NT), 0, s, n, na, nn,
                        gamma, dLR, dRL);

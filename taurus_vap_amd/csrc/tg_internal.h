@@ -209,6 +209,8 @@ struct tg_ctx {
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;                  // overlaps, concurrent with the Thouless chain
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t stream3 = nullptr;                  // the pp contraction beside the ph one
+  hipEvent_t ev_fork3 = nullptr, ev_join3 = nullptr;
   // basis
   std::vector<int> sp_2mj, sp_2mt, sp_l, sp_tr;   // internal order; sp_tr 0-based here
   std::vector<int> sp_perm, sp_iperm;             // internal -> reference index, and back

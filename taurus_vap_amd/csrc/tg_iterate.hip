@@ -940,6 +940,7 @@ int tg_iterate_canonical(tg_ctx* c, double canon_eps, double gradient_eps, doubl
     // leave nothing in flight (a wide degenerate block, TG_E_UNSUPPORTED, leaves
     // the state as it was: the caller takes the host route and tg_iterate)
     (void)hipStreamSynchronize(c->stream2);
+    (void)hipStreamSynchronize(c->stream3);
     (void)hipStreamSynchronize(c->stream);
     c->have_state = 0;
     c->evolved = 1;

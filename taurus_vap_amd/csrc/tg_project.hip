@@ -718,6 +718,7 @@ int timing_collect(tg_ctx* c) {
   if (c->pending.empty()) return TG_OK;
   TG_HIP(hipStreamSynchronize(c->stream));
   TG_HIP(hipStreamSynchronize(c->stream2));
+  TG_HIP(hipStreamSynchronize(c->stream3));
   for (auto& p : c->pending) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, p.e0, p.e1) == hipSuccess) {
@@ -1704,8 +1705,11 @@ int tg_init(tg_ctx** out, int ndim, const int32_t* sp_2mj, const int32_t* sp_2mt
   int r = TG_OK;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork3, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming) != hipSuccess) {
     tg_finalize(c);
     tg_set_error("tg_init: stream / event creation failed");
     return TG_E_HIP;
@@ -1746,8 +1750,12 @@ int tg_finalize(tg_ctx* c) {
     for (DevBuf* b : {&ch->d_inv, &ch->d_groups, &ch->d_W, &ch->d_tile_group, &ch->d_tile_n0})
       b->release();
   if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  if (c->stream3) (void)hipStreamSynchronize(c->stream3);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_fork3) (void)hipEventDestroy(c->ev_fork3);
+  if (c->ev_join3) (void)hipEventDestroy(c->ev_join3);
+  if (c->stream3) (void)hipStreamDestroy(c->stream3);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
