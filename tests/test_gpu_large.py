@@ -114,13 +114,15 @@ def test_emax6_projection_fp32_storage():
     assert abs(e32 - e64_ref) < 1e-3
 
 
+@pytest.mark.parametrize("mode", ["sequential", "overlapped"])
 @pytest.mark.parametrize("case", ["emax6", "emax4_11"])
-def test_large_iterations_match_reference(case):
+def test_large_iterations_match_reference(case, mode):
     """tg_iterate (constraint_20, projection, multipliers, gradient step,
     adjust_constraints) from the reference's first state, with the reference's
     constraint operators (e_max = 6: <Z>, <N>, Q20, Jx); the canonical basis of
-    the evolved state is built on the device (tg_canonical_basis) between the
-    two iterations."""
+    the evolved state is built on the device between the two iterations:
+    "sequential" by tg_canonical_basis, "overlapped" inside tg_iterate_canonical
+    (the bench's step: the canonical basis beside the second projection)."""
     g = load_golden(case)
     assert int(g["gradient_type"]) == 0
     n = int(g["ndim"])
@@ -135,9 +137,14 @@ def test_large_iterations_match_reference(case):
         ctx.set_state(U, V, cb["D"], cb["nocc"], cb["nemp"], cb["ovac"], cb["voveru"],
                       int(g["Mphip"]), int(g["Mphin"]), float(g["facpi"]),
                       float(g["valence_Z"]), float(g["valence_N"]), float(g["factor_delta"]))
+        if mode == "overlapped":
+            ctx.enable_timing(True)
         for it in range(int(g["niter"])):
-            print("%s iteration %d" % (case, it + 1), flush=True)
-            r = ctx.iterate(0.0)
+            print("%s %s iteration %d" % (case, mode, it + 1), flush=True)
+            if mode == "overlapped" and it > 0:
+                r, _ = ctx.iterate_canonical()
+            else:
+                r = ctx.iterate(0.0)
             e = (r["ener"] / r["over"]).real
             assert abs(e - g["hist_ener"][it]) <= 1e-8 * abs(g["hist_ener"][it]), (it, e)
             assert abs((r["prot"] / r["over"]).real - g["hist_prot"][it]) <= 1e-8
@@ -148,6 +155,9 @@ def test_large_iterations_match_reference(case):
                 lam = np.asarray(g["lagrange_lambda1"])
                 tol = np.maximum(1e-8 * np.abs(lam), 1e-10)
                 assert np.all(np.abs(r["lagrange_lambda1"] - lam) <= tol), (r["lagrange_lambda1"], lam)
-            ctx.canonical_basis()
+            if mode == "sequential":
+                ctx.canonical_basis()
+        if mode == "overlapped":
+            assert ctx.timing("canonical_basis_beside")[1] == int(g["niter"]) - 1
     finally:
         ctx.close()
