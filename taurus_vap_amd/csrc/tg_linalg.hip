@@ -338,6 +338,274 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
   }
 }
 
+// ---------------------------------------------------------------------------
+// Gauss-Jordan inverse with 32-column panels, one thread per row (n <= TH).
+// A thread holds its row of the panel as two 16-column register halves A and
+// B.  Half A is reduced as in gj_blocked_kernel; B's columns then receive A's
+// interchanges and rank-16 update (16 x 16 source values through LDS) and are
+// reduced in turn; B's interchanges reach the A half through two LDS tables
+// (the A half of the block-B rows, and of the rows moved into block B).  The
+// other columns then take one rank-32 update, c <- P c + (T - E)(P c)[panel],
+// so a matrix is streamed n/32 times instead of n/16.  While a group of U
+// columns is updated, the next group's loads are in flight.  The pivots are
+// those of gj_blocked_kernel (first maximum of |Re| + |Im|, as IZAMAX in
+// ZGETRF).
+// ---------------------------------------------------------------------------
+template <int TH>
+__device__ __forceinline__ void gj_reduce16(cplx (&pr)[16], int k0, int w, int n, int t,
+                                            cplx (*sw)[2][16], double (*wv)[TH / 64],
+                                            int (*wi)[TH / 64], int* perm, cplx& d, int& inf) {
+  const bool own = t < n;
+  for (int mm = 0; mm < w; ++mm) {
+    const int k = k0 + mm;
+    const int buf = mm & 1;
+    cplx pv = cmk(0.0, 0.0);
+#pragma unroll
+    for (int m = 0; m < 16; ++m)
+      if (m == mm) pv = pr[m];
+    double v = (own && t >= k) ? cabs1(pv) : -1.0;
+    int vi = (own && t >= k) ? t : n;
+    wave_argmax(v, vi);
+    if ((t & 63) == 0) {
+      wv[buf][t >> 6] = v;
+      wi[buf][t >> 6] = vi;
+    }
+    __syncthreads();
+    double mx = wv[buf][0];
+    int p = wi[buf][0];
+    for (int q = 1; q < TH / 64; ++q)
+      if (wv[buf][q] > mx || (wv[buf][q] == mx && wi[buf][q] < p)) {
+        mx = wv[buf][q];
+        p = wi[buf][q];
+      }
+    if (!(mx > 0.0) || p < k || p >= n) {
+      if (inf == 0) inf = k + 1;
+      if (t == 0) perm[k] = k;
+      d = cmk(0.0, 0.0);
+      continue;
+    }
+    if (t == 0) perm[k] = p;
+    if (t == p)
+#pragma unroll
+      for (int m = 0; m < 16; ++m) sw[buf][0][m] = pr[m];
+    if (t == k && p != k)
+#pragma unroll
+      for (int m = 0; m < 16; ++m) sw[buf][1][m] = pr[m];
+    __syncthreads();
+    const cplx piv = sw[buf][0][mm];
+    const cplx pinv = cinv(piv);
+    if (p != k) d = cmk(-d.x, -d.y);
+    d = cmul(d, piv);
+    if (t == p && p != k)
+#pragma unroll
+      for (int m = 0; m < 16; ++m) pr[m] = sw[buf][1][m];
+    if (own) {
+      cplx f = cmk(0.0, 0.0);
+#pragma unroll
+      for (int m = 0; m < 16; ++m)
+        if (m == mm) f = pr[m];
+      const bool pivrow = (t == k);
+      const cplx g = pivrow ? cmk(-pinv.x, -pinv.y) : cmul(f, pinv);
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        if (m == mm) {
+          pr[m] = cmk(-g.x, -g.y);
+        } else {
+          const cplx r = sw[buf][0][m];
+          cplx v2 = pivrow ? cmk(0.0, 0.0) : pr[m];
+          cfms(v2, g, r);
+          pr[m] = v2;
+        }
+      }
+    }
+  }
+}
+
+// source row of row t under the interchanges perm[k0 .. k0 + w)
+__device__ __forceinline__ int gj_source(const int* perm, int k0, int w, int t) {
+  int src = t;
+  for (int m = w - 1; m >= 0; --m) {
+    const int kk = k0 + m, pp = perm[kk];
+    if (src == kk) src = pp;
+    else if (src == pp) src = kk;
+  }
+  return src;
+}
+
+template <int TH, int WIN, int U>
+__global__ void __launch_bounds__(TH)
+gj_wide_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* __restrict__ det,
+               int* __restrict__ info, int* __restrict__ perm_all) {
+  constexpr int H = 16;
+  __shared__ cplx sw[2][2][H];
+  __shared__ cplx Pw[2 * H][WIN + U];   // U columns of slack: no clamping below
+  __shared__ cplx tab[2][H][H];
+  __shared__ int srcs[2 * H];
+  __shared__ double wv[2][TH / 64];
+  __shared__ int wi[2][TH / 64];
+  __shared__ int perm[TH];
+  const int t = threadIdx.x;
+  const bool own = t < n;
+  cplx* a = Aall + (long long)blockIdx.x * sA;
+  cplx d = cmk(1.0, 0.0);
+  int inf = 0;
+  for (int k0 = 0; k0 < n; k0 += 2 * H) {
+    const int w = min(2 * H, n - k0), wA = min(H, w), wB = w - wA, kB = k0 + H;
+    cplx pA[H], pB[H];
+#pragma unroll
+    for (int m = 0; m < H; ++m)
+      pA[m] = (own && m < wA) ? a[t + (long long)(k0 + m) * lda] : cmk(0.0, 0.0);
+    gj_reduce16<TH>(pA, k0, wA, n, t, sw, wv, wi, perm, d, inf);
+#pragma unroll
+    for (int m = 0; m < H; ++m) pB[m] = cmk(0.0, 0.0);
+    if (wB > 0) {
+      __syncthreads();   // perm of half A
+      const int srcA = gj_source(perm, k0, wA, t);
+      const bool inA = own && t >= k0 && t < k0 + wA;
+      if (inA) srcs[t - k0] = srcA;
+      __syncthreads();
+      // (P_A c)[A rows] of B's columns (zero past wB), and this row's P_A c
+      // (zero in block A); wA = H here
+      for (int e = t; e < H * H; e += TH) {
+        const int m = e % H, c = e / H;
+        Pw[m][c] = (c < wB) ? a[srcs[m] + (long long)(kB + c) * lda] : cmk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int m = 0; m < H; ++m)
+        if (own && !inA && m < wB) pB[m] = a[srcA + (long long)(kB + m) * lda];
+      __syncthreads();
+      if (own) {
+#pragma unroll
+        for (int m = 0; m < H; ++m) {
+          const cplx am = pA[m];
+#pragma unroll
+          for (int c = 0; c < H; ++c) cfma(pB[c], am, Pw[m][c]);   // rows m >= wA are zero
+        }
+      }
+      gj_reduce16<TH>(pB, kB, wB, n, t, sw, wv, wi, perm, d, inf);
+      __syncthreads();   // perm of half B
+      // B's interchanges and rank-16 update on the A half
+      const int srcB = gj_source(perm, kB, wB, t);
+      const bool inB = own && t >= kB && t < kB + wB;
+      if (inB) {
+        srcs[H + t - kB] = srcB;
+#pragma unroll
+        for (int m = 0; m < H; ++m) tab[0][t - kB][m] = pA[m];
+      }
+      __syncthreads();
+      if (own && t >= kB)
+        for (int q = 0; q < wB; ++q)
+          if (srcs[H + q] == t) {
+#pragma unroll
+            for (int m = 0; m < H; ++m) tab[1][q][m] = pA[m];
+          }
+      for (int e = t; e < (H - wB) * H; e += TH) tab[1][wB + e / H][e % H] = cmk(0.0, 0.0);
+      __syncthreads();
+      if (own) {
+        if (inB) {
+#pragma unroll
+          for (int m = 0; m < H; ++m) pA[m] = cmk(0.0, 0.0);
+        } else if (srcB != t) {
+#pragma unroll
+          for (int m = 0; m < H; ++m) pA[m] = tab[0][srcB - kB][m];
+        }
+#pragma unroll
+        for (int q = 0; q < H; ++q) {   // pB[q] = 0 for q >= wB
+          const cplx bq = pB[q];
+#pragma unroll
+          for (int m = 0; m < H; ++m) cfma(pA[m], bq, tab[1][q][m]);
+        }
+      }
+    }
+    __syncthreads();   // perm of the panel; LDS tables read
+#pragma unroll
+    for (int m = 0; m < H; ++m) {
+      if (own && m < wA) a[t + (long long)(k0 + m) * lda] = pA[m];
+      if (own && m < wB) a[t + (long long)(kB + m) * lda] = pB[m];
+    }
+    const int src = gj_source(perm, k0, w, t);
+    const int blk = t - k0;
+    const bool inblk = own && blk >= 0 && blk < w;
+    if (inblk) srcs[blk] = src;
+    __syncthreads();
+    // interchanges and rank-w update of the other columns, WIN at a time; Pw row
+    // m < wA is panel position k0 + m, row H + q position kB + q
+    const int nrest = n - w;
+    for (int c0 = 0; c0 < nrest; c0 += WIN) {
+      const int cw = min(WIN, nrest - c0);
+      // all 2H rows are written (rows past the panel width as zeros), so the
+      // update below needs no bounds on m; the U slack columns repeat the last
+      // column, so a group that runs past the window computes that column again
+      for (int e = t; e < 2 * H * (cw + U); e += TH) {
+        const int m2 = e % (2 * H), c = e / (2 * H);
+        const int pos = (m2 < H) ? (m2 < wA ? m2 : -1) : (m2 - H < wB ? m2 : -1);
+        const int jj = c0 + min(c, cw - 1);
+        Pw[m2][c] = (pos >= 0) ? a[srcs[pos] + (long long)((jj < k0) ? jj : jj + w) * lda]
+                               : cmk(0.0, 0.0);
+      }
+      __syncthreads();
+      if (own && !inblk && src != t)
+        for (int c = 0; c < cw; ++c) {
+          const int jj = c0 + c;
+          const long long j = (jj < k0) ? jj : jj + w;
+          a[t + j * lda] = a[src + j * lda];
+        }
+      __syncthreads();
+      if (own) {
+        // column c of the window, clamped into it: a group that runs past the
+        // window loads, computes and stores its last column again (same value),
+        // so loads and stores are unconditional; block rows start from zero
+        auto col = [&](int c) -> cplx* {
+          const int jj = c0 + min(c, cw - 1);
+          return a + t + (long long)((jj < k0) ? jj : jj + w) * lda;
+        };
+        const cplx zero = cmk(0.0, 0.0);
+        cplx nx[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) nx[u] = *col(u);
+        // one basic block per group, so only the LDS operands of a few panel
+        // columns are live at a time
+#pragma unroll 1
+        for (int c = 0; c <= cw - 1; c += U) {
+          cplx acc[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) acc[u] = inblk ? zero : nx[u];
+#pragma unroll
+          for (int u = 0; u < U; ++u) nx[u] = *col(c + U + u);
+#pragma unroll
+          for (int m = 0; m < H; ++m) {
+            const cplx am = pA[m], bm = pB[m];
+            const cplx* ra = &Pw[m][c];
+            const cplx* rb = &Pw[H + m][c];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              cfma(acc[u], am, ra[u]);
+              cfma(acc[u], bm, rb[u]);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) *col(c + u) = acc[u];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (own)
+    for (int k = n - 1; k >= 0; --k) {
+      const int p = perm[k];
+      if (p != k) {
+        cplx x = a[t + (long long)k * lda];
+        a[t + (long long)k * lda] = a[t + (long long)p * lda];
+        a[t + (long long)p * lda] = x;
+      }
+    }
+  for (int k = t; k < n; k += TH) perm_all[(long long)blockIdx.x * n + k] = perm[k];
+  if (t == 0) {
+    det[blockIdx.x] = d;
+    info[blockIdx.x] = inf;
+  }
+}
+
 __global__ void __launch_bounds__(NT)
 pfaffian_kernel(int n, const cplx* __restrict__ Aall, int lda, long long sA,
                 cplx* __restrict__ work, double* __restrict__ logabs, cplx* __restrict__ phase,
@@ -837,18 +1105,26 @@ int batched_inverse(hipStream_t s, int n, cplx* A, int lda, long long sA, int ba
   if (n <= 0 || batch <= 0) return TG_OK;
   if (n <= GJ_THREADS) {
     // one thread per row in the smallest workgroup that holds the rows (the
-    // LDS window shrinks with it, so small matrices share a CU); TG_GJ=2: two
-    // threads per row (the second mirrors the panel reduction and takes half
-    // of the column updates), n <= 384
+    // LDS window shrinks with it, so small matrices share a CU); above 256 rows
+    // the 32-column panels of gj_wide_kernel.  TG_GJ=1: 16-column panels at
+    // every order; TG_GJ=2: 16-column panels with two threads per row (the
+    // second mirrors the panel reduction and takes half of the column updates),
+    // n <= 384
     static const int rp = [] {
       const char* e = std::getenv("TG_GJ");
-      return (e && std::atoi(e) == 2) ? 2 : 1;
+      return e ? std::atoi(e) : 0;
     }();
 #define TG_GJ_LAUNCH(RP, TH, WIN)                                                              \
   hipLaunchKernelGGL((gj_blocked_kernel<16, RP, TH, WIN>), dim3(batch), dim3(TH * RP), 0, s, n, A, \
                      lda, sA, det, info, perm)
     if (rp == 2 && n <= 384)
       TG_GJ_LAUNCH(2, 384, GJ_WIN);
+    else if (rp != 1 && n > 256 && n <= 384)
+      hipLaunchKernelGGL((gj_wide_kernel<384, 256, 4>), dim3(batch), dim3(384), 0, s, n, A, lda, sA,
+                         det, info, perm);
+    else if (rp != 1 && n > 384)
+      hipLaunchKernelGGL((gj_wide_kernel<512, 256, 4>), dim3(batch), dim3(512), 0, s, n, A, lda, sA,
+                         det, info, perm);
     else if (n <= 64)
       TG_GJ_LAUNCH(1, 64, 64);
     else if (n <= 128)

@@ -268,11 +268,12 @@ def test_adjoint_gradient_emax4_finite_difference():
 def test_batched_inverse_large_orders():
     """tg_inverse (the ZGETRF + ZGETRI pair of calculate_thouless) at the orders of
     the e_max = 4 / 6 inverses and up to 512 (each workgroup size of the blocked
-    Gauss-Jordan kernel): A A^-1 = 1, the LAPACK inverse and determinant, and a
-    singular matrix's info."""
+    Gauss-Jordan kernels; above 256 the 32-column panels, whose last panel has 1,
+    16, 17 or 32 columns at n = 257, 272, 273, 288): A A^-1 = 1, the LAPACK
+    inverse and determinant, and singular matrices' info."""
     from taurus_vap_amd import gpu
     rng = np.random.default_rng(21)
-    for n in (24, 64, 128, 129, 140, 300, 336, 384, 512):
+    for n in (24, 64, 128, 129, 140, 257, 272, 273, 288, 300, 336, 384, 512):
         # entries of variance 1/n keep the determinant within the FP64 range
         A = (rng.normal(size=(3, n, n)) + 1j * rng.normal(size=(3, n, n))) / np.sqrt(2.0 * n)
         print("inverse n=%d" % n, flush=True)
@@ -286,8 +287,10 @@ def test_batched_inverse_large_orders():
             sign, logdet = np.linalg.slogdet(A[k])
             assert abs(np.log(abs(det[k])) - logdet) < 1e-10 * n, (n, k)
             assert abs(det[k] / abs(det[k]) - sign) < 1e-10 * n, (n, k)
-    # a zero column: info = first zero pivot column (1-based), det = 0
-    A = rng.normal(size=(200, 200)) + 0j
-    A[:, 57] = 0
-    _, det, info = gpu.inverse(A)
-    assert info == 58 and det == 0
+    # a zero column: info = first zero pivot column (1-based), det = 0; at
+    # n = 336 in the first and in the second half of a 32-column panel
+    for n, z in ((200, 57), (336, 40), (336, 57), (336, 335)):
+        A = rng.normal(size=(n, n)) + 0j
+        A[:, z] = 0
+        _, det, info = gpu.inverse(A)
+        assert info == z + 1 and det == 0, (n, z, info, det)
