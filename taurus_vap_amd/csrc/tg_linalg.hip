@@ -432,7 +432,16 @@ __device__ __forceinline__ int gj_source(const int* perm, int k0, int w, int t) 
   return src;
 }
 
-template <int TH, int WIN, int U>
+typedef double gj_d4 __attribute__((ext_vector_type(4)));
+constexpr int GJ_MFQ = 3;   // row blocks per wave in the matrix-core update (TH = 512)
+
+// MF (TH = 512, n <= 384): the rank-32 update of the other columns as 16 x 16
+// tiles of v_mfma_f64_16x16x4f64 (U = 16 then: one column tile of slack).  The
+// tile computes C^T = Pw^T W^T, so a lane holds 16 consecutive rows of four
+// columns (coalesced loads and stores) and each LDS operand serves 48 rows
+// instead of one; with the next tile's loads in flight during the products,
+// eight waves keep about four times the bytes of the scalar update in flight.
+template <int TH, int WIN, int U, bool MF>
 __global__ void __launch_bounds__(TH)
 gj_wide_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* __restrict__ det,
                int* __restrict__ info, int* __restrict__ perm_all) {
@@ -527,6 +536,39 @@ gj_wide_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* __re
     const int blk = t - k0;
     const bool inblk = own && blk >= 0 && blk < w;
     if (inblk) srcs[blk] = src;
+    // MF: matrix-core operands of W.  Wave v updates row blocks v + 8 q (q < 3,
+    // rows 16 (v + 8 q) + li), so every SIMD holds six of the 24 blocks of
+    // n <= 384; fr/fi[q][kk] = W(16 (v + 8 q) + li, 4 kk + lq), moved between
+    // waves through an n x 16 slab of the (idle) Pw area per half
+    const int lane = t & 63, li = lane & 15, lq = lane >> 4;
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    double fr[GJ_MFQ][8], fi[GJ_MFQ][8];
+    if constexpr (MF) {
+      cplx* slab = &Pw[0][0];
+#pragma unroll
+      for (int m = 0; m < H; ++m) slab[m * TH + t] = pA[m];
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < GJ_MFQ; ++q)
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          const cplx v = slab[(4 * kk + lq) * TH + 16 * (wv + 8 * q) + li];
+          fr[q][kk] = v.x;
+          fi[q][kk] = v.y;
+        }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < H; ++m) slab[m * TH + t] = pB[m];
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < GJ_MFQ; ++q)
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          const cplx v = slab[(4 * kk + lq) * TH + 16 * (wv + 8 * q) + li];
+          fr[q][4 + kk] = v.x;
+          fi[q][4 + kk] = v.y;
+        }
+    }
     __syncthreads();
     // interchanges and rank-w update of the other columns, WIN at a time; Pw row
     // m < wA is panel position k0 + m, row H + q position kB + q
@@ -551,7 +593,78 @@ gj_wide_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* __re
           a[t + j * lda] = a[src + j * lda];
         }
       __syncthreads();
-      if (own) {
+      if constexpr (MF) {
+        // this lane's elements of tile jt: rows 16 (wv + 8 q) + li, columns
+        // jt + lq + 4 r (loads clamped into the matrix and the window; stored
+        // only inside both); block rows start from zero.  The next tile's
+        // values are loaded while the current tile's products run.
+        int rowc[GJ_MFQ];
+        bool ld[GJ_MFQ], st[GJ_MFQ];
+#pragma unroll
+        for (int q = 0; q < GJ_MFQ; ++q) {
+          const int row = 16 * (wv + 8 * q) + li;
+          rowc[q] = min(row, n - 1);
+          st[q] = row < n;
+          ld[q] = row < n && !(row >= k0 && row < k0 + w);
+        }
+        auto colo = [&](int c) -> long long {
+          const int jj = c0 + min(c, cw - 1);
+          return (long long)((jj < k0) ? jj : jj + w) * lda;
+        };
+        cplx nx[GJ_MFQ][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const long long o = colo(lq + 4 * r);
+#pragma unroll
+          for (int q = 0; q < GJ_MFQ; ++q) nx[q][r] = a[rowc[q] + o];
+        }
+#pragma unroll 1
+        for (int jt = 0; jt < cw; jt += 16) {
+          gj_d4 dr[GJ_MFQ], di[GJ_MFQ];
+#pragma unroll
+          for (int q = 0; q < GJ_MFQ; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              dr[q][r] = ld[q] ? nx[q][r].x : 0.0;
+              di[q][r] = ld[q] ? nx[q][r].y : 0.0;
+            }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const long long o = colo(jt + 16 + lq + 4 * r);
+#pragma unroll
+            for (int q = 0; q < GJ_MFQ; ++q) nx[q][r] = a[rowc[q] + o];
+          }
+#pragma unroll
+          for (int kk = 0; kk < 8; ++kk) {
+            const cplx pv = Pw[4 * kk + lq][jt + li];
+#pragma unroll
+            for (int q = 0; q < GJ_MFQ; ++q) {
+              dr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv.x, fr[q][kk], dr[q], 0, 0, 0);
+              dr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-pv.y, fi[q][kk], dr[q], 0, 0, 0);
+              di[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv.x, fi[q][kk], di[q], 0, 0, 0);
+              di[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv.y, fr[q][kk], di[q], 0, 0, 0);
+            }
+          }
+          // the empty asm statements keep the products ahead of the guarded stores
+#pragma unroll
+          for (int q = 0; q < GJ_MFQ; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              double x = dr[q][r], y = di[q][r];
+              asm volatile("" : "+v"(x), "+v"(y));
+              dr[q][r] = x;
+              di[q][r] = y;
+            }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const long long o = colo(jt + lq + 4 * r);
+            const bool cok = jt + lq + 4 * r < cw;
+#pragma unroll
+            for (int q = 0; q < GJ_MFQ; ++q)
+              if (st[q] && cok) a[rowc[q] + o] = cmk(dr[q][r], di[q][r]);
+          }
+        }
+      } else if (own) {
         // column c of the window, clamped into it: a group that runs past the
         // window loads, computes and stores its last column again (same value),
         // so loads and stores are unconditional; block rows start from zero
@@ -1119,12 +1232,15 @@ int batched_inverse(hipStream_t s, int n, cplx* A, int lda, long long sA, int ba
                      lda, sA, det, info, perm)
     if (rp == 2 && n <= 384)
       TG_GJ_LAUNCH(2, 384, GJ_WIN);
+    else if (rp == 4 && n > 256 && n <= 384)
+      hipLaunchKernelGGL((gj_wide_kernel<384, 256, 4, false>), dim3(batch), dim3(384), 0, s, n, A,
+                         lda, sA, det, info, perm);
     else if (rp != 1 && n > 256 && n <= 384)
-      hipLaunchKernelGGL((gj_wide_kernel<384, 256, 4>), dim3(batch), dim3(384), 0, s, n, A, lda, sA,
-                         det, info, perm);
+      hipLaunchKernelGGL((gj_wide_kernel<512, 256, 16, true>), dim3(batch), dim3(512), 0, s, n, A,
+                         lda, sA, det, info, perm);
     else if (rp != 1 && n > 384)
-      hipLaunchKernelGGL((gj_wide_kernel<512, 256, 4>), dim3(batch), dim3(512), 0, s, n, A, lda, sA,
-                         det, info, perm);
+      hipLaunchKernelGGL((gj_wide_kernel<512, 256, 4, false>), dim3(batch), dim3(512), 0, s, n, A,
+                         lda, sA, det, info, perm);
     else if (n <= 64)
       TG_GJ_LAUNCH(1, 64, 64);
     else if (n <= 128)
