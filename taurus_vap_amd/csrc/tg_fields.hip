@@ -300,7 +300,8 @@ int contraction_build(tg_ctx* c, int64_t ntb, const int16_t* abcd, const void* H
 }
 
 int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const cplx* kRL,
-                    cplx* gamma, cplx* dLR, cplx* dRL, int hermitian, const cplx* H1, cplx* hsp) {
+                    cplx* gamma, cplx* dLR, cplx* dRL, int hermitian, const cplx* H1, cplx* hsp,
+                    bool defer_join) {
   const int n = c->n;
   const long long nn = (long long)n * n;
   hipStream_t s = c->stream;
@@ -317,25 +318,34 @@ int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const c
   const int* iph = c->ph.d_inv.as<int>();
   const int* ipp = c->pp.d_inv.as<int>();
   const dim3 tiles((unsigned)((nn + TL - 1) / TL), (unsigned)((na + TA - 1) / TA));
-  hipLaunchKernelGGL(gather_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, iph, rho, na, 0, Xph);
-  hipLaunchKernelGGL(gather_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, ipp, kLR, two * na, 0,
-                     Xpp);
-  if (two == 2)
-    hipLaunchKernelGGL(gather_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, ipp, kRL, two * na, na,
-                       Xpp);
-  TG_HIP(hipGetLastError());
-  // first call for this batch size: pick the tile shapes (outputs recomputed below)
-  TG_TRY(gemm_tune_grouped(s, na, Xph, Cph, c->ph.d_W.p, c->w_fp32, c->ph.d_groups.as<GroupDesc>(),
-                           c->ph.d_tile_group.as<int>(), c->ph.d_tile_n0.as<int>(),
-                           c->ph.ntiles_total, 0));
-  TG_TRY(gemm_tune_grouped(s, two * na, Xpp, Cpp, c->pp.d_W.p, c->w_fp32,
-                           c->pp.d_groups.as<GroupDesc>(), c->pp.d_tile_group.as<int>(),
-                           c->pp.d_tile_n0.as<int>(), c->pp.ntiles_total, 1));
-  // the pp channel on the third stream beside the ph one: the groups are of very
-  // uneven size, so each kernel's tail leaves CUs the other can use
+  // the pp channel (gathers, product, scatters) on the third stream beside the ph
+  // one: the groups are of very uneven size, so each product's tail leaves CUs the
+  // other can use, and the gathers and scatters (HBM-bound) overlap products
   hipStream_t s3 = c->stream3;
   TG_HIP(hipEventRecord(c->ev_fork3, s));
   TG_HIP(hipStreamWaitEvent(s3, c->ev_fork3, 0));
+  hipLaunchKernelGGL(gather_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, iph, rho, na, 0, Xph);
+  hipLaunchKernelGGL(gather_tiled_kernel, tiles, dim3(NT), 0, s3, na, nn, ipp, kLR, two * na, 0,
+                     Xpp);
+  if (two == 2)
+    hipLaunchKernelGGL(gather_tiled_kernel, tiles, dim3(NT), 0, s3, na, nn, ipp, kRL, two * na, na,
+                       Xpp);
+  TG_HIP(hipGetLastError());
+  if (!gemm_grouped_tuned(na, c->ph.ntiles_total, 0) ||
+      !gemm_grouped_tuned(two * na, c->pp.ntiles_total, 1)) {
+    // first call for this batch size: both channels' tile shapes are timed on the
+    // context stream alone (outputs recomputed below)
+    TG_HIP(hipEventRecord(c->ev_join3, s3));
+    TG_HIP(hipStreamWaitEvent(s, c->ev_join3, 0));
+    TG_TRY(gemm_tune_grouped(s, na, Xph, Cph, c->ph.d_W.p, c->w_fp32,
+                             c->ph.d_groups.as<GroupDesc>(), c->ph.d_tile_group.as<int>(),
+                             c->ph.d_tile_n0.as<int>(), c->ph.ntiles_total, 0));
+    TG_TRY(gemm_tune_grouped(s, two * na, Xpp, Cpp, c->pp.d_W.p, c->w_fp32,
+                             c->pp.d_groups.as<GroupDesc>(), c->pp.d_tile_group.as<int>(),
+                             c->pp.d_tile_n0.as<int>(), c->pp.ntiles_total, 1));
+    TG_HIP(hipEventRecord(c->ev_fork3, s));
+    TG_HIP(hipStreamWaitEvent(s3, c->ev_fork3, 0));
+  }
   {
     ScopedTiming tk(c, "kernel:grouped_pp", s3);
     TG_TRY(gemm_grouped(s3, two * na, Xpp, Cpp, c->pp.d_W.p, c->w_fp32,
@@ -359,8 +369,8 @@ int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const c
   hipLaunchKernelGGL(scatter_tiled_kernel, tiles, dim3(NT), 0, s, na, nn, iph, Cph, na, 0, gamma,
                      fuse_h ? H1 : nullptr, fuse_h ? hsp : nullptr);
   TG_HIP(hipGetLastError());
-  TG_HIP(hipEventRecord(c->ev_join3, s3));
-  TG_HIP(hipStreamWaitEvent(s, c->ev_join3, 0));
+  if (defer_join && two == 2 && fuse_h) return TG_OK;
+  TG_TRY(contraction_join(c));
   if (hermitian)
     hipLaunchKernelGGL(hermitize_kernel, dim3(nblk((long long)na * nn)), dim3(NT), 0, s, n, na, nn,
                        gamma, dLR, dRL);
@@ -371,5 +381,11 @@ int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const c
     hipLaunchKernelGGL(add_h1_kernel, dim3(nblk((long long)na * nn)), dim3(NT), 0, s, nn, na, H1,
                        gamma, hsp);
   TG_HIP(hipGetLastError());
+  return TG_OK;
+}
+
+int contraction_join(tg_ctx* c) {
+  TG_HIP(hipEventRecord(c->ev_join3, c->stream3));
+  TG_HIP(hipStreamWaitEvent(c->stream, c->ev_join3, 0));
   return TG_OK;
 }

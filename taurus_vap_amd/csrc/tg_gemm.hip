@@ -336,6 +336,11 @@ int gemm_tune_zd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, const c
   });
 }
 
+bool gemm_grouped_tuned(int M, int ntiles_total, int tag) {
+  if (!sel().mfma || sel().grouped >= 0) return true;
+  return choices().count(GemmKey{K_GROUPED, tag, 0, M, ntiles_total, 0, 1}) != 0;
+}
+
 int gemm_tune_grouped(hipStream_t s, int M, const cplx* X, cplx* C, const void* W, int w_is_fp32,
                       const GroupDesc* groups_dev, const int* tile_group_dev,
                       const int* tile_n0_dev, int ntiles_total, int tag) {

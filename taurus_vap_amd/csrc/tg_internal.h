@@ -139,6 +139,8 @@ int gemm_tune_zd(hipStream_t s, TgOp opa, TgOp opb, int M, int N, int K, const c
 int gemm_tune_grouped(hipStream_t s, int M, const cplx* X, cplx* C, const void* W, int w_is_fp32,
                       const GroupDesc* groups_dev, const int* tile_group_dev,
                       const int* tile_n0_dev, int ntiles_total, int tag);
+// true when gemm_tune_grouped has nothing left to time for this signature
+bool gemm_grouped_tuned(int M, int ntiles_total, int tag);
 // tile variant the dispatcher uses for a signature (kind: 0 zz, 1 dz, 2 zd, 3 grouped)
 int gemm_tuned_variant(int kind, TgOp opa, TgOp opb, int M, int N, int K, int batch);
 
@@ -362,6 +364,10 @@ int contraction_build(tg_ctx* c, int64_t ntb, const int16_t* abcd, const void* H
 // hermitian != 0 reproduces calculate_fields_diag (Gamma made hermitian from its
 // upper triangle, DeltaRL = conj(DeltaLR)).  With hsp != nullptr it also writes
 // hsp = gamma + H1 (H1 one complex n*n matrix shared by the angles).
+// The pp channel runs on the third stream.  defer_join (two pp right-hand sides,
+// no hermitian completion, hsp given): gamma and hsp are complete on the context
+// stream on return, deltaLR and deltaRL only after contraction_join.
 int contraction_run(tg_ctx* c, int na, const cplx* rho, const cplx* kLR, const cplx* kRL,
                     cplx* gamma, cplx* dLR, cplx* dRL, int hermitian,
-                    const cplx* H1 = nullptr, cplx* hsp = nullptr);
+                    const cplx* H1 = nullptr, cplx* hsp = nullptr, bool defer_join = false);
+int contraction_join(tg_ctx* c);

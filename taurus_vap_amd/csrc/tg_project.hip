@@ -1266,11 +1266,27 @@ int project_run(tg_ctx* c) {
       TG_TRY(gemm_zd(s, OP_N, OP_T, n, n, n, one, Vt, n, nn, Uc, n, nn, zero, kLR, n, nn, na, 1, A));
       TG_TRY(gemm_dz(s, OP_N, OP_T, n, n, n, one, Vc, n, nn, Ut, n, nn, zero, kRL, n, nn, na, A, 1));
     }
-    // fields
+    // fields; in the projection the pp channel's scatters (third stream) are joined
+    // after the first H20 product, which needs only h
+    const bool split = proj;
     {
       ScopedTiming tm(c, "contraction");
       TG_TRY(contraction_run(c, na, rho, kLR, proj ? kRL : nullptr, G, dLR, dRL, proj ? 0 : 1,
-                             c->d_H1.as<cplx>(), hsp));
+                             c->d_H1.as<cplx>(), hsp, split));
+    }
+    // H20 (calculate_H20 with U~, V~) as G - G^T,
+    //   X1 = h V~ + fd/2 DeltaLR U~,  X2 = -fd/2 DeltaRL V~,  G = U~^T X1 + V~^T X2
+    const cplx hfd = cmk(0.5 * c->factor_delta, 0.0), mhfd = cmk(-0.5 * c->factor_delta, 0.0);
+    auto zg = [&](TgOp oa, const cplx* Aa, cplx al, const cplx* Bb, cplx be, cplx* Cc) -> int {
+      ScopedTiming tz(c, "kernel:zgemm");
+      return gemm_zz(s, oa, OP_N, n, n, n, al, Aa, n, nn, Bb, n, nn, be, Cc, n, nn, na);
+    };
+    if (split) {
+      {
+        ScopedTiming tm(c, "gemm_h20");
+        TG_TRY(zg(OP_N, hsp, one, Vt, zero, X));
+      }
+      TG_TRY(contraction_join(c));
     }
     // particle numbers and energy
     {
@@ -1281,16 +1297,9 @@ int project_run(tg_ctx* c) {
                          P.d_tr + 9 * (size_t)a0);
     }
     TG_HIP(hipGetLastError());
-    // H20 (calculate_H20 with U~, V~) as G - G^T,
-    //   X1 = h V~ + fd/2 DeltaLR U~,  X2 = -fd/2 DeltaRL V~,  G = U~^T X1 + V~^T X2
     {
       ScopedTiming tm(c, "gemm_h20");
-      const cplx hfd = cmk(0.5 * c->factor_delta, 0.0), mhfd = cmk(-0.5 * c->factor_delta, 0.0);
-      auto zg = [&](TgOp oa, const cplx* Aa, cplx al, const cplx* Bb, cplx be, cplx* Cc) -> int {
-        ScopedTiming tz(c, "kernel:zgemm");
-        return gemm_zz(s, oa, OP_N, n, n, n, al, Aa, n, nn, Bb, n, nn, be, Cc, n, nn, na);
-      };
-      TG_TRY(zg(OP_N, hsp, one, Vt, zero, X));
+      if (!split) TG_TRY(zg(OP_N, hsp, one, Vt, zero, X));
       TG_TRY(zg(OP_N, dLR, hfd, Ut, one, X));
       TG_TRY(zg(OP_N, dRL, mhfd, Vt, zero, Y));
       TG_TRY(zg(OP_T, Ut, one, X, zero, Z));
