@@ -5,8 +5,8 @@
 //   * Pfaffian of a complex skew-symmetric matrix by the Parlett-Reid LTL^T
 //     reduction with the column pivoting of ZSKTF2, UPLO='U', MODE='P'
 //     (src/subroutines_pfaffian.f:1694-2015 called by ZSKPFA :177-205).
-// One 256-thread workgroup owns one matrix; the matrix stays in global memory
-// (L2 / MALL resident), the pivot row and column of each step are staged in LDS.
+// One workgroup owns one matrix, which stays in global memory; pivot rows,
+// panel source rows and matrix-core operands are staged in LDS.
 #include "tg_internal.h"
 
 #include <cstdlib>
