@@ -139,13 +139,48 @@ constexpr int GJ_THREADS = 512;
 constexpr int GJ_WIN = 320;   // columns staged per window (panel width x GJ_WIN complex in LDS)
 constexpr int GJ_U = 8;       // columns per step of a thread's streaming update
 
-__device__ inline void wave_argmax(double& v, int& i) {
-  for (int off = 32; off > 0; off >>= 1) {
-    double v2 = __shfl_xor(v, off, 64);
-    int i2 = __shfl_xor(i, off, 64);
-    if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; }
-  }
+// One DPP step of the argmax below: (v, i) of the lane the row-level move CTRL
+// brings in (lanes it brings nothing to keep their own), the larger v and then
+// the smaller i kept.
+template <int CTRL, int RM>
+__device__ __forceinline__ void gj_dpp_argmax_step(double& v, int& i) {
+  const long long b = __double_as_longlong(v);
+  const int lo = (int)(unsigned long long)b, hi = (int)(b >> 32);
+  const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, RM, 0xf, false);
+  const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, RM, 0xf, false);
+  const int i2 = __builtin_amdgcn_update_dpp(i, i, CTRL, RM, 0xf, false);
+  const double v2 =
+      __longlong_as_double((long long)(((unsigned long long)(unsigned)hi2 << 32) | (unsigned)lo2));
+  const bool take = (v2 > v) | ((v2 == v) & (i2 < i));   // no short circuit: no branch
+  v = take ? v2 : v;
+  i = take ? i2 : i;
 }
+
+// argmax (larger v, then smaller i: the first maximum) over lanes 0 .. W-1 of a
+// wave, every lane receiving the result.  Row shifts by 1, 2, 4, 8 leave the
+// maximum of lanes [16 r, l] in lane l of row r; the row broadcasts of lanes 15
+// and 31 then carry rows 0-1 into lane 31 and all four rows into lane 63.
+template <int W>
+__device__ __forceinline__ void gj_wave_argmax_dpp(double& v, int& i) {
+  static_assert(W == 64 || W <= 16, "a whole wave or lanes of its first row");
+  gj_dpp_argmax_step<0x111, 0xf>(v, i);   // row_shr:1
+  if constexpr (W > 2) gj_dpp_argmax_step<0x112, 0xf>(v, i);   // row_shr:2
+  if constexpr (W > 4) gj_dpp_argmax_step<0x114, 0xf>(v, i);   // row_shr:4
+  if constexpr (W > 8) gj_dpp_argmax_step<0x118, 0xf>(v, i);   // row_shr:8
+  constexpr int last = (W == 64) ? 63 : W - 1;
+  if constexpr (W == 64) {
+    gj_dpp_argmax_step<0x142, 0xa>(v, i);   // row_bcast:15 into rows 1 and 3
+    gj_dpp_argmax_step<0x143, 0xc>(v, i);   // row_bcast:31 into rows 2 and 3
+  }
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned long long)b, last);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), last);
+  v = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+  i = __builtin_amdgcn_readlane(i, last);
+}
+
+// argmax over the wave (first index on ties), every lane receiving it
+__device__ inline void wave_argmax(double& v, int& i) { gj_wave_argmax_dpp<64>(v, i); }
 
 // block-wide argmax (first index on ties) for TH threads
 template <int TH>
@@ -202,13 +237,14 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
         wi[buf][tt >> 6] = vi;
       }
       __syncthreads();
-      double mx = wv[buf][0];
-      int p = wi[buf][0];
-      for (int q = 1; q < TH * RP / 64; ++q)
-        if (wv[buf][q] > mx || (wv[buf][q] == mx && wi[buf][q] < p)) {
-          mx = wv[buf][q];
-          p = wi[buf][q];
-        }
+      // lane q of every wave takes wave q's maximum, one more reduction over them
+      double mx = -1.0;
+      int p = n;
+      if ((tt & 63) < TH * RP / 64) {
+        mx = wv[buf][tt & 63];
+        p = wi[buf][tt & 63];
+      }
+      gj_wave_argmax_dpp<TH * RP / 64>(mx, p);
       const bool singular = !(mx > 0.0) || p < k || p >= n;
       if (singular) {
         if (inf == 0) inf = k + 1;
@@ -351,46 +387,6 @@ gj_blocked_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* _
 // those of gj_blocked_kernel (first maximum of |Re| + |Im|, as IZAMAX in
 // ZGETRF).
 // ---------------------------------------------------------------------------
-// One DPP step of the argmax below: (v, i) of the lane the row-level move CTRL
-// brings in (lanes it brings nothing to keep their own), the larger v and then
-// the smaller i kept.
-template <int CTRL, int RM>
-__device__ __forceinline__ void gj_dpp_argmax_step(double& v, int& i) {
-  const long long b = __double_as_longlong(v);
-  const int lo = (int)(unsigned long long)b, hi = (int)(b >> 32);
-  const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, RM, 0xf, false);
-  const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, RM, 0xf, false);
-  const int i2 = __builtin_amdgcn_update_dpp(i, i, CTRL, RM, 0xf, false);
-  const double v2 =
-      __longlong_as_double((long long)(((unsigned long long)(unsigned)hi2 << 32) | (unsigned)lo2));
-  const bool take = (v2 > v) | ((v2 == v) & (i2 < i));   // no short circuit: no branch
-  v = take ? v2 : v;
-  i = take ? i2 : i;
-}
-
-// argmax (larger v, then smaller i: the first maximum) over lanes 0 .. W-1 of a
-// wave, every lane receiving the result.  Row shifts by 1, 2, 4, 8 leave the
-// maximum of lanes [16 r, l] in lane l of row r; the row broadcasts of lanes 15
-// and 31 then carry rows 0-1 into lane 31 and all four rows into lane 63.
-template <int W>
-__device__ __forceinline__ void gj_wave_argmax_dpp(double& v, int& i) {
-  static_assert(W == 64 || W <= 16, "a whole wave or lanes of its first row");
-  gj_dpp_argmax_step<0x111, 0xf>(v, i);   // row_shr:1
-  if constexpr (W > 2) gj_dpp_argmax_step<0x112, 0xf>(v, i);   // row_shr:2
-  if constexpr (W > 4) gj_dpp_argmax_step<0x114, 0xf>(v, i);   // row_shr:4
-  if constexpr (W > 8) gj_dpp_argmax_step<0x118, 0xf>(v, i);   // row_shr:8
-  constexpr int last = (W == 64) ? 63 : W - 1;
-  if constexpr (W == 64) {
-    gj_dpp_argmax_step<0x142, 0xa>(v, i);   // row_bcast:15 into rows 1 and 3
-    gj_dpp_argmax_step<0x143, 0xc>(v, i);   // row_bcast:31 into rows 2 and 3
-  }
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(unsigned long long)b, last);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), last);
-  v = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-  i = __builtin_amdgcn_readlane(i, last);
-}
-
 template <int TH>
 __device__ __forceinline__ void gj_reduce16(cplx (&pr)[16], int k0, int w, int n, int t,
                                             cplx (*sw)[2][16], double (*wv)[TH / 64],
