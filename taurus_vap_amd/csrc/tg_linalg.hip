@@ -657,13 +657,18 @@ gj_wide_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* __re
         }
 #pragma unroll 1
         for (int jt = 0; jt < cw; jt += 16) {
-          gj_d4 dr[GJ_MFQ], di[GJ_MFQ];
+          // three real products per complex one (Gauss): t1 = C_re + sum Pr Wr,
+          // t2 = sum Pi Wi, t3 = C_re + C_im + sum (Pr + Pi)(Wr + Wi); then
+          // C = (t1 - t2, t3 - t1 - t2)
+          gj_d4 t1[GJ_MFQ], t2[GJ_MFQ], t3[GJ_MFQ];
 #pragma unroll
           for (int q = 0; q < GJ_MFQ; ++q)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              dr[q][r] = ld[q] ? nx[q][r].x : 0.0;
-              di[q][r] = ld[q] ? nx[q][r].y : 0.0;
+              const double cr = ld[q] ? nx[q][r].x : 0.0, ci = ld[q] ? nx[q][r].y : 0.0;
+              t1[q][r] = cr;
+              t2[q][r] = 0.0;
+              t3[q][r] = cr + ci;
             }
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
@@ -674,13 +679,20 @@ gj_wide_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* __re
 #pragma unroll
           for (int kk = 0; kk < 8; ++kk) {
             const cplx pv = Pw[4 * kk + lq][jt + li];
+            const double ps = pv.x + pv.y;
 #pragma unroll
             for (int q = 0; q < GJ_MFQ; ++q) {
-              dr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv.x, fr[q][kk], dr[q], 0, 0, 0);
-              dr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-pv.y, fi[q][kk], dr[q], 0, 0, 0);
-              di[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv.x, fi[q][kk], di[q], 0, 0, 0);
-              di[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv.y, fr[q][kk], di[q], 0, 0, 0);
+              const double ws = fr[q][kk] + fi[q][kk];
+              t1[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv.x, fr[q][kk], t1[q], 0, 0, 0);
+              t2[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv.y, fi[q][kk], t2[q], 0, 0, 0);
+              t3[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ps, ws, t3[q], 0, 0, 0);
             }
+          }
+          gj_d4 dr[GJ_MFQ], di[GJ_MFQ];
+#pragma unroll
+          for (int q = 0; q < GJ_MFQ; ++q) {
+            dr[q] = t1[q] - t2[q];
+            di[q] = t3[q] - t1[q] - t2[q];
           }
           // the empty asm statements keep the products ahead of the guarded stores
 #pragma unroll
