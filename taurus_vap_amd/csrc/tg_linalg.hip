@@ -1050,7 +1050,7 @@ pfaffian_blocked_kernel(int n, const cplx* __restrict__ Aall, int lda, long long
 //     the stored column of the row that carries the label, plus the pending
 //     terms X_t Y_k - Y_t X_k;
 //   * after the panel the leading block [0, m) receives the whole rank-2*PF_PS
-//     update in 16 x 16 tiles of v_mfma_f64_16x16x4f64 (complex as four real
+//     update in 16 x 16 tiles of v_mfma_f64_16x16x4f64 (complex as three real
 //     products), read through the label permutation from one buffer and
 //     written in label order to the other (ping-pong between the input, which
 //     is destroyed, and the work matrix), so the interchanges cost no pass of
@@ -1217,7 +1217,9 @@ pfaffian_mfma_kernel(int n, cplx* __restrict__ A, int lda, long long sA, cplx* _
             }
           }
         }
-        pf_d4 dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0};
+        // three real products per complex one (Gauss): t1 = sum ar br, t2 = sum ai bi,
+        // t3 = sum (ar + ai)(br + bi); the product is (t1 - t2, t3 - t1 - t2)
+        pf_d4 t1 = {0.0, 0.0, 0.0, 0.0}, t2 = {0.0, 0.0, 0.0, 0.0}, t3 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int cstep = 0; cstep < PFM_PS / 2; ++cstep) {
           const int q = lq + 4 * cstep;          // 0 .. 2*PFM_PS-1
@@ -1235,11 +1237,11 @@ pfaffian_mfma_kernel(int n, cplx* __restrict__ A, int lda, long long sA, cplx* _
             if (xq) { br = Xr[qs * LDP + ri]; bi2 = Xi[qs * LDP + ri]; }
             else    { br = Yr[qs * LDP + ri]; bi2 = Yi[qs * LDP + ri]; }
           }
-          dr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, dr, 0, 0, 0);
-          dr = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi2, dr, 0, 0, 0);
-          di = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi2, di, 0, 0, 0);
-          di = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, di, 0, 0, 0);
+          t1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, t1, 0, 0, 0);
+          t2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi2, t2, 0, 0, 0);
+          t3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, br + bi2, t3, 0, 0, 0);
         }
+        const pf_d4 dr = t1 - t2, di = t3 - t1 - t2;
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           if (wr[r]) {
