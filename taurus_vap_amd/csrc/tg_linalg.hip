@@ -484,7 +484,7 @@ gj_wide_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* __re
                int* __restrict__ info, int* __restrict__ perm_all) {
   constexpr int H = 16;
   __shared__ cplx sw[2][2][H];
-  __shared__ cplx Pw[2 * H][WIN + U];   // U columns of slack: no clamping below
+  __shared__ cplx Pw[2 * H][WIN + U];   // U slack columns: copies of the window's last
   __shared__ cplx tab[2][H][H];
   __shared__ int srcs[2 * H];
   __shared__ double wv[2][TH / 64];
@@ -576,7 +576,7 @@ gj_wide_kernel(int n, cplx* __restrict__ Aall, int lda, long long sA, cplx* __re
     // MF: matrix-core operands of W.  Wave v updates row blocks v + 8 q (q < 3,
     // rows 16 (v + 8 q) + li), so every SIMD holds six of the 24 blocks of
     // n <= 384; fr/fi[q][kk] = W(16 (v + 8 q) + li, 4 kk + lq), moved between
-    // waves through an n x 16 slab of the (idle) Pw area per half
+    // waves through a TH x 16 slab of the (idle) Pw area per half
     const int lane = t & 63, li = lane & 15, lq = lane >> 4;
     const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
     double fr[GJ_MFQ][8], fi[GJ_MFQ][8];
